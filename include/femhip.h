@@ -1,0 +1,169 @@
+/*
+ * femhip -- C ABI of the MI355X assembly library for the scikit-fem cell
+ * assembly path (BilinearForm.assemble / LinearForm.assemble on CellBasis).
+ *
+ * Conventions
+ *   - Every pointer named in a femhip_*_t struct or passed as an argument is
+ *     a device pointer owned by the caller unless the comment says "host".
+ *   - Layouts are the reference's: node coordinates p are [dim][nnodes]
+ *     (row-major, i.e. structure-of-arrays), connectivity t is
+ *     [nverts][nelems], element_dofs is [nbfun][nelems] (row = local index).
+ *   - All floating point work is FP64, all indices are int32 (the row/col
+ *     keys of the symbolic phase are int64 internally).
+ *   - Functions enqueue on the given stream and do not synchronize unless the
+ *     comment says so.  Return value 0 = success, negative = FEMHIP_E*;
+ *     femhip_last_error() then holds a message (thread-local).
+ */
+#ifndef FEMHIP_H
+#define FEMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* femhip_stream_t; /* a hipStream_t */
+
+enum {
+  FEMHIP_OK = 0,
+  FEMHIP_EINVAL = -1,       /* invalid argument (ValueError)            */
+  FEMHIP_EUNSUPPORTED = -2, /* element/form/quadrature not supported     */
+  FEMHIP_EOVERFLOW = -3,    /* int32 index range exceeded                */
+  FEMHIP_EHIP = -4,         /* HIP runtime error                         */
+  FEMHIP_EDEGENERATE = -5,  /* zero Jacobian determinant (isoparametric) */
+  FEMHIP_ESIZE = -6         /* caller buffer or workspace too small      */
+};
+
+enum { FEMHIP_TRI = 0, FEMHIP_TET = 1, FEMHIP_HEX = 2 };
+
+enum {
+  FEMHIP_LAPLACE = 0,       /* dot(grad u, grad v)                      */
+  FEMHIP_MASS = 1,          /* u * v                                    */
+  FEMHIP_KAPPA_LAPLACE = 2, /* dot(w.kappa * grad u, grad v)            */
+  FEMHIP_ELASTICITY = 3,    /* ddot(C(sym_grad u), sym_grad v)          */
+  FEMHIP_UNIT_LOAD = 4      /* v  (linear form)                         */
+};
+
+/* Mesh geometry and quadrature (CellBasis.mapping + quadrature). */
+typedef struct femhip_geom {
+  int32_t kind;         /* FEMHIP_TRI / TET (affine) or HEX (isoparametric) */
+  int32_t dim;          /* 2 or 3                                          */
+  int32_t nverts;       /* rows of t: 3, 4 or 8                            */
+  int32_t nq;           /* quadrature points                               */
+  int64_t nelems;
+  int64_t nnodes;       /* columns of p                                    */
+  const double* p;      /* [dim][nnodes]                                   */
+  const int32_t* t;     /* [nverts][nelems]                                */
+  const double* W;      /* [nq] quadrature weights                         */
+  const double* gdphi;  /* HEX: [8][3][nq] Hex1 reference gradients at the
+                           quadrature points (the mapping's element);
+                           NULL for simplices                              */
+} femhip_geom_t;
+
+/* Finite element space (CellBasis.elem + Dofs). */
+typedef struct femhip_space {
+  int32_t nbs;          /* scalar basis functions per element              */
+  int32_t vector;       /* 1 = ElementVector (dim components)              */
+  int32_t nbfun;        /* nbs * (vector ? dim : 1)                        */
+  int32_t reserved;
+  int64_t ndofs;        /* N                                               */
+  const int32_t* element_dofs; /* [nbfun][nelems]                          */
+  const double* phi;    /* [nbs][nq]      reference values at X            */
+  const double* dphi;   /* [nbs][dim][nq] reference gradients at X         */
+} femhip_space_t;
+
+/* Built-in form and its parameters (skfem.models). */
+typedef struct femhip_form {
+  int32_t kind;         /* FEMHIP_LAPLACE ...                              */
+  int32_t kappa_mode;   /* 0: none, 1: kappa[e], 2: kappa[e*nq + q]        */
+  const double* kappa;
+  double lam;           /* linear_elasticity Lambda                        */
+  double two_mu;        /* 2. * Mu, as the reference evaluates it          */
+} femhip_form_t;
+
+const char* femhip_last_error(void);
+int femhip_version(void);
+
+/* ---------------------------------------------------------------------
+ * Mapping.  Replaces MappingAffine._init_Ab/_init_invA/detDF/invDF
+ * (skfem/mapping/mapping_affine.py:55-131,205-232) and
+ * MappingIsoparametric._J/detDF/invDF
+ * (skfem/mapping/mapping_isoparametric.py:112-226), evaluated in the
+ * reference's operation order.  Output G is
+ *   affine: [D*D+1][nelems]        (invA row-major, then |detA|)
+ *   hex:    [10][nelems*nq]        (invDF row-major, then |detDF|)
+ * femhip_geometry_size returns the number of doubles of G.
+ * --------------------------------------------------------------------- */
+int64_t femhip_geometry_size(const femhip_geom_t* g);
+/* status: device int32, set to 1 when an isoparametric detDF is exactly 0
+ * (the reference raises "Zero Jacobian determinant"); may be NULL. */
+int femhip_geometry(const femhip_geom_t* g, double* G, int32_t* status,
+                    femhip_stream_t s);
+
+/* ---------------------------------------------------------------------
+ * Local element matrices / vectors in the reference's COO data order.
+ * Replaces BilinearForm._assemble + _kernel
+ * (skfem/assembly/form/bilinear_form.py:58-128,150-151) and
+ * LinearForm._assemble (skfem/assembly/form/linear_form.py:18-49), with
+ * the CellBasis precompute (cell_basis.py:101-105, element_h1.py:10-24,
+ * element_vector.py:36-48) fused in.  Values are bit-identical to the
+ * reference's NumPy evaluation when phi/dphi/W are (no FMA contraction,
+ * einsum order, NumPy pairwise quadrature sum).
+ *   bilinear: contrib[(j*nbfun + i)*nelems + e], j trial, i test
+ *   linear:   contrib[i*nelems + e]
+ * --------------------------------------------------------------------- */
+int femhip_local_bilinear(const femhip_geom_t* g, const femhip_space_t* sp,
+                          const femhip_form_t* f, const double* G,
+                          double* contrib, femhip_stream_t s);
+int femhip_local_linear(const femhip_geom_t* g, const femhip_space_t* sp,
+                        const femhip_form_t* f, const double* G,
+                        double* contrib, femhip_stream_t s);
+
+/* ---------------------------------------------------------------------
+ * Symbolic phase.  Replaces coo_matrix -> eliminate_zeros -> tocsr
+ * (skfem/assembly/form/coo_data.py:28-36) for the pattern, and
+ * COOData.toarray (coo_data.py:102-108) for the load-vector indexing.
+ *
+ * femhip_pattern_build: contributions k = (j*nb_col... ) as above; a
+ * contribution is dropped when contrib[k] == 0.0 (eliminate_zeros runs
+ * before duplicates are summed).  contrib == NULL keeps all of them.
+ * Outputs: indptr[nrows+1], indices[cap ncontrib], perm[cap ncontrib]
+ * (contribution ids sorted by (row, col), stable), seg[cap ncontrib+1]
+ * (perm range of every CSR slot), *nnz_out (host; this call synchronizes
+ * the stream).  Workspace size from femhip_pattern_workspace.
+ *
+ * femhip_rows_plan_build: the same for a vector (rows only, no pruning):
+ * perm[ncontrib], seg[nrows+1].
+ * --------------------------------------------------------------------- */
+int femhip_pattern_workspace(int64_t ncontrib, size_t* bytes);
+int femhip_pattern_build(const int32_t* row_dofs, const int32_t* col_dofs,
+                         int32_t nb_row, int32_t nb_col, int64_t nelems,
+                         int64_t nrows, int64_t ncols, const double* contrib,
+                         int32_t* indptr, int32_t* indices, int32_t* perm,
+                         int32_t* seg, int64_t* nnz_out, void* workspace,
+                         size_t ws_bytes, femhip_stream_t s);
+int femhip_rows_plan_build(const int32_t* row_dofs, int32_t nb_row,
+                           int64_t nelems, int64_t nrows, int32_t* perm,
+                           int32_t* seg, void* workspace, size_t ws_bytes,
+                           femhip_stream_t s);
+
+/* Signature of the exact-zero positions of contrib[n]: out[0] = number of
+ * entries equal to 0.0, out[1] = wrapping sum of a 64-bit mix of their
+ * indices (device int64[2]).  A plan built from one contrib array is valid
+ * for another one iff the signatures agree (up to hash collisions). */
+int femhip_zero_signature(const double* contrib, int64_t n, int64_t* out,
+                          femhip_stream_t s);
+
+/* Numeric phase of the generic path: out[k] = sum of contrib[perm[j]] for
+ * j in [seg[k], seg[k+1]), summed in perm order (deterministic). */
+int femhip_segment_sum(const double* contrib, const int32_t* perm,
+                       const int32_t* seg, int64_t nseg, double* out,
+                       femhip_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEMHIP_H */

@@ -1,0 +1,906 @@
+"""NumPy restatement of scikit-fem's cell-assembly path (the oracle).
+
+TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.
+
+The restatement keeps the reference's floating-point operation order on
+purpose: SciPy's ``eliminate_zeros`` runs before duplicates are summed
+(``skfem/assembly/form/coo_data.py:34-36``), so the CSR pattern depends on
+which local entries come out as exactly 0.0, and that depends on rounding.
+Measured in this image (numpy 2.2.6): ``np.einsum`` over a summed index
+accumulates sequentially from 0 with separate multiply and add (no FMA), and
+``np.sum(a, axis=1)`` on a C-contiguous ``(E, Q)`` array is
+``0 + pairwise(row)`` with NumPy's 8-accumulator blocking.  The restatement
+below uses the same NumPy calls in the same order, so it inherits that
+arithmetic.
+
+How each part is pinned (tests/test_oracle_*.py):
+
+* quadrature -- exactness on monomials up to the stated degree; weights sum
+  to the reference volume; the literal tables are the published rules.
+* lbasis -- nodality, partition of unity, FD derivative checks; Q2 equals
+  the tensor product of 1-D quadratic Lagrange polynomials (unique nodal
+  basis of Q2 on the reference's node order).
+* mappings -- ``A @ invA = I``; constant Jacobian on an undistorted hex.
+* local matrices -- closed forms (P1 Laplace ``|detA|/d! G G^T``, P1 mass).
+* global CSR and load vector -- golden fixtures written by
+  ``tests/golden/make_fixtures.py`` from the reference itself; the reference
+  doctest 4x4 mass matrix; ex01's ``max(x) = 0.073657185490792``.
+* Q2 hex basis values are NOT bit-identical to the reference's generated
+  polynomial table (``element_hex2.py:11-1210``); values agree to rounding and
+  the pattern parity for Q2 is pinned only by the committed fixtures.
+"""
+from __future__ import annotations
+
+import numpy as np
+from numpy.polynomial.legendre import leggauss
+from scipy.sparse import coo_matrix
+
+# ---------------------------------------------------------------------------
+# reference domains (skfem/refdom.py:55-118,171-215)
+# ---------------------------------------------------------------------------
+TRI_FACETS = [[0, 1], [1, 2], [0, 2]]                      # refdom.py:63
+TET_FACETS = [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]  # refdom.py:95
+TET_EDGES = [[0, 1], [1, 2], [0, 2], [0, 3], [1, 3], [2, 3]]  # refdom.py:101
+HEX_FACETS = [[0, 1, 4, 2], [0, 2, 6, 3], [0, 3, 5, 1],
+              [2, 4, 7, 6], [1, 5, 7, 4], [3, 6, 7, 5]]    # refdom.py:188
+HEX_EDGES = [[0, 1], [0, 2], [0, 3], [1, 4], [1, 5], [2, 4],
+             [2, 6], [3, 5], [3, 6], [4, 7], [5, 7], [6, 7]]  # refdom.py:194
+
+
+# ---------------------------------------------------------------------------
+# quadrature (skfem/quadrature.py)
+# ---------------------------------------------------------------------------
+def quadrature_tri(norder):
+    """Tabulated triangle rules, ``quadrature.py:847-905`` (orders 1..4)."""
+    if norder <= 1:
+        norder = 2
+    s = 1.666666666666666666666e-01
+    b = 6.666666666666666666666e-01
+    rules = {
+        2: (np.array([[s, b, s], [s, s, b]]), np.array([s, s, s])),
+        3: (np.array([[0.333333333333333, 0.2, 0.6, 0.2],
+                      [0.333333333333333, 0.6, 0.2, 0.2]]),
+            np.array([-0.28125, 0.260416666666667, 0.260416666666667,
+                      0.260416666666667])),
+        4: (np.array([[0.445948490915965, 0.0915762135097699,
+                       0.10810301816807, 0.816847572980459,
+                       0.445948490915965, 0.091576213509771],
+                      [0.10810301816807, 0.816847572980459,
+                       0.445948490915965, 0.091576213509771,
+                       0.445948490915965, 0.0915762135097699]]),
+            np.array([0.111690794839006, 0.054975871827661,
+                      0.111690794839006, 0.054975871827661,
+                      0.111690794839006, 0.054975871827661])),
+    }
+    if norder not in rules:
+        raise NotImplementedError("tri quadrature order %d" % norder)
+    return rules[norder]
+
+
+def quadrature_tet(norder):
+    """Tabulated tetrahedron rules, ``quadrature.py:80-200`` (orders 1..4)."""
+    if norder < 1:
+        norder = 1
+    a, c = 0.5854101966249685, 0.1381966011250105
+    k = 0.1666666666666667
+    p, q = 0.7857142857142857, 0.0714285714285714
+    r, t = 0.1005964238332008, 0.3994035761667992
+    rules = {
+        1: (np.array([[.25], [.25], [.25]]), np.array([1 / 6])),
+        2: (np.array([[a, c, c, c], [c, c, c, a], [c, c, a, c]]),
+            np.array([0.25, 0.25, 0.25, 0.25]) / 6.0),
+        3: (np.array([[0.25, 0.5, k, k, k], [0.25, k, k, k, 0.5],
+                      [0.25, k, k, 0.5, k]]),
+            np.array([-0.8, 0.45, 0.45, 0.45, 0.45]) / 6.0),
+        4: (np.array([[0.25, p, q, q, q, r, t, t, t, r, r],
+                      [0.25, q, q, q, p, t, r, t, r, t, r],
+                      [0.25, q, q, p, q, t, t, r, r, r, t]]),
+            np.array([-0.0789333333333333, 0.0457333333333333,
+                      0.0457333333333333, 0.0457333333333333,
+                      0.0457333333333333, 0.1493333333333333,
+                      0.1493333333333333, 0.1493333333333333,
+                      0.1493333333333333, 0.1493333333333333,
+                      0.1493333333333333]) / 6.0),
+    }
+    if norder not in rules:
+        raise NotImplementedError("tet quadrature order %d" % norder)
+    return rules[norder]
+
+
+def quadrature_line(norder):
+    """Gauss-Legendre on [0,1], ``quadrature.py:2839-2844``."""
+    if norder <= 1:
+        norder = 2
+    X, W = leggauss(int(np.ceil((norder + 1.0) / 2.0)))
+    return np.array([0.5 * X + 0.5]), W / 2.0
+
+
+def quadrature_hex(norder):
+    """Tensor product of line rules in Fortran order, ``quadrature.py:64-74``."""
+    X, W = quadrature_line(norder)
+    A, B, C = np.meshgrid(X, X, X)
+    Y = np.vstack((A.flatten(order="F"), B.flatten(order="F"),
+                   C.flatten(order="F")))
+    A, B, C = np.meshgrid(W, W, W)
+    Z = A * B * C
+    return Y, Z.flatten(order="F")
+
+
+# ---------------------------------------------------------------------------
+# meshes (skfem/mesh/*.py)
+# ---------------------------------------------------------------------------
+def build_entities(t, indices, sort=True):
+    """``Mesh.build_entities``, ``mesh.py:1066-1082``."""
+    indexing = np.hstack(tuple([t[ix] for ix in indices]))
+    sorted_indexing = np.sort(indexing, axis=0)
+    sorted_indexing, ixa, ixb = np.unique(sorted_indexing, axis=1,
+                                          return_index=True,
+                                          return_inverse=True)
+    mapping = ixb.reshape((len(indices), t.shape[1]))
+    if sort:
+        return np.ascontiguousarray(sorted_indexing), mapping
+    return np.ascontiguousarray(indexing[:, ixa]), mapping
+
+
+def build_inverse(t, mapping):
+    """``Mesh.build_inverse``, ``mesh.py:1085-1100``."""
+    e = mapping.flatten(order='C')
+    tix = np.tile(np.arange(t.shape[1]), (1, t.shape[0]))[0]
+    e_first, ix_first = np.unique(e, return_index=True)
+    e_last, ix_last = np.unique(e[::-1], return_index=True)
+    ix_last = e.shape[0] - ix_last - 1
+    inverse = np.zeros((2, np.max(mapping) + 1), dtype=np.int32)
+    inverse[0, e_first] = tix[ix_first]
+    inverse[1, e_last] = tix[ix_last]
+    inverse[1, np.nonzero(inverse[0] == inverse[1])[0]] = -1
+    return inverse
+
+
+class Mesh:
+    """Minimal ``Mesh`` (``mesh.py:36-135,544-606``)."""
+
+    kind = None
+    dim = None
+    facet_idx = None
+    edge_idx = None
+    sort_facets = True
+
+    def __init__(self, p, t, sort_t=False):
+        if sort_t:
+            t = np.sort(t, axis=0)          # mesh.py:555-556
+        self.p = np.ascontiguousarray(np.asarray(p, dtype=np.float64))
+        self.t = np.ascontiguousarray(np.asarray(t, dtype=np.int32))
+
+    @property
+    def nelements(self):
+        return self.t.shape[1]
+
+    @property
+    def nvertices(self):
+        return int(np.max(self.t)) + 1
+
+    def _facets(self):
+        if not hasattr(self, '_facets_cache'):
+            self._facets_cache = build_entities(self.t, self.facet_idx,
+                                                sort=self.sort_facets)
+        return self._facets_cache
+
+    @property
+    def facets(self):
+        return self._facets()[0]
+
+    @property
+    def t2f(self):
+        return self._facets()[1]
+
+    @property
+    def f2t(self):
+        if not hasattr(self, '_f2t'):
+            self._f2t = build_inverse(self.t, self.t2f)
+        return self._f2t
+
+    def _edges(self):
+        if not hasattr(self, '_edges_cache'):
+            self._edges_cache = build_entities(self.t, self.edge_idx)
+        return self._edges_cache
+
+    @property
+    def edges(self):
+        return self._edges()[0]
+
+    @property
+    def t2e(self):
+        return self._edges()[1]
+
+    def boundary_facets(self):
+        """``mesh.py:197-199``."""
+        return np.nonzero(self.f2t[1] == -1)[0].astype(np.int32)
+
+    def boundary_nodes(self):
+        """``mesh.py:398-400``."""
+        return np.unique(self.facets[:, self.boundary_facets()])
+
+    def interior_nodes(self):
+        """``mesh.py:402-405``."""
+        return np.setdiff1d(np.arange(0, self.p.shape[1]),
+                            self.boundary_nodes())
+
+
+class MeshTri(Mesh):
+    """``MeshTri1`` (``mesh_tri_1.py:13-79``; ``sort_t=True`` at :28)."""
+
+    kind, dim = 'tri', 2
+    facet_idx = TRI_FACETS
+    affine = True
+
+    def __init__(self, p=None, t=None):
+        if p is None:
+            p = np.array([[0., 0.], [1., 0.], [0., 1.], [1., 1.]]).T
+            t = np.array([[0, 1, 2], [1, 3, 2]], dtype=np.int32).T
+        super().__init__(p, t, sort_t=True)
+
+    @classmethod
+    def init_tensor(cls, x, y):
+        """``mesh_tri_1.py:31-79``."""
+        npx, npy = len(x), len(y)
+        X, Y = np.meshgrid(np.sort(x), np.sort(y))
+        p = np.vstack((X.flatten('F'), Y.flatten('F')))
+        ix = np.arange(npx * npy)
+        nt = (npx - 1) * (npy - 1)
+        t = np.zeros((3, 2 * nt))
+        ix = ix.reshape(npy, npx, order='F').copy()
+        t[0, :nt] = ix[0:npy - 1, 0:npx - 1].reshape(nt, 1, order='F').flatten()
+        t[1, :nt] = ix[1:npy, 0:npx - 1].reshape(nt, 1, order='F').flatten()
+        t[2, :nt] = ix[1:npy, 1:npx].reshape(nt, 1, order='F').flatten()
+        t[0, nt:] = ix[0:npy - 1, 0:npx - 1].reshape(nt, 1, order='F').flatten()
+        t[1, nt:] = ix[0:npy - 1, 1:npx].reshape(nt, 1, order='F').flatten()
+        t[2, nt:] = ix[1:npy, 1:npx].reshape(nt, 1, order='F').flatten()
+        return cls(p, t.astype(np.int32))
+
+
+def _grid3(x, y, z):
+    """Shared part of ``MeshTet1.init_tensor`` (``mesh_tet_1.py:343-381``)
+    and ``MeshHex1.init_tensor`` (``mesh_hex_1.py:114-154``)."""
+    npx, npy, npz = len(x), len(y), len(z)
+    X, Y, Z = np.meshgrid(np.sort(x), np.sort(y), np.sort(z))
+    p = np.vstack((X.flatten('F'), Y.flatten('F'), Z.flatten('F')))
+    ix = np.arange(npx * npy * npz)
+    ne = (npx - 1) * (npy - 1) * (npz - 1)
+    t = np.zeros((8, ne))
+    ix = ix.reshape(npy, npx, npz, order='F').copy()
+    sl = [(slice(0, npy - 1), slice(0, npx - 1), slice(0, npz - 1)),
+          (slice(1, npy), slice(0, npx - 1), slice(0, npz - 1)),
+          (slice(0, npy - 1), slice(1, npx), slice(0, npz - 1)),
+          (slice(0, npy - 1), slice(0, npx - 1), slice(1, npz)),
+          (slice(1, npy), slice(1, npx), slice(0, npz - 1)),
+          (slice(1, npy), slice(0, npx - 1), slice(1, npz)),
+          (slice(0, npy - 1), slice(1, npx), slice(1, npz)),
+          (slice(1, npy), slice(1, npx), slice(1, npz))]
+    for k, s in enumerate(sl):
+        t[k] = ix[s].reshape(ne, 1, order='F').flatten()
+    return p, t, ne
+
+
+class MeshTet(Mesh):
+    """``MeshTet1`` (``mesh_tet_1.py:13-45``, not sorted)."""
+
+    kind, dim = 'tet', 3
+    facet_idx = TET_FACETS
+    edge_idx = TET_EDGES
+    affine = True
+
+    def __init__(self, p=None, t=None):
+        if p is None:
+            p = np.array([[0., 0., 0.], [0., 0., 1.], [0., 1., 0.],
+                          [1., 0., 0.], [0., 1., 1.], [1., 0., 1.],
+                          [1., 1., 0.], [1., 1., 1.]]).T
+            t = np.array([[0, 1, 2, 3], [3, 5, 1, 7], [2, 3, 6, 7],
+                          [2, 3, 1, 7], [1, 2, 4, 7]], dtype=np.int32).T
+        super().__init__(p, t)
+
+    @classmethod
+    def init_tensor(cls, x, y, z):
+        """Kuhn split into six tets, ``mesh_tet_1.py:327-393``."""
+        p, t, ne = _grid3(x, y, z)
+        T = np.zeros((4, 6 * ne))
+        T[:, :ne] = t[[0, 1, 5, 7]]
+        T[:, ne:(2 * ne)] = t[[0, 1, 4, 7]]
+        T[:, (2 * ne):(3 * ne)] = t[[0, 2, 4, 7]]
+        T[:, (3 * ne):(4 * ne)] = t[[0, 3, 5, 7]]
+        T[:, (4 * ne):(5 * ne)] = t[[0, 2, 6, 7]]
+        T[:, (5 * ne):] = t[[0, 3, 6, 7]]
+        return cls(p, T.astype(np.int32))
+
+
+class MeshHex(Mesh):
+    """``MeshHex1`` (``mesh_hex_1.py:12-56``; facets unsorted at :49-55)."""
+
+    kind, dim = 'hex', 3
+    facet_idx = HEX_FACETS
+    edge_idx = HEX_EDGES
+    sort_facets = False
+    affine = False
+
+    def __init__(self, p=None, t=None):
+        if p is None:
+            p = np.array([[0., 0., 0.], [0., 0., 1.], [0., 1., 0.],
+                          [1., 0., 0.], [0., 1., 1.], [1., 0., 1.],
+                          [1., 1., 0.], [1., 1., 1.]]).T
+            t = np.array([[0, 1, 2, 3, 4, 5, 6, 7]], dtype=np.int32).T
+        super().__init__(p, t)
+
+    @classmethod
+    def init_tensor(cls, x, y, z):
+        """``mesh_hex_1.py:98-155``."""
+        p, t, _ = _grid3(x, y, z)
+        return cls(p, t.astype(np.int32))
+
+
+# ---------------------------------------------------------------------------
+# elements (skfem/element/*)
+# ---------------------------------------------------------------------------
+class Element:
+    nodal_dofs = 0
+    edge_dofs = 0
+    facet_dofs = 0
+    interior_dofs = 0
+    maxdeg = -1
+    kind = None
+    dim = None
+    vector = False
+
+    def lbasis(self, X, i):
+        raise NotImplementedError
+
+    def gbasis(self, mapping, X, i):
+        """``ElementH1.gbasis``, ``element_h1.py:10-24``."""
+        phi, dphi = self.lbasis(X, i)
+        invDF = mapping.invDF(X)
+        return (np.broadcast_to(phi, (invDF.shape[2], invDF.shape[3])),
+                np.einsum('ijkl,il->jkl', invDF, dphi))
+
+
+class ElementTriP1(Element):
+    """``element_tri_p1.py:7-33``."""
+    nodal_dofs, maxdeg, kind, dim = 1, 1, 'tri', 2
+    nbfun = 3
+
+    def lbasis(self, X, i):
+        x, y = X
+        if i == 0:
+            return 1. - x - y, np.array([-1. + 0. * x, -1. + 0. * x])
+        if i == 1:
+            return x, np.array([1. + 0. * x, 0. * x])
+        if i == 2:
+            return y, np.array([0. * x, 1. + 0. * x])
+        raise ValueError(i)
+
+
+class ElementTriP2(Element):
+    """``element_tri_p2.py:7-48``."""
+    nodal_dofs, facet_dofs, maxdeg, kind, dim = 1, 1, 2, 'tri', 2
+    nbfun = 6
+
+    def lbasis(self, X, i):
+        x, y = X
+        if i == 0:
+            return (1. - 3. * x - 3. * y + 2. * x ** 2 + 4. * x * y
+                    + 2. * y ** 2,
+                    np.array([-3 + 4. * x + 4. * y, -3 + 4. * x + 4. * y]))
+        if i == 1:
+            return 2. * x ** 2 - x, np.array([4. * x - 1, 0. * x])
+        if i == 2:
+            return 2. * y ** 2 - y, np.array([0. * x, 4. * y - 1])
+        if i == 3:
+            return (4. * x - 4. * x ** 2 - 4. * x * y,
+                    np.array([4 - 8. * x - 4. * y, -4. * x]))
+        if i == 4:
+            return 4. * x * y, np.array([4. * y, 4. * x])
+        if i == 5:
+            return (4. * y - 4. * x * y - 4. * y ** 2,
+                    np.array([-4. * y, 4 - 4. * x - 8. * y]))
+        raise ValueError(i)
+
+
+class ElementTetP1(Element):
+    """``element_tet_p1.py:7-45``."""
+    nodal_dofs, maxdeg, kind, dim = 1, 1, 'tet', 3
+    nbfun = 4
+
+    def lbasis(self, X, i):
+        x, y, z = X
+        if i == 0:
+            return 1 - x - y - z, np.array([-1 + 0 * x, -1 + 0 * x,
+                                            -1 + 0 * x])
+        if i == 1:
+            return x, np.array([1 + 0 * x, 0 * x, 0 * x])
+        if i == 2:
+            return y, np.array([0 * x, 1 + 0 * x, 0 * x])
+        if i == 3:
+            return z, np.array([0 * x, 0 * x, 1 + 0 * x])
+        raise ValueError(i)
+
+
+class ElementTetP2(Element):
+    """``element_tet_p2.py:7-103``."""
+    nodal_dofs, edge_dofs, maxdeg, kind, dim = 1, 1, 2, 'tet', 3
+    nbfun = 10
+
+    def lbasis(self, X, i):
+        x, y, z = X
+        if i == 0:
+            g = -3. + 4. * x + 4. * y + 4. * z
+            return (1. - 3. * x + 2. * x ** 2 - 3. * y + 4. * x * y
+                    + 2. * y ** 2 - 3. * z + 4. * x * z + 4. * y * z
+                    + 2. * z ** 2, np.array([g, g, g]))
+        if i == 1:
+            return -1. * x + 2. * x ** 2, np.array([-1 + 4 * x, 0 * x, 0 * x])
+        if i == 2:
+            return -1. * y + 2. * y ** 2, np.array([0 * x, -1. + 4. * y,
+                                                    0 * x])
+        if i == 3:
+            return -1. * z + 2. * z ** 2, np.array([0 * x, 0 * x,
+                                                    -1. + 4. * z])
+        if i == 4:
+            return (4. * x - 4. * x ** 2 - 4. * x * y - 4 * x * z,
+                    np.array([4. - 8. * x - 4. * y - 4. * z, -4. * x,
+                              -4. * x]))
+        if i == 5:
+            return 4. * x * y, np.array([4. * y, 4. * x, 0 * x])
+        if i == 6:
+            return (0. + 4. * y - 4. * x * y - 4. * y ** 2 - 4. * y * z,
+                    np.array([-4. * y, 4. - 4. * x - 8. * y - 4. * z,
+                              -4. * y]))
+        if i == 7:
+            return (0. + 4. * z - 4. * x * z - 4. * y * z - 4. * z ** 2,
+                    np.array([-4. * z, -4. * z,
+                              4. - 4. * x - 4. * y - 8. * z]))
+        if i == 8:
+            return 0. + 4. * x * z, np.array([4. * z, 0 * x, 4 * x])
+        if i == 9:
+            return 0. + 4. * y * z, np.array([0 * x, 4 * z, 4 * y])
+        raise ValueError(i)
+
+
+class ElementHex1(Element):
+    """``element_hex1.py:7-69`` (node 0 is reference (1,1,1))."""
+    nodal_dofs, maxdeg, kind, dim = 1, 3, 'hex', 3
+    nbfun = 8
+
+    def lbasis(self, X, i):
+        x, y, z = X
+        if i == 0:
+            return x * y * z, np.array([y * z, x * z, x * y])
+        if i == 1:
+            return (x * y * (1 - z),
+                    np.array([y * (1 - z), x * (1 - z), -x * y]))
+        if i == 2:
+            return (x * (1 - y) * z,
+                    np.array([(1 - y) * z, -x * z, x * (1 - y)]))
+        if i == 3:
+            return ((1 - x) * y * z,
+                    np.array([-y * z, (1 - x) * z, (1 - x) * y]))
+        if i == 4:
+            return (x * (1 - y) * (1 - z),
+                    np.array([(1 - y) * (1 - z), -x * (1 - z),
+                              -x * (1 - y)]))
+        if i == 5:
+            return ((1 - x) * y * (1 - z),
+                    np.array([-y * (1 - z), (1 - x) * (1 - z),
+                              -(1 - x) * y]))
+        if i == 6:
+            return ((1 - x) * (1 - y) * z,
+                    np.array([-(1 - y) * z, -(1 - x) * z,
+                              (1 - x) * (1 - y)]))
+        if i == 7:
+            return ((1 - x) * (1 - y) * (1 - z),
+                    np.array([-(1 - y) * (1 - z), -(1 - x) * (1 - z),
+                              -(1 - x) * (1 - y)]))
+        raise ValueError(i)
+
+
+# Q2 node locations in the reference order (element_hex2.py:1222-1252).
+HEX2_DOFLOCS = np.array([
+    [1., 1., 1.], [1., 1., 0.], [1., 0., 1.], [0., 1., 1.], [1., 0., 0.],
+    [0., 1., 0.], [0., 0., 1.], [0., 0., 0.],
+    [1., 1., .5], [1., .5, 1.], [.5, 1., 1.], [1., .5, 0.], [.5, 1., 0.],
+    [1., 0., .5], [.5, 0., 1.], [0., 1., .5], [0., .5, 1.], [.5, 0., 0.],
+    [0., .5, 0.], [0., 0., .5],
+    [1., .5, .5], [.5, .5, 1.], [.5, 1., .5], [.5, 0., .5], [.5, .5, 0.],
+    [0., .5, .5],
+    [.5, .5, .5]])
+
+
+def _lagrange2(s, node):
+    """1-D quadratic Lagrange polynomial on nodes {0, 1/2, 1} and its
+    derivative."""
+    if node == 0.0:
+        return (1. - s) * (1. - 2. * s), 4. * s - 3.
+    if node == 0.5:
+        return 4. * s * (1. - s), 4. - 8. * s
+    return s * (2. * s - 1.), 4. * s - 1.
+
+
+class ElementHex2(Element):
+    """Triquadratic element (``element_hex2.py:1213-1260``), written as the
+    tensor product of 1-D quadratic Lagrange polynomials in the reference's
+    node order instead of the reference's generated polynomial table."""
+    nodal_dofs = edge_dofs = facet_dofs = interior_dofs = 1
+    maxdeg, kind, dim = 6, 'hex', 3
+    nbfun = 27
+
+    def lbasis(self, X, i):
+        x, y, z = X
+        ax, bx, cx = HEX2_DOFLOCS[i]
+        lx, dx = _lagrange2(x, ax)
+        ly, dy = _lagrange2(y, bx)
+        lz, dz = _lagrange2(z, cx)
+        return lx * ly * lz, np.array([dx * ly * lz, lx * dy * lz,
+                                       lx * ly * dz])
+
+
+class ElementVector(Element):
+    """``element_vector.py:6-48``."""
+    vector = True
+
+    def __init__(self, elem):
+        self.elem = elem
+        self.dim = elem.dim
+        self.kind = elem.kind
+        self.maxdeg = elem.maxdeg
+        self.nodal_dofs = elem.nodal_dofs * self.dim
+        self.facet_dofs = elem.facet_dofs * self.dim
+        self.interior_dofs = elem.interior_dofs * self.dim
+        self.edge_dofs = elem.edge_dofs * self.dim
+        self.nbfun = elem.nbfun * self.dim
+
+    def gbasis(self, mapping, X, i):
+        ind = int(np.floor(float(i) / float(self.dim)))
+        n = i - self.dim * ind
+        value, grad = self.elem.gbasis(mapping, X, ind)
+        out = []
+        for field in (value, grad):
+            tmp = np.zeros((self.dim,) + field.shape)
+            tmp[n] = field
+            out.append(tmp)
+        return tuple(out)
+
+
+# ---------------------------------------------------------------------------
+# mappings (skfem/mapping/*)
+# ---------------------------------------------------------------------------
+class MappingAffine:
+    """``mapping_affine.py:6-232`` (A, b, detA, invA; tiled detDF, invDF)."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        dim = mesh.p.shape[0]
+        p, t = mesh.p, mesh.t
+        nt = t.shape[1]
+        A = np.empty((dim, dim, nt))
+        b = np.empty((dim, nt))
+        for i in range(dim):                       # _init_Ab :55-79
+            b[i] = p[i, t[0]]
+            for j in range(dim):
+                A[i, j] = p[i, t[j + 1]] - p[i, t[0]]
+        if dim == 2:                               # _init_invA :81-131
+            detA = A[0, 0] * A[1, 1] - A[0, 1] * A[1, 0]
+            invA = np.empty((dim, dim, nt))
+            invA[0, 0] = A[1, 1] / detA
+            invA[0, 1] = -A[0, 1] / detA
+            invA[1, 0] = -A[1, 0] / detA
+            invA[1, 1] = A[0, 0] / detA
+        else:
+            detA = (A[0, 0] * (A[1, 1] * A[2, 2] - A[1, 2] * A[2, 1])
+                    - A[0, 1] * (A[1, 0] * A[2, 2] - A[1, 2] * A[2, 0])
+                    + A[0, 2] * (A[1, 0] * A[2, 1] - A[1, 1] * A[2, 0]))
+            invA = np.empty((dim, dim, nt))
+            invA[0, 0] = (-A[1, 2] * A[2, 1] + A[1, 1] * A[2, 2]) / detA
+            invA[1, 0] = (A[1, 2] * A[2, 0] - A[1, 0] * A[2, 2]) / detA
+            invA[2, 0] = (-A[1, 1] * A[2, 0] + A[1, 0] * A[2, 1]) / detA
+            invA[0, 1] = (A[0, 2] * A[2, 1] - A[0, 1] * A[2, 2]) / detA
+            invA[1, 1] = (-A[0, 2] * A[2, 0] + A[0, 0] * A[2, 2]) / detA
+            invA[2, 1] = (A[0, 1] * A[2, 0] - A[0, 0] * A[2, 1]) / detA
+            invA[0, 2] = (-A[0, 2] * A[1, 1] + A[0, 1] * A[1, 2]) / detA
+            invA[1, 2] = (A[0, 2] * A[1, 0] - A[0, 0] * A[1, 2]) / detA
+            invA[2, 2] = (-A[0, 1] * A[1, 0] + A[0, 0] * A[1, 1]) / detA
+        self.A, self.b, self.detA, self.invA = A, b, detA, invA
+
+    def F(self, X):
+        """``mapping_affine.py:183-192``."""
+        return (np.einsum('ijk,jl', self.A, X).T + self.b.T).T
+
+    def detDF(self, X):
+        """``mapping_affine.py:205-211``."""
+        return np.tile(self.detA, (X.shape[-1], 1)).T
+
+    def invDF(self, X):
+        """``mapping_affine.py:225-232``."""
+        return np.einsum('ijk,l->ijkl', self.invA, np.ones(X.shape[-1]))
+
+
+class MappingIsoparametric:
+    """``mapping_isoparametric.py:12-226`` with the mesh's own Hex1."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.elem = ElementHex1()
+        self._cache = {}
+
+    def J(self, i, j, X):
+        """``_J``, ``mapping_isoparametric.py:112-126``."""
+        key = (i, j, X.tobytes())
+        if key not in self._cache:
+            p, t = self.mesh.p, self.mesh.t
+            out = np.zeros((t.shape[1], X.shape[1]))
+            for itr in range(t.shape[0]):
+                _, dphi = self.elem.lbasis(X, itr)
+                out += p[i, t[itr, :]][:, None] * dphi[j]
+            self._cache[key] = out
+        return self._cache[key]
+
+    def F(self, X):
+        p, t = self.mesh.p, self.mesh.t
+        out = []
+        for i in range(3):
+            o = np.zeros((t.shape[1], X.shape[1]))
+            for itr in range(t.shape[0]):
+                phi, _ = self.elem.lbasis(X, itr)
+                o += p[i, t[itr, :]][:, None] * phi
+            out.append(o)
+        return np.array(out)
+
+    def detDF(self, X, J=None):
+        """``mapping_isoparametric.py:179-198``."""
+        if J is None:
+            J = [[self.J(i, j, X) for j in range(3)] for i in range(3)]
+        detDF = (J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                 J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                 J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]))
+        if np.sum(detDF == 0) > 0:
+            raise Exception("Zero Jacobian determinant")
+        return detDF
+
+    def invDF(self, X):
+        """``mapping_isoparametric.py:200-226``."""
+        J = [[self.J(i, j, X) for j in range(3)] for i in range(3)]
+        detDF = self.detDF(X, J=J)
+        invDF = np.empty((3, 3) + J[0][0].shape)
+        invDF[0, 0] = -J[1][2] * J[2][1] + J[1][1] * J[2][2]
+        invDF[1, 0] = J[1][2] * J[2][0] - J[1][0] * J[2][2]
+        invDF[2, 0] = -J[1][1] * J[2][0] + J[1][0] * J[2][1]
+        invDF[0, 1] = J[0][2] * J[2][1] - J[0][1] * J[2][2]
+        invDF[1, 1] = -J[0][2] * J[2][0] + J[0][0] * J[2][2]
+        invDF[2, 1] = J[0][1] * J[2][0] - J[0][0] * J[2][1]
+        invDF[0, 2] = -J[0][2] * J[1][1] + J[0][1] * J[1][2]
+        invDF[1, 2] = J[0][2] * J[1][0] - J[0][0] * J[1][2]
+        invDF[2, 2] = -J[0][1] * J[1][0] + J[0][0] * J[1][1]
+        return invDF / detDF
+
+
+# ---------------------------------------------------------------------------
+# DOF numbering (skfem/assembly/dofs.py:264-334)
+# ---------------------------------------------------------------------------
+def element_dofs(mesh, elem):
+    """``Dofs.__init__``: nodal, edge (3-D), facet, interior blocks."""
+    nv = mesh.nvertices
+    offset = 0
+    nodal = np.reshape(np.arange(elem.nodal_dofs * nv, dtype=np.int32),
+                       (elem.nodal_dofs, nv), order='F') + offset
+    offset += elem.nodal_dofs * nv
+    edge = facet = None
+    if elem.dim == 3 and elem.edge_dofs > 0:
+        ne = mesh.edges.shape[1]
+        edge = np.reshape(np.arange(elem.edge_dofs * ne, dtype=np.int32),
+                          (elem.edge_dofs, ne), order='F') + offset
+        offset += elem.edge_dofs * ne
+    if elem.facet_dofs > 0:
+        nf = mesh.facets.shape[1]
+        facet = np.reshape(np.arange(elem.facet_dofs * nf, dtype=np.int32),
+                           (elem.facet_dofs, nf), order='F') + offset
+        offset += elem.facet_dofs * nf
+    interior = np.reshape(
+        np.arange(elem.interior_dofs * mesh.nelements, dtype=np.int32),
+        (elem.interior_dofs, mesh.nelements), order='F') + offset
+    rows = [np.zeros((0, mesh.nelements), dtype=np.int32)]
+    for itr in range(mesh.t.shape[0]):
+        rows.append(nodal[:, mesh.t[itr]])
+    if edge is not None:
+        for itr in range(mesh.t2e.shape[0]):
+            rows.append(edge[:, mesh.t2e[itr]])
+    if facet is not None:
+        for itr in range(mesh.t2f.shape[0]):
+            rows.append(facet[:, mesh.t2f[itr]])
+    rows.append(interior)
+    ed = np.vstack(rows)
+    return ed, int(np.max(ed)) + 1
+
+
+# ---------------------------------------------------------------------------
+# CellBasis (skfem/assembly/basis/cell_basis.py:42-106, abstract_basis.py)
+# ---------------------------------------------------------------------------
+class CellBasis:
+
+    def __init__(self, mesh, elem, intorder=None):
+        self.mesh, self.elem = mesh, elem
+        self.mapping = (MappingAffine(mesh) if mesh.affine
+                        else MappingIsoparametric(mesh))
+        self.element_dofs, self.N = element_dofs(mesh, elem)
+        self.Nbfun = self.element_dofs.shape[0]
+        self.nelems = mesh.nelements
+        order = intorder if intorder is not None else 2 * elem.maxdeg
+        self.X, self.W = {'tri': quadrature_tri, 'tet': quadrature_tet,
+                          'hex': quadrature_hex}[mesh.kind](order)
+        self.basis = [elem.gbasis(self.mapping, self.X, j)
+                      for j in range(self.Nbfun)]
+        self.dx = (np.abs(self.mapping.detDF(self.X))
+                   * np.broadcast_to(self.W, (self.nelems, self.W.shape[-1])))
+
+
+# ---------------------------------------------------------------------------
+# helpers and models (skfem/helpers.py, skfem/models/*.py)
+# ---------------------------------------------------------------------------
+def dot(u, v):
+    return np.einsum('i...,i...', u, v)          # helpers.py:108-110
+
+
+def ddot(u, v):
+    return np.einsum('ij...,ij...', u, v)        # helpers.py:113-115
+
+
+def trace(T):
+    return np.einsum('ii...', T)                 # helpers.py:137-139
+
+
+def transpose(T):
+    return np.einsum('ij...->ji...', T)          # helpers.py:142-144
+
+
+def eye(w, n):
+    return np.array([[w if i == j else 0. * w for i in range(n)]
+                     for j in range(n)])         # helpers.py:147-150
+
+
+def sym_grad(g):
+    return .5 * (g + transpose(g))               # helpers.py:71-73
+
+
+def lame_parameters(E, nu):
+    """``models/elasticity.py:7-26``."""
+    return (E * nu / ((1. + nu) * (1. - 2. * nu)), E / (2. * (1. + nu)))
+
+
+def form_laplace(u, v, w):
+    return dot(u[1], v[1])                       # models/poisson.py:7-9
+
+
+def form_mass(u, v, w):
+    return u[0] * v[0]                           # models/poisson.py:17-19
+
+
+def form_kappa_laplace(u, v, w):
+    return dot(w['kappa'] * u[1], v[1])
+
+
+def form_elasticity(Lambda, Mu):
+    """``models/elasticity.py:35-53``."""
+    def C(T):
+        return 2. * Mu * T + Lambda * eye(trace(T), T.shape[0])
+
+    def f(u, v, w):
+        return ddot(C(sym_grad(u[1])), sym_grad(v[1]))
+    return f
+
+
+def form_unit_load(v, w):
+    return v[0]                                  # models/poisson.py:22-24
+
+
+# ---------------------------------------------------------------------------
+# assembly (skfem/assembly/form/*.py)
+# ---------------------------------------------------------------------------
+def bilinear_coo(form, basis, **w):
+    """``BilinearForm._assemble``, ``bilinear_form.py:58-128``: the data is
+    laid out ``(j, i, e)`` C-order, rows from the test (i), cols from the
+    trial (j) basis."""
+    nt, nb = basis.nelems, basis.Nbfun
+    data = np.zeros((nb, nb, nt))
+    rows = np.zeros(nb * nb * nt, dtype=np.int32)
+    cols = np.zeros(nb * nb * nt, dtype=np.int32)
+    for j in range(nb):
+        for i in range(nb):
+            ixs = slice(nt * (nb * j + i), nt * (nb * j + i + 1))
+            rows[ixs] = basis.element_dofs[i]
+            cols[ixs] = basis.element_dofs[j]
+            data[j, i, :] = np.sum(form(basis.basis[j], basis.basis[i], w)
+                                   * basis.dx, axis=1)
+    return np.array([rows, cols]), data.flatten('C'), (basis.N, basis.N)
+
+
+def coo_to_csr(indices, data, shape):
+    """``COOData._assemble_scipy_csr``, ``coo_data.py:28-36``."""
+    K = coo_matrix((data, (indices[0], indices[1])), shape=shape)
+    K.eliminate_zeros()
+    return K.tocsr()
+
+
+def bilinear_assemble(form, basis, **w):
+    """``BilinearForm.assemble``, ``bilinear_form.py:130-148``."""
+    return coo_to_csr(*bilinear_coo(form, basis, **w))
+
+
+def linear_assemble(form, basis, **w):
+    """``LinearForm._assemble`` (``linear_form.py:18-46``) followed by
+    ``COOData.toarray`` (``coo_data.py:102-108``)."""
+    nt, nb = basis.nelems, basis.Nbfun
+    data = np.zeros(nb * nt)
+    rows = np.zeros(nb * nt, dtype=np.int32)
+    for i in range(nb):
+        ixs = slice(nt * i, nt * (i + 1))
+        rows[ixs] = basis.element_dofs[i]
+        data[ixs] = np.sum(form(basis.basis[i], w) * basis.dx, axis=1)
+    return coo_matrix((data, (rows, np.zeros_like(rows))),
+                      shape=(basis.N, 1)).toarray().T[0]
+
+
+def functional_elemental(integrand, basis):
+    """``Functional.elemental`` (``functional.py:26-34``): per-element
+    integrals ``sum_q f(e,q) dx(e,q)``."""
+    return np.sum(integrand * basis.dx, axis=1)
+
+
+def enforce(A, b=None, x=None, D=None, diag=1.):
+    """``utils.enforce`` with ``D`` given, ``utils.py:295-400``."""
+    D = np.asarray(D)
+    if x is None:
+        x = np.zeros(A.shape[0], dtype=A.dtype)
+    Aout = A.copy()
+    start = Aout.indptr[D]
+    stop = Aout.indptr[D + 1]
+    count = stop - start
+    idx = np.ones(count.sum(), dtype=np.int32)
+    idx[np.cumsum(count)[:-1]] -= count[:-1]
+    idx = np.repeat(start, count) + np.cumsum(idx) - 1
+    Aout.data[idx] = 0.
+    d = Aout.diagonal()
+    d[D] = diag
+    Aout.setdiag(d)
+    if b is None:
+        return Aout
+    bout = b.copy()
+    bout[D] = x[D]
+    return Aout, bout
+
+
+# ---------------------------------------------------------------------------
+# convenience: the built-in forms by name
+# ---------------------------------------------------------------------------
+def assemble_named(name, basis, **kw):
+    """Assemble one of the built-in forms with the reference algorithm."""
+    if name == 'laplace':
+        return bilinear_assemble(form_laplace, basis)
+    if name == 'mass':
+        return bilinear_assemble(form_mass, basis)
+    if name == 'kappa_laplace':
+        return bilinear_assemble(form_kappa_laplace, basis,
+                                 kappa=kw['kappa'])
+    if name == 'linear_elasticity':
+        return bilinear_assemble(form_elasticity(kw['Lambda'], kw['Mu']),
+                                 basis)
+    if name == 'unit_load':
+        return linear_assemble(form_unit_load, basis)
+    raise KeyError(name)
+
+
+def jitter(mesh, a, h, seed=0):
+    """Seeded interior-node jitter of the benchmark configurations:
+    ``p[:, I] += U(-a h, a h)`` with ``I = mesh.interior_nodes()`` and
+    ``rng = default_rng(seed)``.  Returns a new mesh of the same class."""
+    rng = np.random.default_rng(seed)
+    I = mesh.interior_nodes()
+    p = mesh.p.copy()
+    p[:, I] += rng.uniform(-a * h, a * h, size=(p.shape[0], len(I)))
+    out = type(mesh).__new__(type(mesh))
+    out.__dict__.update(mesh.__dict__)
+    out.p = p
+    return out

@@ -1,0 +1,221 @@
+// Templates of the exact-order local element kernels, instantiated per
+// element family in exact_tri.hip, exact_tet.hip and exact_hex.hip (split so
+// the heavy instantiations compile in parallel).  Every translation unit that
+// includes this header is compiled with -ffp-contract=off.
+#pragma once
+
+#include "common.h"
+
+namespace femhip {
+namespace exact {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// local entries
+// ---------------------------------------------------------------------------
+struct LocalArgs {
+  const double* G;
+  const double* W;
+  const double* phi;
+  const double* dphi;
+  const double* kappa;
+  double* out;
+  int64_t E;
+  int nb;
+  int kappa_mode;
+  double lam, two_mu;
+};
+
+// Inverse Jacobian and |det| at (e, q): affine values are per element,
+// isoparametric ones per quadrature point.
+template <int D, bool AFF, int Q>
+struct MapAt {
+  double inv[D][D];
+  double adet;
+  __device__ __forceinline__ void load(const double* __restrict__ G, int64_t E,
+                                       int64_t e, int q) {
+    const int64_t stride = AFF ? E : E * Q;
+    const int64_t at = AFF ? e : e * Q + q;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) inv[i][j] = G[(i * D + j) * stride + at];
+    adet = G[(D * D) * stride + at];
+  }
+};
+
+// grad[j] = sum_i invDF[i, j] * dphi[i]   (einsum 'ijkl,il->jkl',
+// element_h1.py:17), accumulated from 0 in index order.
+template <int D>
+__device__ __forceinline__ void global_grad(const double (&inv)[D][D],
+                                            const double* __restrict__ dphi,
+                                            int fn, int q, int Q,
+                                            double (&g)[D]) {
+#pragma clang fp contract(off)
+  double d[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) d[i] = dphi[(fn * D + i) * Q + q];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s = s + inv[i][j] * d[i];
+    g[j] = s;
+  }
+}
+
+// ddot(C(sym_grad u), sym_grad v) for u = phi_a e_cu, v = phi_b e_cv
+// (models/elasticity.py:35-53, helpers.py:71-150, element_vector.py:36-48).
+template <int D>
+__device__ __forceinline__ double elasticity_value(const double (&gu)[D], int cu,
+                                                   const double (&gv)[D], int cv,
+                                                   double lam, double two_mu) {
+#pragma clang fp contract(off)
+  double Gu[D][D], Gv[D][D];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      Gu[i][j] = (i == cu) ? gu[j] : 0.0;
+      Gv[i][j] = (i == cv) ? gv[j] : 0.0;
+    }
+  double Eu[D][D], Ev[D][D];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      Eu[i][j] = 0.5 * (Gu[i][j] + Gu[j][i]);
+      Ev[i][j] = 0.5 * (Gv[i][j] + Gv[j][i]);
+    }
+  double tr = 0.0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) tr = tr + Eu[i][i];
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const double c = two_mu * Eu[i][j] + lam * ((i == j) ? tr : 0.0 * tr);
+      s = s + c * Ev[i][j];
+    }
+  return s;
+}
+
+template <int D, bool AFF, int Q, int FK, bool VEC>
+__global__ void __launch_bounds__(kBlock) local_bilinear_kernel(LocalArgs a) {
+#pragma clang fp contract(off)
+  const int64_t E = a.E;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= E) return;
+  const int entry = blockIdx.y;  // = j * nb + i
+  const int j = entry / a.nb;
+  const int i = entry - j * a.nb;
+  const int fu = VEC ? j / D : j, cu = VEC ? j - (j / D) * D : 0;
+  const int fv = VEC ? i / D : i, cv = VEC ? i - (i / D) * D : 0;
+
+  MapAt<D, AFF, Q> m;
+  if constexpr (AFF) m.load(a.G, E, e, 0);
+
+  auto term = [&](int q) -> double {
+#pragma clang fp contract(off)
+    if constexpr (!AFF) m.load(a.G, E, e, q);
+    const double dx = m.adet * a.W[q];
+    double val;
+    if constexpr (FK == FEMHIP_MASS) {
+      val = a.phi[fu * Q + q] * a.phi[fv * Q + q];
+    } else {
+      double gu[D], gv[D];
+      global_grad<D>(m.inv, a.dphi, fu, q, Q, gu);
+      global_grad<D>(m.inv, a.dphi, fv, q, Q, gv);
+      if constexpr (FK == FEMHIP_LAPLACE) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s = s + gu[d] * gv[d];
+        val = s;
+      } else if constexpr (FK == FEMHIP_KAPPA_LAPLACE) {
+        const double k = (a.kappa_mode == 1) ? a.kappa[e] : a.kappa[e * Q + q];
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s = s + (k * gu[d]) * gv[d];
+        val = s;
+      } else {
+        val = elasticity_value<D>(gu, cu, gv, cv, a.lam, a.two_mu);
+      }
+    }
+    return val * dx;
+  };
+  const double r = 0.0 + np_pairwise<0, Q>(term);
+  a.out[(int64_t)entry * E + e] = r;
+}
+
+template <int D, bool AFF, int Q>
+__global__ void __launch_bounds__(kBlock) local_load_kernel(LocalArgs a) {
+#pragma clang fp contract(off)
+  const int64_t E = a.E;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= E) return;
+  const int i = blockIdx.y;
+  MapAt<D, AFF, Q> m;
+  if constexpr (AFF) m.load(a.G, E, e, 0);
+  auto term = [&](int q) -> double {
+#pragma clang fp contract(off)
+    if constexpr (!AFF) m.load(a.G, E, e, q);
+    const double dx = m.adet * a.W[q];
+    return a.phi[i * Q + q] * dx;
+  };
+  a.out[(int64_t)i * E + e] = 0.0 + np_pairwise<0, Q>(term);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <int D, bool AFF, int Q>
+int launch_bilinear_q(const LocalArgs& a, int fk, bool vec, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(a.E, kBlock), (unsigned)(a.nb * a.nb));
+  if (vec) {
+    FEMHIP_REQUIRE(fk == FEMHIP_ELASTICITY, FEMHIP_EUNSUPPORTED,
+                   "vector elements support only linear_elasticity");
+    local_bilinear_kernel<D, AFF, Q, FEMHIP_ELASTICITY, true><<<grid, kBlock, 0, s>>>(a);
+  } else {
+    switch (fk) {
+      case FEMHIP_LAPLACE:
+        local_bilinear_kernel<D, AFF, Q, FEMHIP_LAPLACE, false><<<grid, kBlock, 0, s>>>(a);
+        break;
+      case FEMHIP_MASS:
+        local_bilinear_kernel<D, AFF, Q, FEMHIP_MASS, false><<<grid, kBlock, 0, s>>>(a);
+        break;
+      case FEMHIP_KAPPA_LAPLACE:
+        local_bilinear_kernel<D, AFF, Q, FEMHIP_KAPPA_LAPLACE, false><<<grid, kBlock, 0, s>>>(a);
+        break;
+      default:
+        set_error("scalar elements support laplace, mass and kappa_laplace");
+        return FEMHIP_EUNSUPPORTED;
+    }
+  }
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+template <int D, bool AFF, int Q>
+int launch_load_q(const LocalArgs& a, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(a.E, kBlock), (unsigned)a.nb);
+  local_load_kernel<D, AFF, Q><<<grid, kBlock, 0, s>>>(a);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+
+#define FEMHIP_QCASE(D, AFF, QQ, CALL) \
+  case QQ:                             \
+    return CALL<D, AFF, QQ>
+
+int dispatch_bilinear_tri(int Q, const LocalArgs& a, int fk, bool vec, hipStream_t s);
+int dispatch_bilinear_tet(int Q, const LocalArgs& a, int fk, bool vec, hipStream_t s);
+int dispatch_bilinear_hex(int Q, const LocalArgs& a, int fk, bool vec, hipStream_t s);
+int dispatch_load_tri(int Q, const LocalArgs& a, hipStream_t s);
+int dispatch_load_tet(int Q, const LocalArgs& a, hipStream_t s);
+int dispatch_load_hex(int Q, const LocalArgs& a, hipStream_t s);
+
+}  // namespace exact
+}  // namespace femhip

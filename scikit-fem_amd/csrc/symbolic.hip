@@ -1,0 +1,336 @@
+// Symbolic phase: the canonical CSR pattern of the assembled matrix and
+// the reduction plan that maps every CSR slot to its contributions.
+//
+// SciPy's coo_matrix -> eliminate_zeros -> tocsr (coo_data.py:28-36)
+// drops explicit zeros per contribution, then sorts by (row, col) and sums
+// duplicates; the result is canonical (sorted unique int32 column indices
+// per row).  Here the same pattern is built from integer keys
+// key = row * ncols + col (int64): contributions whose exact local value is
+// 0.0 get a sentinel key, a stable LSD radix sort orders the rest, head
+// flags + an integer scan number the unique keys, and one pass writes
+// indices, indptr and the per-slot contribution ranges.  The sort is
+// stable, so within a slot contributions stay in the reference's COO order
+// and the numeric reduction (femhip_segment_sum) is deterministic.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "common.h"
+
+namespace femhip {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void __launch_bounds__(kBlock)
+make_keys_kernel(const int32_t* __restrict__ row_dofs, const int32_t* __restrict__ col_dofs,
+                 int nb_row, int64_t E, int64_t ncols, int64_t C, uint64_t sentinel,
+                 const double* __restrict__ contrib, uint64_t* __restrict__ keys,
+                 int32_t* __restrict__ vals) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= C) return;
+  const int64_t entry = c / E;
+  const int64_t e = c - entry * E;
+  const int64_t j = entry / nb_row;  // trial (column) local index
+  const int64_t i = entry - j * nb_row;  // test (row) local index
+  const int64_t r = row_dofs[i * E + e];
+  const int64_t col = col_dofs[j * E + e];
+  const bool keep = (contrib == nullptr) || (contrib[c] != 0.0);
+  keys[c] = keep ? (uint64_t)(r * ncols + col) : sentinel;
+  vals[c] = (int32_t)c;
+}
+
+__global__ void __launch_bounds__(kBlock)
+make_row_keys_kernel(const int32_t* __restrict__ row_dofs, int64_t E, int64_t C,
+                     uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= C) return;
+  // contribution c = i * E + e reads element_dofs[i][e] = row_dofs[c]
+  keys[c] = (uint64_t)row_dofs[c];
+  vals[c] = (int32_t)c;
+}
+
+__global__ void __launch_bounds__(kBlock)
+head_flags_kernel(const uint64_t* __restrict__ keys, int64_t C, uint64_t sentinel,
+                  int32_t* __restrict__ flags) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const uint64_t key = keys[k];
+  flags[k] = (key != sentinel && (k == 0 || keys[k - 1] != key)) ? 1 : 0;
+}
+
+// For every head: the slot's column index and the start of its range; the
+// end of the last range is the first sentinel position.
+__global__ void __launch_bounds__(kBlock)
+fill_slots_kernel(const uint64_t* __restrict__ keys, const int32_t* __restrict__ scan,
+                  int64_t C, uint64_t sentinel, int64_t ncols,
+                  int32_t* __restrict__ indices, int32_t* __restrict__ seg,
+                  int32_t* __restrict__ slot_row) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const uint64_t key = keys[k];
+  if (key == sentinel) return;
+  const bool head = (k == 0 || keys[k - 1] != key);
+  if (head) {
+    const int32_t s = scan[k] - 1;
+    indices[s] = (int32_t)(key % (uint64_t)ncols);
+    slot_row[s] = (int32_t)(key / (uint64_t)ncols);
+    seg[s] = (int32_t)k;
+  }
+  if (k == C - 1 || keys[k + 1] == sentinel) seg[scan[k]] = (int32_t)(k + 1);
+}
+
+// indptr[r] = first slot whose row is >= r.
+__global__ void __launch_bounds__(kBlock)
+fill_indptr_kernel(const int32_t* __restrict__ slot_row, int64_t nnz, int64_t nrows,
+                   int32_t* __restrict__ indptr) {
+  const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (s >= nnz) return;
+  const int64_t r = slot_row[s];
+  const int64_t rp = (s == 0) ? -1 : slot_row[s - 1];
+  for (int64_t rr = rp + 1; rr <= r; ++rr) indptr[rr] = (int32_t)s;
+  if (s == nnz - 1)
+    for (int64_t rr = r + 1; rr <= nrows; ++rr) indptr[rr] = (int32_t)nnz;
+}
+
+// seg[r] = first sorted position whose row key is >= r (rows plan).
+__global__ void __launch_bounds__(kBlock)
+fill_row_seg_kernel(const uint64_t* __restrict__ keys, int64_t C, int64_t nrows,
+                    int32_t* __restrict__ seg) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const int64_t r = (int64_t)keys[k];
+  const int64_t rp = (k == 0) ? -1 : (int64_t)keys[k - 1];
+  for (int64_t rr = rp + 1; rr <= r; ++rr) seg[rr] = (int32_t)k;
+  if (k == C - 1)
+    for (int64_t rr = r + 1; rr <= nrows; ++rr) seg[rr] = (int32_t)C;
+}
+
+__global__ void __launch_bounds__(kBlock)
+segment_sum_kernel(const double* __restrict__ contrib, const int32_t* __restrict__ perm,
+                   const int32_t* __restrict__ seg, int64_t nseg, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nseg) return;
+  const int32_t b = seg[k], en = seg[k + 1];
+  double acc = 0.0;
+  for (int32_t j = b; j < en; ++j) acc += contrib[perm[j]];
+  out[k] = acc;
+}
+
+// Order-independent signature of the exact-zero positions of contrib:
+// (number of zeros, wrapping sum of a 64-bit mix of their indices).
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+
+__global__ void __launch_bounds__(kBlock)
+zero_signature_kernel(const double* __restrict__ contrib, int64_t n,
+                      unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s_cnt[kBlock], s_sum[kBlock];
+  unsigned long long cnt = 0, sum = 0;
+  for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * kBlock) {
+    if (contrib[c] == 0.0) {
+      cnt += 1;
+      sum += mix64((uint64_t)c + 0x9e3779b97f4a7c15ULL);
+    }
+  }
+  s_cnt[threadIdx.x] = cnt;
+  s_sum[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + w];
+      s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], s_cnt[0]);
+    atomicAdd(&out[1], s_sum[0]);
+  }
+}
+
+unsigned bits_for(uint64_t v) {
+  unsigned b = 1;
+  while (b < 64 && (v >> b) != 0) ++b;
+  return b;
+}
+
+struct Workspace {
+  uint64_t* keys_in;
+  uint64_t* keys_out;
+  int32_t* vals_in;
+  int32_t* scan;
+  int32_t* slot_row;
+  void* temp;
+  size_t temp_bytes;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+size_t rocprim_temp_bytes(int64_t C) {
+  size_t sort_bytes = 0, scan_bytes = 0;
+  (void)rocprim::radix_sort_pairs<rocprim::default_config>(
+      nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr,
+      (int32_t*)nullptr, (size_t)C, 0, 64);
+  (void)rocprim::inclusive_scan<rocprim::default_config>(
+      nullptr, scan_bytes, (int32_t*)nullptr, (int32_t*)nullptr, (size_t)C,
+      rocprim::plus<int32_t>());
+  return sort_bytes > scan_bytes ? sort_bytes : scan_bytes;
+}
+
+size_t workspace_bytes(int64_t C) {
+  return align256(8 * C) * 2 + align256(4 * C) * 3 + align256(rocprim_temp_bytes(C));
+}
+
+Workspace carve(void* ws, int64_t C) {
+  char* b = static_cast<char*>(ws);
+  Workspace w;
+  w.keys_in = reinterpret_cast<uint64_t*>(b);
+  b += align256(8 * C);
+  w.keys_out = reinterpret_cast<uint64_t*>(b);
+  b += align256(8 * C);
+  w.vals_in = reinterpret_cast<int32_t*>(b);
+  b += align256(4 * C);
+  w.scan = reinterpret_cast<int32_t*>(b);
+  b += align256(4 * C);
+  w.slot_row = reinterpret_cast<int32_t*>(b);
+  b += align256(4 * C);
+  w.temp = b;
+  w.temp_bytes = rocprim_temp_bytes(C);
+  return w;
+}
+
+}  // namespace
+}  // namespace femhip
+
+using namespace femhip;
+
+extern "C" int femhip_pattern_workspace(int64_t ncontrib, size_t* bytes) {
+  FEMHIP_REQUIRE(bytes != nullptr && ncontrib >= 0, FEMHIP_EINVAL, "bad arguments");
+  FEMHIP_REQUIRE(ncontrib < INT32_MAX, FEMHIP_EOVERFLOW,
+                 "more than 2^31-1 contributions");
+  *bytes = workspace_bytes(ncontrib > 0 ? ncontrib : 1);
+  return FEMHIP_OK;
+}
+
+extern "C" int femhip_pattern_build(const int32_t* row_dofs, const int32_t* col_dofs,
+                                    int32_t nb_row, int32_t nb_col, int64_t nelems,
+                                    int64_t nrows, int64_t ncols, const double* contrib,
+                                    int32_t* indptr, int32_t* indices, int32_t* perm,
+                                    int32_t* seg, int64_t* nnz_out, void* workspace,
+                                    size_t ws_bytes, femhip_stream_t s) {
+  FEMHIP_REQUIRE(nb_row > 0 && nb_col > 0 && nelems >= 0 && nrows >= 0 && ncols > 0,
+                 FEMHIP_EINVAL, "bad sizes");
+  FEMHIP_REQUIRE(nnz_out != nullptr && indptr != nullptr, FEMHIP_EINVAL, "NULL output");
+  FEMHIP_REQUIRE(nrows < INT32_MAX && ncols < INT32_MAX, FEMHIP_EOVERFLOW,
+                 "matrix dimension exceeds int32");
+  const int64_t C = (int64_t)nb_row * nb_col * nelems;
+  FEMHIP_REQUIRE(C < INT32_MAX, FEMHIP_EOVERFLOW, "more than 2^31-1 contributions");
+  hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  if (C == 0) {
+    FEMHIP_TRY_HIP(hipMemsetAsync(indptr, 0, sizeof(int32_t) * (nrows + 1), st));
+    *nnz_out = 0;
+    return FEMHIP_OK;
+  }
+  FEMHIP_REQUIRE(workspace != nullptr && ws_bytes >= workspace_bytes(C), FEMHIP_ESIZE,
+                 "workspace too small");
+  FEMHIP_REQUIRE(row_dofs && col_dofs && indices && perm && seg, FEMHIP_EINVAL,
+                 "NULL input/output");
+  Workspace w = carve(workspace, C);
+  const uint64_t sentinel = (uint64_t)nrows * (uint64_t)ncols;
+  const unsigned end_bit = bits_for(sentinel);
+  const unsigned grid = (unsigned)ceil_div(C, kBlock);
+  make_keys_kernel<<<grid, kBlock, 0, st>>>(row_dofs, col_dofs, nb_row, nelems, ncols, C,
+                                            sentinel, contrib, w.keys_in, w.vals_in);
+  FEMHIP_CHECK_LAUNCH();
+  size_t tb = w.temp_bytes;
+  FEMHIP_TRY_HIP(rocprim::radix_sort_pairs<rocprim::default_config>(
+      w.temp, tb, w.keys_in, w.keys_out, w.vals_in, perm, (size_t)C, 0, end_bit, st));
+  head_flags_kernel<<<grid, kBlock, 0, st>>>(w.keys_out, C, sentinel, w.vals_in);
+  FEMHIP_CHECK_LAUNCH();
+  tb = w.temp_bytes;
+  FEMHIP_TRY_HIP(rocprim::inclusive_scan<rocprim::default_config>(
+      w.temp, tb, w.vals_in, w.scan, (size_t)C, rocprim::plus<int32_t>(), st));
+  int32_t nnz32 = 0;
+  FEMHIP_TRY_HIP(hipMemcpyAsync(&nnz32, w.scan + (C - 1), sizeof(int32_t),
+                                hipMemcpyDeviceToHost, st));
+  FEMHIP_TRY_HIP(hipStreamSynchronize(st));
+  const int64_t nnz = nnz32;
+  FEMHIP_TRY_HIP(hipMemsetAsync(seg, 0, sizeof(int32_t), st));
+  fill_slots_kernel<<<grid, kBlock, 0, st>>>(w.keys_out, w.scan, C, sentinel, ncols,
+                                             indices, seg, w.slot_row);
+  FEMHIP_CHECK_LAUNCH();
+  if (nnz == 0) {
+    FEMHIP_TRY_HIP(hipMemsetAsync(indptr, 0, sizeof(int32_t) * (nrows + 1), st));
+  } else {
+    fill_indptr_kernel<<<(unsigned)ceil_div(nnz, kBlock), kBlock, 0, st>>>(
+        w.slot_row, nnz, nrows, indptr);
+    FEMHIP_CHECK_LAUNCH();
+  }
+  FEMHIP_TRY_HIP(hipStreamSynchronize(st));
+  *nnz_out = nnz;
+  return FEMHIP_OK;
+}
+
+extern "C" int femhip_rows_plan_build(const int32_t* row_dofs, int32_t nb_row,
+                                      int64_t nelems, int64_t nrows, int32_t* perm,
+                                      int32_t* seg, void* workspace, size_t ws_bytes,
+                                      femhip_stream_t s) {
+  FEMHIP_REQUIRE(nb_row > 0 && nelems >= 0 && nrows >= 0, FEMHIP_EINVAL, "bad sizes");
+  FEMHIP_REQUIRE(seg != nullptr, FEMHIP_EINVAL, "NULL seg");
+  const int64_t C = (int64_t)nb_row * nelems;
+  FEMHIP_REQUIRE(C < INT32_MAX, FEMHIP_EOVERFLOW, "more than 2^31-1 contributions");
+  hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  if (C == 0) {
+    FEMHIP_TRY_HIP(hipMemsetAsync(seg, 0, sizeof(int32_t) * (nrows + 1), st));
+    return FEMHIP_OK;
+  }
+  FEMHIP_REQUIRE(workspace != nullptr && ws_bytes >= workspace_bytes(C), FEMHIP_ESIZE,
+                 "workspace too small");
+  FEMHIP_REQUIRE(row_dofs && perm, FEMHIP_EINVAL, "NULL input/output");
+  Workspace w = carve(workspace, C);
+  const unsigned grid = (unsigned)ceil_div(C, kBlock);
+  make_row_keys_kernel<<<grid, kBlock, 0, st>>>(row_dofs, nelems, C, w.keys_in, w.vals_in);
+  FEMHIP_CHECK_LAUNCH();
+  size_t tb = w.temp_bytes;
+  FEMHIP_TRY_HIP(rocprim::radix_sort_pairs<rocprim::default_config>(
+      w.temp, tb, w.keys_in, w.keys_out, w.vals_in, perm, (size_t)C, 0,
+      bits_for((uint64_t)nrows), st));
+  fill_row_seg_kernel<<<grid, kBlock, 0, st>>>(w.keys_out, C, nrows, seg);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+extern "C" int femhip_segment_sum(const double* contrib, const int32_t* perm,
+                                  const int32_t* seg, int64_t nseg, double* out,
+                                  femhip_stream_t s) {
+  FEMHIP_REQUIRE(nseg >= 0, FEMHIP_EINVAL, "negative segment count");
+  if (nseg == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(contrib && perm && seg && out, FEMHIP_EINVAL, "NULL argument");
+  segment_sum_kernel<<<(unsigned)ceil_div(nseg, kBlock), kBlock, 0,
+                       reinterpret_cast<hipStream_t>(s)>>>(contrib, perm, seg, nseg, out);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+extern "C" int femhip_zero_signature(const double* contrib, int64_t n, int64_t* out,
+                                     femhip_stream_t s) {
+  FEMHIP_REQUIRE(n >= 0 && out != nullptr, FEMHIP_EINVAL, "bad arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  FEMHIP_TRY_HIP(hipMemsetAsync(out, 0, 2 * sizeof(int64_t), st));
+  if (n == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(contrib != nullptr, FEMHIP_EINVAL, "NULL contrib");
+  int64_t blocks = ceil_div(n, kBlock);
+  if (blocks > 4096) blocks = 4096;
+  zero_signature_kernel<<<(unsigned)blocks, kBlock, 0, st>>>(
+      contrib, n, reinterpret_cast<unsigned long long*>(out));
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}

@@ -1,0 +1,84 @@
+"""``CellBasis``: mesh + element + quadrature, the input of every assembly.
+
+Unlike the reference (``cell_basis.py:42-106``), which precomputes the
+global basis values and gradients at every (element, quadrature point) on
+the host, this basis is lazy: it keeps the mesh arrays, the DOF numbering
+and the reference tables at the quadrature points.  The device copies
+(int32 connectivity, SoA float64 coordinates, tables) are made once per
+basis on first use and reused by every assembly on it.
+"""
+import numpy as np
+
+from .dofs import Dofs
+from .quadrature import get_quadrature
+from .element import ElementHex1
+
+
+class CellBasis:
+
+    def __init__(self, mesh, elem, mapping=None, intorder=None,
+                 elements=None, quadrature=None, dofs=None,
+                 disable_doflocs=False):
+        if mesh.kind != elem.kind:
+            raise ValueError("Incompatible Mesh and Element.")
+        if mapping is not None:
+            raise NotImplementedError("custom mappings are not accelerated")
+        if elements is not None:
+            raise NotImplementedError("element subsets are not accelerated")
+        self.mesh = mesh
+        self.elem = elem
+        self.dofs = Dofs(mesh, elem) if dofs is None else dofs
+        if quadrature is not None:
+            self.X, self.W = (np.asarray(quadrature[0], dtype=np.float64),
+                              np.asarray(quadrature[1], dtype=np.float64))
+        else:
+            order = intorder if intorder is not None else 2 * elem.maxdeg
+            self.X, self.W = get_quadrature(mesh.kind, order)
+        self.tind = None
+        self.nelems = mesh.nelements
+        self._device = None
+
+    @property
+    def N(self):
+        return self.dofs.N
+
+    @property
+    def Nbfun(self):
+        return self.element_dofs.shape[0]
+
+    @property
+    def element_dofs(self):
+        return self.dofs.element_dofs
+
+    @property
+    def nodal_dofs(self):
+        return self.dofs.nodal_dofs
+
+    def tables(self):
+        """Reference tables at the quadrature points:
+        (phi (nbs, Q), dphi (nbs, dim, Q), gdphi (8, 3, Q) or None)."""
+        phi, dphi = self.elem.tables(self.X)
+        gdphi = None
+        if self.mesh.kind == 'hex':
+            _, gdphi = ElementHex1().tables(self.X)
+        return phi, dphi, gdphi
+
+    def zeros(self):
+        return np.zeros(self.N)
+
+    def with_element(self, elem):
+        return type(self)(self.mesh, elem, quadrature=(self.X, self.W))
+
+    def device(self):
+        """Device copy of this basis (created on first use)."""
+        if self._device is None:
+            from .device import DeviceBasis
+            self._device = DeviceBasis(self)
+        return self._device
+
+    def __repr__(self):
+        return "CellBasis(%r, %r, %d quadrature points)" % (
+            self.mesh, self.elem, self.W.shape[0])
+
+
+Basis = CellBasis

@@ -1,0 +1,229 @@
+"""Device side of a ``CellBasis``: uploaded mesh arrays, the symbolic plans
+and the launch sequence of the generic assembly path.
+
+Generic path (every element / form):
+
+1. ``femhip_geometry`` -- mapping (invDF, |detDF|) in the reference's
+   operation order, once per basis;
+2. ``femhip_local_bilinear`` -- all local entries, bit-identical to the
+   reference's NumPy values, in its COO order;
+3. symbolic plan -- CSR pattern of the nonzero entries + per-slot
+   contribution ranges, built once and revalidated on every call with an
+   order-independent signature of the zero positions;
+4. ``femhip_segment_sum`` -- deterministic per-slot sums.
+
+The fused P1 path (``femhip.fused``) replaces 2-4 for P1 triangles and
+tetrahedra once the plan exists.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise RuntimeError("femhip needs a ROCm GPU (torch.cuda is unavailable)")
+    return torch.device('cuda')
+
+
+def to_dev(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(_dev())
+
+
+_KIND = {'tri': _lib.TRI, 'tet': _lib.TET, 'hex': _lib.HEX}
+_FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS,
+         'kappa_laplace': _lib.KAPPA_LAPLACE,
+         'linear_elasticity': _lib.ELASTICITY, 'unit_load': _lib.UNIT_LOAD}
+
+
+class DeviceCSR:
+    """A CSR matrix whose arrays live in HBM (int32 indptr/indices, float64
+    data).  ``to_scipy`` copies it to the host as ``csr_matrix``."""
+
+    def __init__(self, indptr, indices, data, shape):
+        self.indptr, self.indices, self.data, self.shape = indptr, indices, data, shape
+
+    @property
+    def nnz(self):
+        return int(self.indices.shape[0])
+
+    def to_scipy(self):
+        from scipy.sparse import csr_matrix
+        torch.cuda.current_stream().synchronize()
+        A = csr_matrix((self.data.cpu().numpy(), self.indices.cpu().numpy(),
+                        self.indptr.cpu().numpy()), shape=self.shape)
+        return A
+
+
+class Plan:
+    """Symbolic result for one (basis, form): the CSR pattern and the
+    reduction plan (``perm``/``seg``), plus the signature of the exact-zero
+    positions it was built from."""
+
+    def __init__(self, indptr, indices, perm, seg, nnz, signature):
+        self.indptr, self.indices, self.perm, self.seg = indptr, indices, perm, seg
+        self.nnz, self.signature = nnz, signature
+        self.fused = None   # fused-kernel plan, built on demand
+
+
+class DeviceBasis:
+
+    def __init__(self, basis):
+        lib = _lib.load()
+        self.lib = lib
+        self.basis = basis
+        mesh, elem = basis.mesh, basis.elem
+        self.kind = mesh.kind
+        self.dim = mesh.dim
+        self.E = mesh.nelements
+        self.Q = basis.W.shape[0]
+        self.nbfun = basis.Nbfun
+        self.N = basis.N
+        self.p = to_dev(mesh.p, np.float64)
+        self.t = to_dev(mesh.t, np.int32)
+        self.dofs = to_dev(basis.element_dofs, np.int32)
+        phi, dphi, gdphi = basis.tables()
+        self.W = to_dev(basis.W, np.float64)
+        self.phi = to_dev(phi, np.float64)
+        self.dphi = to_dev(dphi, np.float64)
+        self.gdphi = None if gdphi is None else to_dev(gdphi, np.float64)
+        self.geom = _lib.Geom(_KIND[self.kind], self.dim, mesh.nverts, self.Q,
+                              self.E, mesh.p.shape[1], self.p.data_ptr(),
+                              self.t.data_ptr(), self.W.data_ptr(),
+                              0 if self.gdphi is None else self.gdphi.data_ptr())
+        self.space = _lib.Space(elem.nbs, 1 if elem.vector else 0, self.nbfun, 0,
+                                self.N, self.dofs.data_ptr(), self.phi.data_ptr(),
+                                self.dphi.data_ptr())
+        self._G = None
+        self.plans = {}
+        self.row_plan = None
+
+    # -- coordinates ---------------------------------------------------------
+    def set_coordinates(self, p):
+        """Replace the node coordinates (device tensor (dim, nnodes) float64);
+        the mapping is recomputed on next use, plans are revalidated."""
+        assert p.shape == self.p.shape and p.dtype == torch.float64
+        self.p = p.contiguous()
+        self.geom.p = self.p.data_ptr()
+        self._G = None
+
+    def geometry(self):
+        if self._G is None:
+            n = self.lib.femhip_geometry_size(ctypes.byref(self.geom))
+            if n < 0:
+                _lib.check(_lib.EINVAL)
+            G = torch.empty(max(n, 1), dtype=torch.float64, device=self.p.device)
+            status = torch.zeros(1, dtype=torch.int32, device=self.p.device)
+            _lib.check(self.lib.femhip_geometry(ctypes.byref(self.geom), _lib.ptr(G),
+                                                _lib.ptr(status), _lib.stream_ptr()))
+            if self.kind == 'hex' and int(status.item()) != 0:
+                raise Exception("Zero Jacobian determinant")
+            self._G = G
+        return self._G
+
+    # -- forms ---------------------------------------------------------------
+    def form_struct(self, name, kappa=None, lam=0.0, mu=0.0):
+        mode, kp = 0, None
+        if name == 'kappa_laplace':
+            kp = kappa
+            mode = 1 if kp.dim() == 1 else 2
+        fs = _lib.Form(_FORM[name], mode, 0 if kp is None else kp.data_ptr(),
+                       float(lam), float(2. * mu))
+        return fs, kp
+
+    def local_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, out=None):
+        G = self.geometry()
+        fs, keep = self.form_struct(name, kappa, lam, mu)
+        if out is None:
+            out = torch.empty(self.nbfun * self.nbfun * self.E, dtype=torch.float64,
+                              device=self.p.device)
+        _lib.check(self.lib.femhip_local_bilinear(
+            ctypes.byref(self.geom), ctypes.byref(self.space), ctypes.byref(fs),
+            _lib.ptr(G), _lib.ptr(out), _lib.stream_ptr()))
+        return out
+
+    def local_linear(self, name='unit_load'):
+        G = self.geometry()
+        fs, _ = self.form_struct(name)
+        out = torch.empty(self.nbfun * self.E, dtype=torch.float64, device=self.p.device)
+        _lib.check(self.lib.femhip_local_linear(
+            ctypes.byref(self.geom), ctypes.byref(self.space), ctypes.byref(fs),
+            _lib.ptr(G), _lib.ptr(out), _lib.stream_ptr()))
+        return out
+
+    # -- symbolic ------------------------------------------------------------
+    def _workspace(self, ncontrib):
+        nb = ctypes.c_size_t(0)
+        _lib.check(self.lib.femhip_pattern_workspace(ncontrib, ctypes.byref(nb)))
+        return torch.empty(int(nb.value), dtype=torch.uint8, device=self.p.device), int(nb.value)
+
+    def build_plan(self, contrib, signature=None):
+        C = self.nbfun * self.nbfun * self.E
+        dev = self.p.device
+        ws, wsb = self._workspace(C)
+        indptr = torch.empty(self.N + 1, dtype=torch.int32, device=dev)
+        indices = torch.empty(max(C, 1), dtype=torch.int32, device=dev)
+        perm = torch.empty(max(C, 1), dtype=torch.int32, device=dev)
+        seg = torch.empty(max(C, 1) + 1, dtype=torch.int32, device=dev)
+        nnz = ctypes.c_int64(0)
+        _lib.check(self.lib.femhip_pattern_build(
+            _lib.ptr(self.dofs), _lib.ptr(self.dofs), self.nbfun, self.nbfun, self.E,
+            self.N, self.N, _lib.ptr(contrib), _lib.ptr(indptr), _lib.ptr(indices),
+            _lib.ptr(perm), _lib.ptr(seg), ctypes.byref(nnz), _lib.ptr(ws), wsb,
+            _lib.stream_ptr()))
+        n = int(nnz.value)
+        del ws
+        return Plan(indptr, indices[:n].clone(), perm, seg[:n + 1].clone(), n, signature)
+
+    def zero_signature(self, contrib):
+        out = torch.empty(2, dtype=torch.int64, device=self.p.device)
+        _lib.check(self.lib.femhip_zero_signature(_lib.ptr(contrib), contrib.numel(),
+                                                  _lib.ptr(out), _lib.stream_ptr()))
+        return tuple(int(v) for v in out.cpu().tolist())
+
+    def plan_for(self, key, contrib):
+        """Plan whose zero positions match ``contrib`` (built when absent or
+        when the zero signature changed)."""
+        sig = self.zero_signature(contrib)
+        plan = self.plans.get(key)
+        if plan is None or plan.signature != sig:
+            plan = self.build_plan(contrib, sig)
+            self.plans[key] = plan
+        return plan
+
+    def rows_plan(self):
+        if self.row_plan is None:
+            C = self.nbfun * self.E
+            dev = self.p.device
+            ws, wsb = self._workspace(C)
+            perm = torch.empty(max(C, 1), dtype=torch.int32, device=dev)
+            seg = torch.empty(self.N + 1, dtype=torch.int32, device=dev)
+            _lib.check(self.lib.femhip_rows_plan_build(
+                _lib.ptr(self.dofs), self.nbfun, self.E, self.N, _lib.ptr(perm),
+                _lib.ptr(seg), _lib.ptr(ws), wsb, _lib.stream_ptr()))
+            torch.cuda.current_stream().synchronize()
+            self.row_plan = (perm, seg)
+        return self.row_plan
+
+    def segment_sum(self, contrib, perm, seg, nseg, out=None):
+        if out is None:
+            out = torch.empty(max(nseg, 1), dtype=torch.float64, device=self.p.device)[:nseg]
+        _lib.check(self.lib.femhip_segment_sum(_lib.ptr(contrib), _lib.ptr(perm),
+                                               _lib.ptr(seg), nseg, _lib.ptr(out),
+                                               _lib.stream_ptr()))
+        return out
+
+    # -- assembly ------------------------------------------------------------
+    def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, key=None):
+        contrib = self.local_bilinear(name, kappa, lam, mu)
+        plan = self.plan_for(key if key is not None else (name, lam, mu), contrib)
+        data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
+        return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N))
+
+    def assemble_linear(self, name='unit_load'):
+        contrib = self.local_linear(name)
+        perm, seg = self.rows_plan()
+        return self.segment_sum(contrib, perm, seg, self.N)

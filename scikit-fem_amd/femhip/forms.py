@@ -1,0 +1,163 @@
+"""``BilinearForm`` / ``LinearForm`` / ``asm`` with scikit-fem's call
+signatures (``skfem/assembly/form/form.py:28-121``,
+``bilinear_form.py:130-148``, ``linear_form.py:18-49``,
+``skfem/assembly/__init__.py:69-97``).
+
+A form is accelerated when it is one of the built-in models of
+``femhip.models`` (``laplace``, ``mass``, ``unit_load``, ``kappa_laplace``,
+``linear_elasticity(Lambda, Mu)``), or a user form that the tracer in
+``femhip.trace`` recognises as one of them times a constant.  Anything else
+raises ``NotImplementedError``: there is no CPU fallback.
+"""
+import numbers
+
+import numpy as np
+import torch
+
+from .trace import identify
+
+
+class Form:
+
+    form = None
+
+    def __init__(self, form=None, dtype=np.float64, nthreads=0, **params):
+        self.form = form.form if isinstance(form, Form) else form
+        self.dtype = dtype
+        self.nthreads = nthreads
+        self.params = params
+        self._spec = getattr(form, '_spec', None)
+
+    def __call__(self, *args):
+        if self.form is None:  # used as a decorator with arguments
+            return type(self)(form=args[0], dtype=self.dtype,
+                              nthreads=self.nthreads, **self.params)
+        raise NotImplementedError("direct evaluation of forms is not supported")
+
+    # identification of the built-in kernel ---------------------------------
+    def spec(self):
+        """(kernel name, constant factor, parameters) of this form."""
+        if self._spec is None:
+            self._spec = identify(self.form, self.nargs)
+        return self._spec
+
+    def _check_dtype(self):
+        if np.dtype(self.dtype) != np.float64:
+            raise NotImplementedError("only float64 assembly is accelerated")
+
+
+def _kappa_tensor(w, basis, name='kappa'):
+    """Coefficient array of ``w[name]`` as a device tensor: (E,) per element
+    or (E*Q,) per quadrature point (the reference accepts an (E, Q) array or
+    a DiscreteField, ``form.py:92-121``)."""
+    if name not in w:
+        raise ValueError("Attribute '%s' not found in 'w'." % name)
+    k = w[name]
+    if hasattr(k, 'value') and not isinstance(k, np.ndarray):
+        k = k.value
+    E, Q = basis.nelems, basis.W.shape[0]
+    if isinstance(k, torch.Tensor):
+        kt = k.detach().to(dtype=torch.float64, device='cuda')
+    else:
+        k = np.asarray(k, dtype=np.float64)
+        if k.ndim == 1:
+            raise NotImplementedError(
+                "1-D coefficient arrays are DOF vectors in the reference "
+                "(Basis.interpolate); pass a per-element (E, 1) or per-point "
+                "(E, Q) array")
+        kt = torch.from_numpy(np.ascontiguousarray(k)).to('cuda')
+    if kt.dim() == 2 and kt.shape == (E, 1):
+        return kt.reshape(E).contiguous()
+    if kt.dim() == 2 and kt.shape == (E, Q):
+        return kt.reshape(E * Q).contiguous()
+    if kt.dim() == 1 and kt.shape[0] == E:
+        return kt.contiguous()
+    raise ValueError("Quadrature mismatch: '%s' should have shape (E, Q) = "
+                     "(%d, %d) or (E, 1)" % (name, E, Q))
+
+
+class BilinearForm(Form):
+    """Bilinear form ``a(u, v)``; ``assemble`` returns a SciPy CSR matrix
+    with the reference's canonical pattern."""
+
+    nargs = 3
+
+    def assemble_device(self, ubasis, vbasis=None, **kwargs):
+        """Assemble and keep the result in HBM (``DeviceCSR``)."""
+        self._check_dtype()
+        if vbasis is not None and vbasis is not ubasis:
+            raise NotImplementedError("different trial and test bases")
+        name, scale, params = self.spec()
+        dev = ubasis.device()
+        if name == 'kappa_laplace':
+            kappa = _kappa_tensor({**self.params, **kwargs}, ubasis,
+                                  params.get('coef', 'kappa'))
+            key = (name, 'kappa', kappa.data_ptr(), getattr(kappa, '_version', 0))
+            A = dev.assemble_bilinear(name, kappa=kappa, key=key)
+        elif name == 'linear_elasticity':
+            A = dev.assemble_bilinear(name, lam=params['Lambda'], mu=params['Mu'])
+        else:
+            A = dev.assemble_bilinear(name)
+        if scale != 1.0:
+            A.data = A.data * scale
+        return A
+
+    def assemble(self, ubasis, vbasis=None, **kwargs):
+        return self.assemble_device(ubasis, vbasis, **kwargs).to_scipy()
+
+    def elemental(self, ubasis, **kwargs):
+        """Local element matrices (E, Nbfun, Nbfun) as ``COOData.tolocal``
+        returns them (``coo_data.py:45-66``)."""
+        name, scale, params = self.spec()
+        dev = ubasis.device()
+        kappa = None
+        if name == 'kappa_laplace':
+            kappa = _kappa_tensor({**self.params, **kwargs}, ubasis,
+                                  params.get('coef', 'kappa'))
+        c = dev.local_bilinear(name, kappa=kappa, lam=params.get('Lambda', 0.0),
+                               mu=params.get('Mu', 0.0))
+        nb = ubasis.Nbfun
+        loc = c.reshape(nb, nb, ubasis.nelems).permute(2, 1, 0)  # (e, i, j)
+        loc = loc.cpu().numpy() * scale
+        return loc
+
+
+class LinearForm(Form):
+    """Linear form ``l(v)``; ``assemble`` returns the load vector."""
+
+    nargs = 2
+
+    def assemble_device(self, basis, vbasis=None, **kwargs):
+        self._check_dtype()
+        if vbasis is not None:
+            raise ValueError("LinearForm takes one basis")
+        name, scale, _ = self.spec()
+        if name != 'unit_load':
+            raise NotImplementedError("only unit_load-type linear forms are accelerated")
+        b = basis.device().assemble_linear('unit_load')
+        if scale != 1.0:
+            b = b * scale
+        return b
+
+    def assemble(self, basis, vbasis=None, **kwargs):
+        b = self.assemble_device(basis, vbasis, **kwargs)
+        torch.cuda.current_stream().synchronize()
+        return b.cpu().numpy()
+
+
+def asm(form, *bases, **kwargs):
+    """``skfem.assembly.asm`` for one basis (``assembly/__init__.py:69-97``)."""
+    if len(bases) != 1:
+        raise NotImplementedError("asm over several bases")
+    if isinstance(form, Form):
+        return form.assemble(bases[0], **kwargs)
+    if callable(form):
+        import inspect
+        n = len(inspect.signature(form).parameters)
+        wrapped = BilinearForm(form) if n == 3 else LinearForm(form)
+        return wrapped.assemble(bases[0], **kwargs)
+    raise ValueError("unknown form type")
+
+
+def is_number(x):
+    return isinstance(x, numbers.Number)
