@@ -162,6 +162,56 @@ int femhip_segment_sum(const double* contrib, const int32_t* perm,
                        const int32_t* seg, int64_t nseg, double* out,
                        femhip_stream_t s);
 
+/* ---------------------------------------------------------------------
+ * Fused P1 path (triangles / tetrahedra; laplace, mass, kappa_laplace).
+ * One kernel replaces BilinearForm._assemble + COOData._assemble_scipy_csr
+ * (bilinear_form.py:58-148, coo_data.py:28-36) once the CSR pattern is
+ * known: every workgroup owns a set of rows (a Morton-ordered node
+ * partition), recomputes the local matrices of the elements touching them
+ * in registers, sums the contributions in LDS in a fixed colour order and
+ * writes each CSR row once.
+ *
+ * femhip_fused_plan_build runs on the host (host pointers: t, p, indptr,
+ * indices of the pattern) and keeps the plan in library-owned host memory
+ * until femhip_fused_plan_free; femhip_fused_plan_export copies its arrays
+ * into caller buffers sized from the info struct:
+ *   parts[8*nparts] nodemap[nlocal] rows[2*nrows] chunks[2*nchunks]
+ *   recs[rec_words*nrecs] rec_elem[nrecs].
+ * --------------------------------------------------------------------- */
+typedef struct femhip_fused_plan_info {
+  int32_t nparts, nchunks, nrows, max_slots;
+  int32_t max_nodes, max_rows, rec_words, idbits;
+  int32_t offbits, max_colors, block, reserved;
+  int64_t nrecs, nlocal;
+  double dup;           /* element visits / elements (halo recomputation)  */
+} femhip_fused_plan_info_t;
+
+int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t nelems, int64_t nnodes,
+                            const int32_t* t, const double* p, const int32_t* indptr,
+                            const int32_t* indices, int32_t slot_cap, int32_t node_cap,
+                            int32_t block, void** handle, femhip_fused_plan_info_t* info);
+int femhip_fused_plan_export(void* handle, int32_t* parts, int32_t* nodemap, int32_t* rows,
+                             int32_t* chunks, uint32_t* recs, int32_t* rec_elem);
+void femhip_fused_plan_free(void* handle);
+
+/* The exported plan in device memory. */
+typedef struct femhip_fused_dev {
+  const int32_t* parts;
+  const int32_t* nodemap;
+  const int32_t* rows;
+  const int32_t* chunks;
+  const uint32_t* recs;
+  const int32_t* rec_elem;
+  femhip_fused_plan_info_t info;
+} femhip_fused_dev_t;
+
+/* data[nnz] <- assembled CSR values of `form` (FEMHIP_LAPLACE, FEMHIP_MASS,
+ * FEMHIP_KAPPA_LAPLACE with kappa[nelems] per element) on the pattern the
+ * plan was built for.  p is [dim][nnodes] on the device. */
+int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t form, int32_t dim,
+                          const double* p, int64_t nnodes, const double* kappa,
+                          double* data, femhip_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,359 @@
+// Host-side builder of the fused P1 assembly plan (one-time, per mesh and
+// CSR pattern).
+//
+// The fused kernel assembles P1 forms owner-computes style: rows are cut
+// into partitions of spatially close nodes (Morton order of the node
+// coordinates), one workgroup per partition accumulates the partition's
+// CSR rows in LDS and writes them once.  Every element with at least one
+// owned vertex is visited by that partition (halo elements are recomputed
+// by each partition they touch instead of exchanging partial sums).
+//
+// Determinism without floating-point atomics: the partition's elements are
+// greedily coloured so that two elements of one colour never share an
+// owned vertex, sorted by colour and cut into chunks of at most `block`
+// elements spanning at most 16 colours.  The kernel processes a chunk in
+// rounds, one colour per round with a barrier between rounds, so every LDS
+// slot receives its contributions in a fixed order.
+//
+// Per element visit the plan stores one packed record: the local ids of the
+// element's vertices (partition-local numbering, owned nodes first) and, for
+// every owned vertex a and other vertex b, the offset of column b inside
+// row a of the CSR pattern (a sentinel when the slot was pruned by
+// eliminate_zeros), plus the colour relative to the chunk.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/femhip.h"
+
+namespace femhip {
+void set_error(const std::string& msg);
+}
+
+namespace {
+
+struct Plan {
+  femhip_fused_plan_info_t info;
+  std::vector<int32_t> parts;    // 8 int32 per partition
+  std::vector<int32_t> nodemap;  // global node id per local node
+  std::vector<int32_t> rows;     // 2 int32 per owned row: gstart, packed
+  std::vector<int32_t> chunks;   // 2 int32 per chunk: rec_start, count|nrounds<<16
+  std::vector<uint32_t> recs;    // rec_words uint32 per record
+  std::vector<int32_t> rec_elem; // global element id per record
+};
+
+uint64_t spread3(uint64_t x) {  // 21 bits -> every third bit
+  x &= 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffULL;
+  x = (x | x << 16) & 0x1f0000ff0000ffULL;
+  x = (x | x << 8) & 0x100f00f00f00f00fULL;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+  x = (x | x << 2) & 0x1249249249249249ULL;
+  return x;
+}
+
+uint64_t spread2(uint64_t x) {  // 31 bits -> every second bit
+  x &= 0x7fffffff;
+  x = (x | x << 16) & 0x0000ffff0000ffffULL;
+  x = (x | x << 8) & 0x00ff00ff00ff00ffULL;
+  x = (x | x << 4) & 0x0f0f0f0f0f0f0f0fULL;
+  x = (x | x << 2) & 0x3333333333333333ULL;
+  x = (x | x << 1) & 0x5555555555555555ULL;
+  return x;
+}
+
+// Writes `nbits` of `v` at bit position `pos` of a little-endian word array.
+void put_bits(uint32_t* w, int pos, int nbits, uint32_t v) {
+  for (int i = 0; i < nbits; ++i) {
+    const int bit = pos + i;
+    if ((v >> i) & 1u) w[bit >> 5] |= (1u << (bit & 31));
+  }
+}
+
+}  // namespace
+
+extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, int64_t Nv,
+                                       const int32_t* t, const double* p,
+                                       const int32_t* indptr, const int32_t* indices,
+                                       int32_t slot_cap, int32_t node_cap, int32_t block,
+                                       void** handle_out, femhip_fused_plan_info_t* info_out) {
+  using femhip::set_error;
+  if (!(nverts == 3 && dim == 2) && !(nverts == 4 && dim == 3)) {
+    set_error("fused plan: P1 triangles or tetrahedra only");
+    return FEMHIP_EUNSUPPORTED;
+  }
+  if (!t || !p || !indptr || !indices || !handle_out || !info_out || E < 0 || Nv <= 0 ||
+      slot_cap <= 0 || node_cap <= 0 || block <= 0 || block > 1024) {
+    set_error("fused plan: bad arguments");
+    return FEMHIP_EINVAL;
+  }
+  const int NV = nverts;
+  const int noff = NV * (NV - 1);
+  // record format: local ids and 4- or 5-bit offsets, 4-bit relative colour
+  int max_row = 0;
+  for (int64_t n = 0; n < Nv; ++n) max_row = std::max(max_row, indptr[n + 1] - indptr[n]);
+  int offbits = max_row <= 15 ? 4 : (max_row <= 31 ? 5 : 0);
+  if (offbits == 0) {
+    set_error("fused plan: CSR rows longer than 31 entries");
+    return FEMHIP_EUNSUPPORTED;
+  }
+  const int idbits = (NV == 4) ? (offbits == 4 ? 11 : 16) : (offbits == 4 ? 12 : 16);
+  const int total_bits = NV * idbits + noff * offbits + 4;
+  const int rec_words = (total_bits + 31) / 32;
+  const int id_cap = 1 << idbits;
+  if (node_cap > id_cap) node_cap = id_cap;
+  const uint32_t sentinel = (1u << offbits) - 1;
+
+  // 1. Morton order of the nodes
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int d = 0; d < dim; ++d)
+    for (int64_t n = 0; n < Nv; ++n) {
+      lo[d] = std::min(lo[d], p[d * Nv + n]);
+      hi[d] = std::max(hi[d], p[d * Nv + n]);
+    }
+  std::vector<std::pair<uint64_t, int32_t>> code(Nv);
+  const double qmax = (dim == 3) ? 2097151.0 : 2147483647.0;
+  for (int64_t n = 0; n < Nv; ++n) {
+    uint64_t c = 0;
+    for (int d = 0; d < dim; ++d) {
+      const double span = hi[d] - lo[d];
+      const double u = span > 0 ? (p[d * Nv + n] - lo[d]) / span : 0.0;
+      const uint64_t q = (uint64_t)std::min(qmax, std::max(0.0, u * qmax));
+      c |= (dim == 3 ? spread3(q) : spread2(q)) << d;
+    }
+    code[n] = {c, (int32_t)n};
+  }
+  std::sort(code.begin(), code.end());
+
+  // 2. node -> element incidence
+  std::vector<int64_t> inc_ptr(Nv + 1, 0);
+  for (int64_t e = 0; e < E; ++e)
+    for (int k = 0; k < NV; ++k) {
+      const int32_t v = t[(int64_t)k * E + e];
+      if (v < 0 || v >= Nv) {
+        set_error("fused plan: connectivity index out of range");
+        return FEMHIP_EINVAL;
+      }
+      inc_ptr[v + 1]++;
+    }
+  for (int64_t n = 0; n < Nv; ++n) inc_ptr[n + 1] += inc_ptr[n];
+  std::vector<int32_t> inc(inc_ptr[Nv]);
+  {
+    std::vector<int64_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+    for (int64_t e = 0; e < E; ++e)
+      for (int k = 0; k < NV; ++k) inc[fill[t[(int64_t)k * E + e]]++] = (int32_t)e;
+  }
+
+  Plan* P = new Plan();
+  std::memset(&P->info, 0, sizeof(P->info));
+  std::vector<int32_t> nstamp(Nv, -1), lid(Nv, -1), estamp(E, -1);
+  std::vector<int32_t> owned, halo, elems;
+  std::vector<uint64_t> used;  // per owned local row: colours used
+  std::vector<int32_t> color;
+  int32_t pid = 0;
+  int64_t pos = 0;
+  int max_slots = 0, max_nodes = 0, max_rows = 0, max_colors = 0;
+  int64_t visits = 0;
+
+  while (pos < Nv) {
+    owned.clear();
+    halo.clear();
+    elems.clear();
+    int slots = 0, nloc = 0;
+    // grow the partition along the Morton order
+    while (pos < Nv) {
+      const int32_t n = code[pos].second;
+      const int rl = indptr[n + 1] - indptr[n];
+      // conservative count of the new local nodes this row brings
+      int extra = (nstamp[n] == pid) ? 0 : 1;
+      for (int64_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
+        const int32_t e = inc[k];
+        if (estamp[e] == pid) continue;
+        for (int j = 0; j < NV; ++j)
+          if (nstamp[t[(int64_t)j * E + e]] != pid) ++extra;
+      }
+      if (!owned.empty() &&
+          (slots + rl > slot_cap || nloc + extra > node_cap || (int)owned.size() >= 65535))
+        break;
+      if (owned.empty() && (rl > slot_cap || extra > node_cap)) {
+        delete P;
+        set_error("fused plan: one row does not fit the LDS budget");
+        return FEMHIP_EUNSUPPORTED;
+      }
+      owned.push_back(n);
+      slots += rl;
+      if (nstamp[n] != pid) {
+        nstamp[n] = pid;
+        ++nloc;
+      }
+      for (int64_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
+        const int32_t e = inc[k];
+        if (estamp[e] == pid) continue;
+        estamp[e] = pid;
+        elems.push_back(e);
+        for (int j = 0; j < NV; ++j) {
+          const int32_t v = t[(int64_t)j * E + e];
+          if (nstamp[v] != pid) {
+            nstamp[v] = pid;
+            ++nloc;
+          }
+        }
+      }
+      ++pos;
+    }
+    // local numbering: owned first (partition order), then halo
+    const int R = (int)owned.size();
+    for (int r = 0; r < R; ++r) lid[owned[r]] = r;
+    int L = R;
+    std::vector<int32_t> locals(owned.begin(), owned.end());
+    for (int32_t e : elems)
+      for (int j = 0; j < NV; ++j) {
+        const int32_t v = t[(int64_t)j * E + e];
+        if (lid[v] < 0) {
+          lid[v] = L++;
+          locals.push_back(v);
+        }
+      }
+    // colouring: no two elements of a colour share an owned vertex
+    used.assign(R, 0);
+    color.assign(elems.size(), 0);
+    int ncolors = 0;
+    for (size_t i = 0; i < elems.size(); ++i) {
+      const int32_t e = elems[i];
+      uint64_t forbid = 0;
+      for (int j = 0; j < NV; ++j) {
+        const int l = lid[t[(int64_t)j * E + e]];
+        if (l < R) forbid |= used[l];
+      }
+      if (~forbid == 0) {
+        delete P;
+        set_error("fused plan: more than 64 colours needed");
+        return FEMHIP_EUNSUPPORTED;
+      }
+      const int c = __builtin_ctzll(~forbid);
+      color[i] = c;
+      ncolors = std::max(ncolors, c + 1);
+      for (int j = 0; j < NV; ++j) {
+        const int l = lid[t[(int64_t)j * E + e]];
+        if (l < R) used[l] |= (1ull << c);
+      }
+    }
+    // stable sort by colour
+    std::vector<int32_t> order(elems.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return color[a] < color[b]; });
+    // rows of the partition
+    const int32_t row_start = (int32_t)(P->rows.size() / 2);
+    int lds = 0;
+    std::vector<int> diag_off(R, 255), row_len(R, 0), row_lds(R, 0);
+    for (int r = 0; r < R; ++r) {
+      const int32_t g = owned[r];
+      const int b = indptr[g], en = indptr[g + 1];
+      row_len[r] = en - b;
+      row_lds[r] = lds;
+      for (int k = b; k < en; ++k)
+        if (indices[k] == g) diag_off[r] = k - b;
+      P->rows.push_back(b);
+      P->rows.push_back((int32_t)((uint32_t)lds | ((uint32_t)diag_off[r] << 16) |
+                                  ((uint32_t)row_len[r] << 24)));
+      lds += row_len[r];
+    }
+    // chunks and records
+    const int32_t chunk_start = (int32_t)(P->chunks.size() / 2);
+    size_t i = 0;
+    while (i < order.size()) {
+      const int c0 = color[order[i]];
+      size_t j = i;
+      while (j < order.size() && (int)(j - i) < block && color[order[j]] - c0 < 16) ++j;
+      const int nrounds = color[order[j - 1]] - c0 + 1;
+      const int32_t rec0 = (int32_t)P->rec_elem.size();
+      P->chunks.push_back(rec0);
+      P->chunks.push_back((int32_t)((j - i) | ((size_t)nrounds << 16)));
+      for (size_t q = i; q < j; ++q) {
+        const int32_t e = elems[order[q]];
+        uint32_t w[4] = {0, 0, 0, 0};
+        int32_t gv[4];
+        int lv[4];
+        for (int a = 0; a < NV; ++a) {
+          gv[a] = t[(int64_t)a * E + e];
+          lv[a] = lid[gv[a]];
+          put_bits(w, a * idbits, idbits, (uint32_t)lv[a]);
+        }
+        int bitpos = NV * idbits;
+        for (int a = 0; a < NV; ++a)
+          for (int b = 0; b < NV; ++b) {
+            if (b == a) continue;
+            uint32_t off = sentinel;
+            if (lv[a] < R) {
+              const int32_t g = gv[a];
+              const int32_t* s = indices + indptr[g];
+              const int32_t* f = indices + indptr[g + 1];
+              const int32_t* it = std::lower_bound(s, f, gv[b]);
+              if (it != f && *it == gv[b]) off = (uint32_t)(it - s);
+            }
+            put_bits(w, bitpos, offbits, off);
+            bitpos += offbits;
+          }
+        put_bits(w, bitpos, 4, (uint32_t)(color[order[q]] - c0));
+        for (int k = 0; k < rec_words; ++k) P->recs.push_back(w[k]);
+        P->rec_elem.push_back(e);
+      }
+      i = j;
+    }
+    const int32_t nchunks = (int32_t)(P->chunks.size() / 2) - chunk_start;
+    const int32_t node_start = (int32_t)P->nodemap.size();
+    for (int32_t v : locals) P->nodemap.push_back(v);
+    P->parts.insert(P->parts.end(), {node_start, L, row_start, R, lds, chunk_start, nchunks,
+                                     (int32_t)elems.size()});
+    for (int32_t v : locals) lid[v] = -1;
+    max_slots = std::max(max_slots, lds);
+    max_nodes = std::max(max_nodes, L);
+    max_rows = std::max(max_rows, R);
+    max_colors = std::max(max_colors, ncolors);
+    visits += (int64_t)elems.size();
+    ++pid;
+  }
+  femhip_fused_plan_info_t& I = P->info;
+  I.nparts = pid;
+  I.nrecs = (int64_t)P->rec_elem.size();
+  I.nchunks = (int32_t)(P->chunks.size() / 2);
+  I.nlocal = (int64_t)P->nodemap.size();
+  I.nrows = (int32_t)(P->rows.size() / 2);
+  I.max_slots = max_slots;
+  I.max_nodes = max_nodes;
+  I.max_rows = max_rows;
+  I.rec_words = rec_words;
+  I.idbits = idbits;
+  I.offbits = offbits;
+  I.max_colors = max_colors;
+  I.block = block;
+  I.dup = E > 0 ? (double)visits / (double)E : 0.0;
+  *info_out = I;
+  *handle_out = P;
+  return FEMHIP_OK;
+}
+
+extern "C" int femhip_fused_plan_export(void* handle, int32_t* parts, int32_t* nodemap,
+                                        int32_t* rows, int32_t* chunks, uint32_t* recs,
+                                        int32_t* rec_elem) {
+  if (!handle) {
+    femhip::set_error("fused plan: NULL handle");
+    return FEMHIP_EINVAL;
+  }
+  Plan* P = static_cast<Plan*>(handle);
+  auto cp = [](void* dst, const void* src, size_t bytes) {
+    if (dst && bytes) std::memcpy(dst, src, bytes);
+  };
+  cp(parts, P->parts.data(), P->parts.size() * 4);
+  cp(nodemap, P->nodemap.data(), P->nodemap.size() * 4);
+  cp(rows, P->rows.data(), P->rows.size() * 4);
+  cp(chunks, P->chunks.data(), P->chunks.size() * 4);
+  cp(recs, P->recs.data(), P->recs.size() * 4);
+  cp(rec_elem, P->rec_elem.data(), P->rec_elem.size() * 4);
+  return FEMHIP_OK;
+}
+
+extern "C" void femhip_fused_plan_free(void* handle) { delete static_cast<Plan*>(handle); }

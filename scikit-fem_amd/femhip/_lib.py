@@ -48,6 +48,19 @@ class Form(ctypes.Structure):
                 ('lam', _f64), ('two_mu', _f64)]
 
 
+class FusedInfo(ctypes.Structure):
+    _fields_ = [('nparts', _i32), ('nchunks', _i32), ('nrows', _i32), ('max_slots', _i32),
+                ('max_nodes', _i32), ('max_rows', _i32), ('rec_words', _i32),
+                ('idbits', _i32), ('offbits', _i32), ('max_colors', _i32),
+                ('block', _i32), ('reserved', _i32), ('nrecs', _i64), ('nlocal', _i64),
+                ('dup', _f64)]
+
+
+class FusedDev(ctypes.Structure):
+    _fields_ = [('parts', _p), ('nodemap', _p), ('rows', _p), ('chunks', _p),
+                ('recs', _p), ('rec_elem', _p), ('info', FusedInfo)]
+
+
 class FemhipError(RuntimeError):
     """HIP runtime failure reported by the library."""
 
@@ -73,6 +86,14 @@ SIGNATURES = {
                                               ctypes.c_size_t, _p]),
     'femhip_segment_sum': (ctypes.c_int, [_p, _p, _p, _i64, _p, _p]),
     'femhip_zero_signature': (ctypes.c_int, [_p, _i64, _p, _p]),
+    'femhip_fused_plan_build': (ctypes.c_int, [_i32, _i32, _i64, _i64, _p, _p, _p, _p,
+                                               _i32, _i32, _i32,
+                                               ctypes.POINTER(ctypes.c_void_p),
+                                               ctypes.POINTER(FusedInfo)]),
+    'femhip_fused_plan_export': (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
+    'femhip_fused_plan_free': (None, [_p]),
+    'femhip_fused_assemble': (ctypes.c_int, [ctypes.POINTER(FusedDev), _i32, _i32, _p,
+                                             _i64, _p, _p, _p]),
 }
 
 _lib = None

@@ -98,7 +98,10 @@ class DeviceBasis:
                                 self.N, self.dofs.data_ptr(), self.phi.data_ptr(),
                                 self.dphi.data_ptr())
         self._G = None
-        self.plans = {}
+        self.p_version = 0
+        self.plans = {}          # zero signature -> Plan
+        self.validated = {}      # (form, params, coefficient token, p version) -> Plan
+        self.fused_plans = {}    # zero signature -> FusedPlan
         self.row_plan = None
 
     # -- coordinates ---------------------------------------------------------
@@ -109,6 +112,7 @@ class DeviceBasis:
         self.p = p.contiguous()
         self.geom.p = self.p.data_ptr()
         self._G = None
+        self.p_version += 1
 
     def geometry(self):
         if self._G is None:
@@ -184,15 +188,25 @@ class DeviceBasis:
                                                   _lib.ptr(out), _lib.stream_ptr()))
         return tuple(int(v) for v in out.cpu().tolist())
 
-    def plan_for(self, key, contrib):
-        """Plan whose zero positions match ``contrib`` (built when absent or
-        when the zero signature changed)."""
+    def plan_for(self, contrib):
+        """Plan whose exact-zero positions match ``contrib`` (patterns are
+        shared by every form with the same zero set on this basis)."""
         sig = self.zero_signature(contrib)
-        plan = self.plans.get(key)
-        if plan is None or plan.signature != sig:
+        plan = self.plans.get(sig)
+        if plan is None:
             plan = self.build_plan(contrib, sig)
-            self.plans[key] = plan
+            self.plans[sig] = plan
         return plan
+
+    def fused_plan(self, plan):
+        from .fused import FusedPlan
+        fp = self.fused_plans.get(plan.signature)
+        if fp is None:
+            torch.cuda.current_stream().synchronize()
+            fp = FusedPlan(self.basis.mesh, plan.indptr.cpu().numpy(),
+                           plan.indices.cpu().numpy())
+            self.fused_plans[plan.signature] = fp
+        return fp
 
     def rows_plan(self):
         if self.row_plan is None:
@@ -217,10 +231,33 @@ class DeviceBasis:
         return out
 
     # -- assembly ------------------------------------------------------------
-    def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, key=None):
+    def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, token=None,
+                          path='auto'):
+        """CSR assembly of a built-in form.
+
+        The first call for given inputs runs the exact path (local entries in
+        the reference's arithmetic -> pattern -> segmented sums) and returns
+        its result; it also validates the pattern for these inputs.  Later
+        calls with the same inputs go through the fused kernel when the
+        element/form has one (P1 triangles/tetrahedra), else the exact path.
+        """
+        from . import fused
+        fusable = (path != 'generic' and fused.eligible(self.basis, name)
+                   and (kappa is None or kappa.dim() == 1))
+        vkey = (name, lam, mu, token, self.p_version)
+        if fusable and vkey in self.validated:
+            plan = self.validated[vkey]
+            fp = self.fused_plans[plan.signature]
+            out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
+                              device=self.p.device)[:plan.nnz]
+            fp.assemble(name, self.p, out, kappa)
+            return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N))
         contrib = self.local_bilinear(name, kappa, lam, mu)
-        plan = self.plan_for(key if key is not None else (name, lam, mu), contrib)
+        plan = self.plan_for(contrib)
         data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
+        if fusable:
+            self.fused_plan(plan)
+            self.validated[vkey] = plan
         return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N))
 
     def assemble_linear(self, name='unit_load'):

@@ -46,10 +46,21 @@ class Form:
             raise NotImplementedError("only float64 assembly is accelerated")
 
 
+_KAPPA_CACHE = {}
+
+
 def _kappa_tensor(w, basis, name='kappa'):
     """Coefficient array of ``w[name]`` as a device tensor: (E,) per element
     or (E*Q,) per quadrature point (the reference accepts an (E, Q) array or
-    a DiscreteField, ``form.py:92-121``)."""
+    a DiscreteField, ``form.py:92-121``), and a token identifying its
+    contents (content hash for host arrays, storage and version counter for
+    device tensors).  An (E, Q) array with identical columns is treated as
+    per-element."""
+    kt, token = _kappa_tensor_impl(w, basis, name)
+    return kt, token
+
+
+def _kappa_tensor_impl(w, basis, name):
     if name not in w:
         raise ValueError("Attribute '%s' not found in 'w'." % name)
     k = w[name]
@@ -58,6 +69,7 @@ def _kappa_tensor(w, basis, name='kappa'):
     E, Q = basis.nelems, basis.W.shape[0]
     if isinstance(k, torch.Tensor):
         kt = k.detach().to(dtype=torch.float64, device='cuda')
+        token = ('torch', kt.data_ptr(), int(k._version), tuple(kt.shape))
     else:
         k = np.asarray(k, dtype=np.float64)
         if k.ndim == 1:
@@ -65,13 +77,22 @@ def _kappa_tensor(w, basis, name='kappa'):
                 "1-D coefficient arrays are DOF vectors in the reference "
                 "(Basis.interpolate); pass a per-element (E, 1) or per-point "
                 "(E, Q) array")
-        kt = torch.from_numpy(np.ascontiguousarray(k)).to('cuda')
+        if k.ndim == 2 and k.shape == (E, Q) and Q > 1 and np.all(k == k[:, :1]):
+            k = k[:, :1]
+        k = np.ascontiguousarray(k)
+        import hashlib
+        token = ('np', k.shape, hashlib.blake2b(k.tobytes(), digest_size=16).hexdigest())
+        kt = _KAPPA_CACHE.get(token)
+        if kt is None:
+            kt = torch.from_numpy(k).to('cuda')
+            _KAPPA_CACHE.clear()
+            _KAPPA_CACHE[token] = kt
     if kt.dim() == 2 and kt.shape == (E, 1):
-        return kt.reshape(E).contiguous()
+        return kt.reshape(E).contiguous(), token
     if kt.dim() == 2 and kt.shape == (E, Q):
-        return kt.reshape(E * Q).contiguous()
+        return kt.reshape(E * Q).contiguous(), token
     if kt.dim() == 1 and kt.shape[0] == E:
-        return kt.contiguous()
+        return kt.contiguous(), token
     raise ValueError("Quadrature mismatch: '%s' should have shape (E, Q) = "
                      "(%d, %d) or (E, 1)" % (name, E, Q))
 
@@ -82,22 +103,22 @@ class BilinearForm(Form):
 
     nargs = 3
 
-    def assemble_device(self, ubasis, vbasis=None, **kwargs):
-        """Assemble and keep the result in HBM (``DeviceCSR``)."""
+    def assemble_device(self, ubasis, vbasis=None, path='auto', **kwargs):
+        """Assemble and keep the result in HBM (``DeviceCSR``).  ``path``:
+        'auto' (fused kernel when available), 'generic' (exact path)."""
         self._check_dtype()
         if vbasis is not None and vbasis is not ubasis:
             raise NotImplementedError("different trial and test bases")
         name, scale, params = self.spec()
         dev = ubasis.device()
         if name == 'kappa_laplace':
-            kappa = _kappa_tensor({**self.params, **kwargs}, ubasis,
-                                  params.get('coef', 'kappa'))
-            key = (name, 'kappa', kappa.data_ptr(), getattr(kappa, '_version', 0))
-            A = dev.assemble_bilinear(name, kappa=kappa, key=key)
+            kappa, token = _kappa_tensor({**self.params, **kwargs}, ubasis,
+                                         params.get('coef', 'kappa'))
+            A = dev.assemble_bilinear(name, kappa=kappa, token=token, path=path)
         elif name == 'linear_elasticity':
-            A = dev.assemble_bilinear(name, lam=params['Lambda'], mu=params['Mu'])
+            A = dev.assemble_bilinear(name, lam=params['Lambda'], mu=params['Mu'], path=path)
         else:
-            A = dev.assemble_bilinear(name)
+            A = dev.assemble_bilinear(name, path=path)
         if scale != 1.0:
             A.data = A.data * scale
         return A
@@ -112,8 +133,8 @@ class BilinearForm(Form):
         dev = ubasis.device()
         kappa = None
         if name == 'kappa_laplace':
-            kappa = _kappa_tensor({**self.params, **kwargs}, ubasis,
-                                  params.get('coef', 'kappa'))
+            kappa, _ = _kappa_tensor({**self.params, **kwargs}, ubasis,
+                                     params.get('coef', 'kappa'))
         c = dev.local_bilinear(name, kappa=kappa, lam=params.get('Lambda', 0.0),
                                mu=params.get('Mu', 0.0))
         nb = ubasis.Nbfun

@@ -1,0 +1,109 @@
+"""GPU parity of the fused P1 kernel: the second assembly on the same inputs
+runs the fused kernel; it must give the first (exact-path) call's pattern
+and values within 1e-12, be bitwise repeatable, and match the reference's
+digests at the benchmark sizes."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import femhip
+from femhip.models import laplace, mass, kappa_laplace, unit_load
+from oracle import skfem_np as O
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+TOL = 1e-12
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(kind, n, a):
+    x = np.linspace(0, 1, n)
+    m = (femhip.MeshTri.init_tensor(x, x) if kind == 'tri'
+         else femhip.MeshTet.init_tensor(x, x, x))
+    return m.jittered(a, 1.0 / (n - 1)) if a else m
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize('kind,n,a,form', [
+    ('tri', 33, 0.0, 'laplace'), ('tri', 33, 0.2, 'laplace'), ('tri', 33, 0.2, 'mass'),
+    ('tet', 9, 0.0, 'laplace'), ('tet', 12, 0.2, 'laplace'), ('tet', 9, 0.2, 'mass'),
+    ('tet', 9, 0.2, 'kappa_laplace'), ('tri', 17, 0.0, 'kappa_laplace'),
+])
+def test_fused_matches_exact_and_oracle(kind, n, a, form):
+    m = _mesh(kind, n, a)
+    el = femhip.ElementTriP1() if kind == 'tri' else femhip.ElementTetP1()
+    b = femhip.Basis(m, el)
+    kw = {}
+    if form == 'kappa_laplace':
+        kw['kappa'] = np.random.default_rng(5).lognormal(0, 1, m.nelements)[:, None]
+    f = {'laplace': laplace, 'mass': mass, 'kappa_laplace': kappa_laplace}[form]
+    print('case', kind, n, a, form, flush=True)
+    A1 = f.assemble_device(b, **kw)      # exact path (validates the pattern)
+    A2 = f.assemble_device(b, **kw)      # fused kernel
+    A3 = f.assemble_device(b, **kw)
+    torch.cuda.synchronize()
+    assert b.device().validated, "fused path was not taken"
+    d1, d2, d3 = (A.data.cpu().numpy() for A in (A1, A2, A3))
+    assert np.array_equal(d2, d3), "fused kernel is not bitwise repeatable"
+    om = (O.MeshTri if kind == 'tri' else O.MeshTet)(m.p, m.t)
+    ob = O.CellBasis(om, O.ElementTriP1() if kind == 'tri' else O.ElementTetP1())
+    okw = {}
+    if form == 'kappa_laplace':
+        okw['kappa'] = np.repeat(kw['kappa'], ob.X.shape[1], axis=1)
+    Ar = O.assemble_named(form, ob, **okw)
+    for A, d in ((A1, d1), (A2, d2)):
+        assert np.array_equal(A.indptr.cpu().numpy(), Ar.indptr)
+        assert np.array_equal(A.indices.cpu().numpy(), Ar.indices)
+        assert np.abs(d - Ar.data).max() <= TOL * np.abs(Ar.data).max()
+
+
+def _check_digest(A, dig):
+    assert A.nnz == dig['nnz']
+    ip = A.indptr.cpu().numpy().astype(np.int32)
+    ix = A.indices.cpu().numpy().astype(np.int32)
+    assert _sha(ip) == dig['indptr_sha']
+    assert _sha(ix) == dig['indices_sha']
+    data = A.data.cpu().numpy()
+    slots = np.array(dig['slots'])
+    vals = np.array(dig['values'])
+    scale = dig['max_abs']
+    assert abs(np.abs(data).max() - scale) <= TOL * scale
+    assert np.abs(data[slots] - vals).max() <= TOL * scale
+    assert abs(data.sum() - dig['sum']) <= 1e-9 * max(1.0, abs(dig['sum']))
+
+
+def test_tet101_digest_laplace_mass_load():
+    """The benchmark mesh: 6M jittered tets, 1.03M DoF, against the
+    reference's digests (pattern hashes, sampled values)."""
+    dig = json.load(open(os.path.join(GOLD, 'digest_tet1_n101j.json')))
+    m = _mesh('tet', 101, 0.2)
+    assert _sha(m.p) == dig['p_sha'] and _sha(m.t) == dig['t_sha']
+    b = femhip.Basis(m, femhip.ElementTetP1())
+    for form, f in (('laplace', laplace), ('mass', mass)):
+        print('form', form, flush=True)
+        f.assemble_device(b)
+        A = f.assemble_device(b)          # fused
+        _check_digest(A, dig[form])
+    u = unit_load.assemble(b)
+    d = dig['unit_load']
+    ref = np.array(d['values'])
+    assert np.abs(u[::d['stride']][:len(ref)] - ref).max() <= TOL * np.abs(u).max()
+
+
+def test_tet65_kappa_digest():
+    dig = json.load(open(os.path.join(GOLD, 'digest_tet1_n65.json')))
+    m = _mesh('tet', 65, 0.0)
+    b = femhip.Basis(m, femhip.ElementTetP1())
+    kappa = np.random.default_rng(0).lognormal(0, 1, m.nelements)
+    for rep in range(2):
+        A = kappa_laplace.assemble_device(b, kappa=kappa[:, None])
+    _check_digest(A, dig['kappa_laplace'])
+    for rep in range(2):
+        A = laplace.assemble_device(b)
+    _check_digest(A, dig['laplace'])
