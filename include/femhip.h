@@ -212,6 +212,29 @@ int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t form, int32_t 
                           const double* p, int64_t nnodes, const double* kappa,
                           double* data, femhip_stream_t s);
 
+/* ---------------------------------------------------------------------
+ * Adjoint of the P1 (kappa-)Laplace + unit-load assembly.  The reference
+ * has no adjoint (its autodiff, skfem/autodiff, differentiates with
+ * respect to the solution through JAX); this replaces finite differences
+ * of BilinearForm.assemble / LinearForm.assemble (bilinear_form.py:58-148,
+ * linear_form.py:18-49) with respect to per-element kappa and the node
+ * coordinates Mesh.p.
+ *
+ * femhip_invert_plan: slot_of[c] = CSR slot of contribution c (entries of
+ * contributions not in any slot are left untouched: initialise to -1).
+ * femhip_p1_adjoint: per element, from grad_data[nnz] (via slot_of) and
+ * grad_load[nnodes] (either may be NULL): dkappa[e] (may be NULL) and the
+ * per-vertex coordinate gradients gX[dim][(d+1)*nelems] (contribution
+ * layout a*nelems + e, summed per node with femhip_segment_sum and the rows
+ * plan of the mesh connectivity).  kappa NULL means kappa = 1.
+ * --------------------------------------------------------------------- */
+int femhip_invert_plan(const int32_t* perm, const int32_t* seg, int64_t nseg,
+                       int32_t* slot_of, femhip_stream_t s);
+int femhip_p1_adjoint(int32_t dim, int64_t nelems, int64_t nnodes, const double* p,
+                      const int32_t* t, const double* kappa, const int32_t* slot_of,
+                      const double* grad_data, const double* grad_load, double* dkappa,
+                      double* gX, femhip_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,7 +20,7 @@
 namespace femhip {
 namespace {
 
-constexpr int kFusedBlock = 256;
+constexpr int kFusedBlock = 512;
 
 struct FusedArgs {
   const int32_t* parts;
@@ -64,13 +64,90 @@ __device__ __forceinline__ int xcd_remap(int i, int n) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// Local matrix of a P1 simplex from its vertex coordinates in LDS (SoA,
+// stride ld): Laplace K_ab = coef * (n_a . n_b) / (d! |det|) with n_a the
+// cross-product (2-D: rotated-edge) gradients scaled by det; mass
+// |det| / (d! (d+1)(d+2)) * (1 + delta_ab).
+template <int D, int FORM>
+__device__ __forceinline__ void element_matrix(const double* X, int ld, const int (&lv)[D + 1],
+                                               double coef, double (&K)[D + 1][D + 1]) {
+  double x[D + 1][D];
+#pragma unroll
+  for (int v = 0; v < D + 1; ++v)
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[v][d] = X[d * ld + lv[v]];
+  if constexpr (D == 3) {
+    double c1[3], c2[3], c3[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      c1[d] = x[1][d] - x[0][d];
+      c2[d] = x[2][d] - x[0][d];
+      c3[d] = x[3][d] - x[0][d];
+    }
+    double n[4][3];
+    n[1][0] = c2[1] * c3[2] - c2[2] * c3[1];
+    n[1][1] = c2[2] * c3[0] - c2[0] * c3[2];
+    n[1][2] = c2[0] * c3[1] - c2[1] * c3[0];
+    n[2][0] = c3[1] * c1[2] - c3[2] * c1[1];
+    n[2][1] = c3[2] * c1[0] - c3[0] * c1[2];
+    n[2][2] = c3[0] * c1[1] - c3[1] * c1[0];
+    n[3][0] = c1[1] * c2[2] - c1[2] * c2[1];
+    n[3][1] = c1[2] * c2[0] - c1[0] * c2[2];
+    n[3][2] = c1[0] * c2[1] - c1[1] * c2[0];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) n[0][d] = -(n[1][d] + n[2][d] + n[3][d]);
+    const double det = c1[0] * n[1][0] + c1[1] * n[1][1] + c1[2] * n[1][2];
+    if constexpr (FORM == FEMHIP_MASS) {
+      const double m = fabs(det) * (1.0 / 120.0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
+    } else {
+      const double s = coef / (6.0 * fabs(det));
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j) {
+          const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1] + n[i][2] * n[j][2]);
+          K[i][j] = v;
+          K[j][i] = v;
+        }
+    }
+  } else {
+    const double c1x = x[1][0] - x[0][0], c1y = x[1][1] - x[0][1];
+    const double c2x = x[2][0] - x[0][0], c2y = x[2][1] - x[0][1];
+    const double det = c1x * c2y - c1y * c2x;
+    double n[3][2] = {{0, 0}, {c2y, -c2x}, {-c1y, c1x}};
+    n[0][0] = -(n[1][0] + n[2][0]);
+    n[0][1] = -(n[1][1] + n[2][1]);
+    if constexpr (FORM == FEMHIP_MASS) {
+      const double m = fabs(det) * (1.0 / 24.0);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
+    } else {
+      const double s = coef / (2.0 * fabs(det));
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+          const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1]);
+          K[i][j] = v;
+          K[j][i] = v;
+        }
+    }
+  }
+}
+
 template <int D, int RW, int IDB, int OFB, int FORM>
 __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
   constexpr int NV = D + 1;
   constexpr uint32_t SENT = (1u << OFB) - 1u;
   extern __shared__ double smem[];
   double* acc = smem;
-  double* X = acc + a.max_slots;
+  double* X = acc + a.max_slots + 2;
   uint32_t* rinfo = reinterpret_cast<uint32_t*>(X + D * a.max_nodes);
 
   const int part = xcd_remap(blockIdx.x, a.nparts);
@@ -87,117 +164,77 @@ __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
   }
   for (int r = tid; r < R; r += kFusedBlock)
     rinfo[r] = (uint32_t)a.rows[2 * (int64_t)(row_start + r) + 1];
+  __syncthreads();  // staged coordinates are read before the first round
+
+  // records of the next chunk are fetched while the current one is summed
+  uint32_t wn[RW];
+  int32_t rec_n = 0;
+  uint32_t cw_n = 0;
+  auto fetch = [&](int c) {
+    rec_n = a.chunks[2 * (int64_t)(chunk_start + c)];
+    cw_n = (uint32_t)a.chunks[2 * (int64_t)(chunk_start + c) + 1];
+    if (tid < (int)(cw_n & 0xffffu)) {
+      const uint32_t* rp = a.recs + (int64_t)(rec_n + tid) * RW;
+#pragma unroll
+      for (int k = 0; k < RW; ++k) wn[k] = __builtin_nontemporal_load(rp + k);
+    }
+  };
+  if (nchunks > 0) fetch(0);
+  double* const dummy = acc + a.max_slots;  // sink for skipped contributions
 
   for (int c = 0; c < nchunks; ++c) {
-    const int32_t rec0 = a.chunks[2 * (int64_t)(chunk_start + c)];
-    const uint32_t cw = (uint32_t)a.chunks[2 * (int64_t)(chunk_start + c) + 1];
+    const int32_t rec0 = rec_n;
+    const uint32_t cw = cw_n;
     const int count = (int)(cw & 0xffffu), nrounds = (int)(cw >> 16);
     const bool active = tid < count;
-    int lv[NV];
     uint32_t w[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) w[k] = wn[k];
+    if (c + 1 < nchunks) fetch(c + 1);
+    int lv[NV];
     uint32_t offs[NV * (NV - 1)];
     double K[NV][NV];
     int rc = -1;
     if (active) {
-      const uint32_t* rp = a.recs + (int64_t)(rec0 + tid) * RW;
-#pragma unroll
-      for (int k = 0; k < RW; ++k) w[k] = __builtin_nontemporal_load(rp + k);
       lv[0] = bits<0, IDB, RW>(w);
       lv[1] = bits<IDB, IDB, RW>(w);
       lv[2] = bits<2 * IDB, IDB, RW>(w);
       if constexpr (NV == 4) lv[3] = bits<3 * IDB, IDB, RW>(w);
       rc = (int)bits<NV * IDB + NV * (NV - 1) * OFB, 4, RW>(w);
       decode_offsets<NV * IDB, OFB, RW>(w, offs, std::make_integer_sequence<int, NV * (NV - 1)>{});
-      double x[NV][D];
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-#pragma unroll
-        for (int d = 0; d < D; ++d) x[v][d] = X[d * a.max_nodes + lv[v]];
-      if constexpr (D == 3) {
-        double c1[3], c2[3], c3[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          c1[d] = x[1][d] - x[0][d];
-          c2[d] = x[2][d] - x[0][d];
-          c3[d] = x[3][d] - x[0][d];
-        }
-        double n[4][3];
-        n[1][0] = c2[1] * c3[2] - c2[2] * c3[1];
-        n[1][1] = c2[2] * c3[0] - c2[0] * c3[2];
-        n[1][2] = c2[0] * c3[1] - c2[1] * c3[0];
-        n[2][0] = c3[1] * c1[2] - c3[2] * c1[1];
-        n[2][1] = c3[2] * c1[0] - c3[0] * c1[2];
-        n[2][2] = c3[0] * c1[1] - c3[1] * c1[0];
-        n[3][0] = c1[1] * c2[2] - c1[2] * c2[1];
-        n[3][1] = c1[2] * c2[0] - c1[0] * c2[2];
-        n[3][2] = c1[0] * c2[1] - c1[1] * c2[0];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) n[0][d] = -(n[1][d] + n[2][d] + n[3][d]);
-        const double det = c1[0] * n[1][0] + c1[1] * n[1][1] + c1[2] * n[1][2];
-        if constexpr (FORM == FEMHIP_MASS) {
-          const double m = fabs(det) * (1.0 / 120.0);
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
-        } else {
-          double s = 1.0 / (6.0 * fabs(det));
-          if constexpr (FORM == FEMHIP_KAPPA_LAPLACE) s *= a.kappa[a.rec_elem[rec0 + tid]];
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = i; j < 4; ++j) {
-              const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1] + n[i][2] * n[j][2]);
-              K[i][j] = v;
-              K[j][i] = v;
-            }
-        }
-      } else {
-        const double c1x = x[1][0] - x[0][0], c1y = x[1][1] - x[0][1];
-        const double c2x = x[2][0] - x[0][0], c2y = x[2][1] - x[0][1];
-        const double det = c1x * c2y - c1y * c2x;
-        double n[3][2] = {{0, 0}, {c2y, -c2x}, {-c1y, c1x}};
-        n[0][0] = -(n[1][0] + n[2][0]);
-        n[0][1] = -(n[1][1] + n[2][1]);
-        if constexpr (FORM == FEMHIP_MASS) {
-          const double m = fabs(det) * (1.0 / 24.0);
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
-        } else {
-          double s = 1.0 / (2.0 * fabs(det));
-          if constexpr (FORM == FEMHIP_KAPPA_LAPLACE) s *= a.kappa[a.rec_elem[rec0 + tid]];
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = i; j < 3; ++j) {
-              const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1]);
-              K[i][j] = v;
-              K[j][i] = v;
-            }
-        }
-      }
+      element_matrix<D, FORM>(X, a.max_nodes, lv, FORM == FEMHIP_KAPPA_LAPLACE ? a.kappa[a.rec_elem[rec0 + tid]] : 1.0, K);
     }
     for (int r = 0; r < nrounds; ++r) {
       __syncthreads();
       if (rc == r) {
+        // all slot addresses first, then all loads, then all stores: the
+        // slots of one element are distinct, so nothing aliases
+        double* ptr[NV * NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
-          if (lv[i] >= R) continue;
-          const uint32_t info = rinfo[lv[i]];
+          const bool own = lv[i] < R;
+          const uint32_t info = own ? rinfo[lv[i]] : 0u;
           double* row = acc + (info & 0xffffu);
           const uint32_t dg = (info >> 16) & 0xffu;
-          if (dg != 0xffu) row[dg] += K[i][i];
+          ptr[i * NV + i] = (own && dg != 0xffu) ? row + dg : dummy;
           int q = 0;
 #pragma unroll
           for (int j = 0; j < NV; ++j) {
             if (j == i) continue;
             const uint32_t off = offs[i * (NV - 1) + q];
             ++q;
-            if (off != SENT) row[off] += K[i][j];
+            ptr[i * NV + j] = (own && off != SENT) ? row + off : dummy;
           }
         }
+        double v[NV * NV];
+#pragma unroll
+        for (int k = 0; k < NV * NV; ++k) v[k] = *ptr[k];
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+          for (int j = 0; j < NV; ++j) v[i * NV + j] += K[i][j];
+#pragma unroll
+        for (int k = 0; k < NV * NV; ++k) *ptr[k] = v[k];
       }
     }
   }
@@ -251,7 +288,7 @@ extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t for
   if (I.nparts == 0) return FEMHIP_OK;
   FusedArgs a{plan->parts, plan->nodemap, plan->rows, plan->chunks, plan->recs,
               plan->rec_elem, p, kappa, data, nnodes, I.nparts, I.max_slots, I.max_nodes};
-  const size_t lds = 8 * (size_t)I.max_slots + 8 * (size_t)dim * I.max_nodes +
+  const size_t lds = 8 * ((size_t)I.max_slots + 2) + 8 * (size_t)dim * I.max_nodes +
                      4 * (size_t)I.max_rows;
   FEMHIP_REQUIRE(lds <= 160 * 1024, FEMHIP_EINVAL, "fused: plan exceeds LDS");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);

@@ -1,0 +1,105 @@
+"""Differentiable P1 assembly: ``torch.autograd.Function`` whose backward is
+the hand-written adjoint kernel (``femhip_p1_adjoint``).
+
+    data, b, (indptr, indices) = kappa_laplace_with_load(basis, kappa, p)
+
+``data`` holds the kappa-Laplace CSR values on the *connectivity* pattern
+of the mesh (every (row, col) pair coupled by an element, no exact-zero
+pruning: a slot that happens to be 0 for these coordinates still has a
+nonzero derivative with respect to them), ``b`` the unit-load vector.  The
+backward maps (dL/ddata, dL/db) to dL/dkappa (per element) and dL/dp (node
+coordinates), recomputing the element geometry instead of storing it; node
+sums use the deterministic rows plan (no atomics).
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from . import fused as _fused
+
+
+class _Ctx:
+    """Per-basis state of the differentiable path (connectivity plan, its
+    fused plan, slot map and rows plan)."""
+
+    def __init__(self, basis):
+        d = basis.device()
+        if not (_fused.eligible(basis, 'kappa_laplace')):
+            raise NotImplementedError("the adjoint supports P1 triangles/tetrahedra")
+        self.d = d
+        self.plan = d.build_plan(None)            # connectivity pattern
+        self.fp = _fused.FusedPlan(basis.mesh, self.plan.indptr.cpu().numpy(),
+                                   self.plan.indices.cpu().numpy())
+        C = d.nbfun * d.nbfun * d.E
+        self.slot_of = torch.full((C,), -1, dtype=torch.int32, device=d.p.device)
+        _lib.check(d.lib.femhip_invert_plan(_lib.ptr(self.plan.perm), _lib.ptr(self.plan.seg),
+                                            self.plan.nnz, _lib.ptr(self.slot_of),
+                                            _lib.stream_ptr()))
+        self.rows = d.rows_plan()
+
+
+def _ctx(basis):
+    if getattr(basis, '_adjoint_ctx', None) is None:
+        basis._adjoint_ctx = _Ctx(basis)
+    return basis._adjoint_ctx
+
+
+class _KappaLaplaceLoad(torch.autograd.Function):
+
+    @staticmethod
+    def forward(ctx, kappa, p, basis):
+        c = _ctx(basis)
+        d = c.d
+        kappa = kappa.detach().contiguous()
+        p = p.detach().contiguous()
+        out = torch.empty(c.plan.nnz, dtype=torch.float64, device=p.device)
+        c.fp.assemble('kappa_laplace', p, out, kappa)
+        # unit load on the current coordinates (exact path, deterministic)
+        if p.data_ptr() != d.p.data_ptr():
+            d.set_coordinates(p)
+        b = d.assemble_linear('unit_load')
+        ctx.save_for_backward(kappa, p)
+        ctx.basis = basis
+        return out, b
+
+    @staticmethod
+    def backward(ctx, gdata, gload):
+        kappa, p = ctx.saved_tensors
+        c = _ctx(ctx.basis)
+        d = c.d
+        E, nv, dim = d.E, d.nbfun, d.dim
+        dev = p.device
+        gdata = None if gdata is None else gdata.contiguous()
+        gload = None if gload is None else gload.contiguous()
+        dk = torch.empty(E, dtype=torch.float64, device=dev)
+        gX = torch.empty(dim * nv * E, dtype=torch.float64, device=dev)
+        _lib.check(d.lib.femhip_p1_adjoint(
+            dim, E, p.shape[1], _lib.ptr(p), _lib.ptr(d.t), _lib.ptr(kappa),
+            _lib.ptr(c.slot_of), _lib.ptr(gdata), _lib.ptr(gload), _lib.ptr(dk),
+            _lib.ptr(gX), _lib.stream_ptr()))
+        perm, seg = c.rows
+        dX = torch.empty_like(p)
+        for i in range(dim):
+            d.segment_sum(gX[i * nv * E:(i + 1) * nv * E], perm, seg, p.shape[1], out=dX[i])
+        return dk if ctx.needs_input_grad[0] else None, \
+            dX if ctx.needs_input_grad[1] else None, None
+
+
+def kappa_laplace_with_load(basis, kappa, p=None):
+    """Differentiable (CSR values, load vector) of kappa-Laplace + unit load
+    on a P1 basis.  ``kappa``: (E,) float64 cuda tensor; ``p``: (dim, Nv)
+    float64 cuda tensor (defaults to the mesh coordinates)."""
+    c = _ctx(basis)
+    if p is None:
+        p = c.d.p
+    data, b = _KappaLaplaceLoad.apply(kappa, p, basis)
+    return data, b, (c.plan.indptr, c.plan.indices)
+
+
+def quadratic_loss(data, indptr, indices, u, f):
+    """L = 1/2 u^T A u - f^T u for A given by CSR arrays (torch ops)."""
+    n = indptr.numel() - 1
+    counts = (indptr[1:] - indptr[:-1]).long()
+    rows = torch.repeat_interleave(torch.arange(n, device=u.device), counts)
+    return 0.5 * (data * u[rows] * u[indices.long()]).sum() - (f * u).sum()

@@ -16,7 +16,7 @@ from . import _lib
 # LDS budget per workgroup: two workgroups per CU (2 x 80 KB of 160 KB)
 DEFAULT_SLOT_CAP = 6144
 DEFAULT_NODE_CAP = 1024
-BLOCK = 256
+BLOCK = 512
 
 _FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS, 'kappa_laplace': _lib.KAPPA_LAPLACE}
 
