@@ -207,10 +207,23 @@ typedef struct femhip_fused_dev {
 
 /* data[nnz] <- assembled CSR values of `form` (FEMHIP_LAPLACE, FEMHIP_MASS,
  * FEMHIP_KAPPA_LAPLACE with kappa[nelems] per element) on the pattern the
- * plan was built for.  p is [dim][nnodes] on the device. */
+ * plan was built for; if load != NULL also load[nnodes] <- the unit-load
+ * vector (LinearForm unit_load, linear_form.py:18-49) in the same pass.
+ * p is [dim][nnodes] on the device.  groups = number of persistent
+ * workgroups (0: one per partition).  A plan may concatenate several meshes
+ * (femhip.batch): then p, data and load are the concatenated arrays. */
 int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t form, int32_t dim,
                           const double* p, int64_t nnodes, const double* kappa,
-                          double* data, femhip_stream_t s);
+                          double* data, double* load, int32_t groups, femhip_stream_t s);
+
+/* out[S][nseg] <- kappa-weighted segmented sums  sum_k kappa[s][c_k % E] *
+ * K1[c_k]  over the plan (perm, seg): many per-element coefficient samples
+ * (kappa[S][E]) assembled on one shared pattern from the kappa = 1 local
+ * entries K1.  Batched counterpart of BilinearForm.assemble
+ * (bilinear_form.py:130-148) called once per sample. */
+int femhip_batched_kappa_values(const double* K1, const int32_t* perm, const int32_t* seg,
+                                int64_t nseg, int64_t nelems, const double* kappa,
+                                int64_t nsamples, double* out, femhip_stream_t s);
 
 /* ---------------------------------------------------------------------
  * Adjoint of the P1 (kappa-)Laplace + unit-load assembly.  The reference
@@ -234,6 +247,23 @@ int femhip_p1_adjoint(int32_t dim, int64_t nelems, int64_t nnodes, const double*
                       const int32_t* t, const double* kappa, const int32_t* slot_of,
                       const double* grad_data, const double* grad_load, double* dkappa,
                       double* gX, femhip_stream_t s);
+
+/* ---------------------------------------------------------------------
+ * Vector Q2 hexahedron linear elasticity on FP64 matrix cores.  Numeric
+ * counterpart of BilinearForm._assemble for linear_elasticity(lam, mu) on
+ * ElementVector(ElementHex2) with 27 Gauss points
+ * (bilinear_form.py:58-128, models/elasticity.py:35-53): out[e][j*81 + i]
+ * (element-major; j trial, i test) from the mapping G of femhip_geometry.
+ * groups: persistent workgroups (0 = automatic).
+ * femhip_segment_sum_em: the segmented sums of femhip_segment_sum for
+ * element-major local matrices (nb2 = nbfun^2 entries per element).
+ * --------------------------------------------------------------------- */
+int femhip_hex2_elasticity_mfma(const femhip_geom_t* g, const femhip_space_t* sp, double lam,
+                                double mu, const double* G, double* out, int32_t groups,
+                                femhip_stream_t s);
+int femhip_segment_sum_em(const double* loc, const int32_t* perm, const int32_t* seg,
+                          int64_t nseg, int64_t nelems, int64_t nb2, double* out,
+                          femhip_stream_t s);
 
 #ifdef __cplusplus
 }

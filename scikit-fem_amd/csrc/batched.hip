@@ -1,0 +1,65 @@
+// Batched assembly of many per-element coefficient samples on one mesh.
+//
+// For a form linear in a per-element coefficient (kappa-Laplace), the CSR
+// values of sample s are  out[s][slot] = sum_k kappa[s][e_k] * K1[c_k]
+// over the contributions c_k of the slot (in the symbolic plan's fixed
+// order), with K1 the local entries at kappa = 1 and e_k = c_k mod E.  All
+// samples share one pattern.  A workgroup owns a tile of 64 consecutive
+// slots and TS samples at a time: lanes run over slots (coalesced stores of
+// out[s][slot..]), the contribution list of a slot is read once per sample
+// tile, the TS accumulators stay in registers, and each slot is summed in
+// plan order (deterministic).
+#include "common.h"
+
+namespace femhip {
+namespace {
+
+constexpr int kSlots = 64;   // slots per workgroup (one wave wide)
+constexpr int kGroups = 4;   // sample groups per workgroup
+constexpr int kTS = 8;       // samples per thread and pass
+
+__global__ void __launch_bounds__(kSlots * kGroups)
+batched_kappa_kernel(const double* __restrict__ K1, const int32_t* __restrict__ perm,
+                     const int32_t* __restrict__ seg, int64_t nseg, int64_t E,
+                     const double* __restrict__ kappa, int64_t S,
+                     double* __restrict__ out) {
+  const int64_t slot = (int64_t)blockIdx.x * kSlots + (threadIdx.x % kSlots);
+  const int group = threadIdx.x / kSlots;
+  if (slot >= nseg) return;
+  const int32_t b = seg[slot], en = seg[slot + 1];
+  for (int64_t s0 = (int64_t)group * kTS; s0 < S; s0 += (int64_t)kGroups * kTS) {
+    double acc[kTS];
+#pragma unroll
+    for (int j = 0; j < kTS; ++j) acc[j] = 0.0;
+    for (int32_t k = b; k < en; ++k) {
+      const int64_t c = perm[k];
+      const double w = K1[c];
+      const int64_t e = c % E;
+#pragma unroll
+      for (int j = 0; j < kTS; ++j)
+        if (s0 + j < S) acc[j] += kappa[(s0 + j) * E + e] * w;
+    }
+#pragma unroll
+    for (int j = 0; j < kTS; ++j)
+      if (s0 + j < S) __builtin_nontemporal_store(acc[j], out + (s0 + j) * nseg + slot);
+  }
+}
+
+}  // namespace
+}  // namespace femhip
+
+using namespace femhip;
+
+extern "C" int femhip_batched_kappa_values(const double* K1, const int32_t* perm,
+                                           const int32_t* seg, int64_t nseg, int64_t nelems,
+                                           const double* kappa, int64_t nsamples, double* out,
+                                           femhip_stream_t s) {
+  FEMHIP_REQUIRE(nseg >= 0 && nelems > 0 && nsamples >= 0, FEMHIP_EINVAL, "batched: bad sizes");
+  if (nseg == 0 || nsamples == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(K1 && perm && seg && kappa && out, FEMHIP_EINVAL, "batched: NULL argument");
+  batched_kappa_kernel<<<(unsigned)ceil_div(nseg, kSlots), kSlots * kGroups, 0,
+                         reinterpret_cast<hipStream_t>(s)>>>(K1, perm, seg, nseg, nelems,
+                                                             kappa, nsamples, out);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}

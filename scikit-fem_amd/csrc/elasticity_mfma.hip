@@ -1,0 +1,161 @@
+// Local matrices of linear elasticity on ElementVector(ElementHex2) with
+// FP64 matrix cores (v_mfma_f64_16x16x4f64).
+//
+// With g^a_k(q) the global gradients of the 27 scalar Q2 functions and
+// omega_q = w_q |det J_q|, the 81 x 81 local matrix of
+// ddot(C(sym_grad u), sym_grad v) (models/elasticity.py:35-53) for test
+// function (a, c) and trial function (b, d) is
+//     K[(a,c),(b,d)] = lam S_cd[a,b] + mu S_dc[a,b] + mu delta_cd tr(S)[a,b]
+// with S_kl = G_k^T diag(omega) G_l, G_k = [g^a_k(q)] (27 points x 27
+// functions).  The six products S_kl (k <= l; S_lk = S_kl^T) are 27 x 27 x 27
+// GEMMs, padded to 32 x 32 x 28 and computed as 16 x 16 tiles on MFMA; the
+// gradient tables are built per element in LDS from the mapping (invDF,
+// |detDF| per point, from femhip_geometry) and the reference gradients.
+//
+// One workgroup (4 waves) per element at a time (grid-stride over
+// elements); each wave owns 6 of the 24 output tiles.  The local matrix is
+// written element-major, out[e][j*81 + i] (j trial, i test), contiguous per
+// element.
+#include "common.h"
+
+namespace femhip {
+namespace {
+
+constexpr int kNB = 27;    // scalar Q2 functions
+constexpr int kQ = 27;     // 3 x 3 x 3 Gauss points
+constexpr int kQP = 28;    // points padded to a multiple of 4 (MFMA k)
+constexpr int kBP = 32;    // functions padded to two 16-wide tiles
+constexpr int kThreads = 256;
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(kThreads)
+hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restrict__ W,
+                            const double* __restrict__ dphi, int64_t E, double lam,
+                            double mu, double* __restrict__ out) {
+  __shared__ double s_dphi[kNB * 3 * kQ];           // reference gradients
+  __shared__ double s_map[kQ * 10];                 // invDF (9) + omega per point
+  __shared__ double s_ga[3][kBP][kQP];              // G_k^T: [k][function][point]
+  __shared__ double s_gb[3][kQP][kBP];              // diag(omega) G_l: [l][point][fn]
+  __shared__ double s_S[6][kBP][kBP];               // S_kl, k <= l
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int64_t EQ = E * kQ;
+
+  for (int i = tid; i < kNB * 3 * kQ; i += kThreads) s_dphi[i] = dphi[i];
+  // zero padding of the operand tables once (never overwritten)
+  for (int i = tid; i < 3 * kBP * kQP; i += kThreads) {
+    (&s_ga[0][0][0])[i] = 0.0;
+    (&s_gb[0][0][0])[i] = 0.0;
+  }
+
+  for (int64_t e = blockIdx.x; e < E; e += gridDim.x) {
+    __syncthreads();
+    for (int i = tid; i < kQ * 10; i += kThreads) {
+      const int q = i / 10, c = i % 10;
+      double v = G[(int64_t)c * EQ + e * kQ + q];
+      if (c == 9) v *= W[q];
+      s_map[i] = v;
+    }
+    __syncthreads();
+    // global gradients g^a_k(q) = sum_i invDF[i][k] dphi[a][i][q]
+    for (int i = tid; i < 3 * kNB * kQ; i += kThreads) {
+      const int k = i / (kNB * kQ);
+      const int a = (i / kQ) % kNB;
+      const int q = i % kQ;
+      const double* m = s_map + q * 10;
+      const double g = m[0 * 3 + k] * s_dphi[(a * 3 + 0) * kQ + q] +
+                       m[1 * 3 + k] * s_dphi[(a * 3 + 1) * kQ + q] +
+                       m[2 * 3 + k] * s_dphi[(a * 3 + 2) * kQ + q];
+      s_ga[k][a][q] = g;
+      s_gb[k][q][a] = g * m[9];
+    }
+    __syncthreads();
+    // 6 products x 4 tiles = 24 tiles, 6 per wave
+    for (int tile = wave; tile < 24; tile += 4) {
+      const int kl = tile >> 2, ti = (tile >> 1) & 1, tj = tile & 1;
+      const int k = kl < 3 ? kl : (kl == 3 ? 0 : (kl == 4 ? 0 : 1));
+      const int l = kl < 3 ? kl : (kl == 3 ? 1 : 2);
+      double4_t acc = {0.0, 0.0, 0.0, 0.0};
+      const int r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+      for (int q0 = 0; q0 < kQP; q0 += 4) {
+        const double a = s_ga[k][ti * 16 + r16][q0 + k4];   // A[i][kk]
+        const double b = s_gb[l][q0 + k4][tj * 16 + r16];   // B[kk][j]
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s_S[kl][ti * 16 + 4 * k4 + r][tj * 16 + r16] = acc[r];
+    }
+    __syncthreads();
+    // 81 x 81 entries, element-major, index j*81 + i (j trial, i test)
+    double* o = out + e * (int64_t)(81 * 81);
+    for (int idx = tid; idx < 81 * 81; idx += kThreads) {
+      const int j = idx / 81, i = idx - j * 81;
+      const int a = i / 3, c = i - 3 * a, b = j / 3, d = j - 3 * b;
+      auto S = [&](int p, int r, int x, int y) -> double {
+        // S_pr[x][y] with S_rp = S_pr^T
+        if (p <= r) {
+          const int kl = p == r ? p : (p == 0 ? (r == 1 ? 3 : 4) : 5);
+          return s_S[kl][x][y];
+        }
+        const int kl = (r == 0 ? (p == 1 ? 3 : 4) : 5);
+        return s_S[kl][y][x];
+      };
+      double v = lam * S(c, d, a, b) + mu * S(d, c, a, b);
+      if (c == d) v += mu * (s_S[0][a][b] + s_S[1][a][b] + s_S[2][a][b]);
+      __builtin_nontemporal_store(v, o + idx);
+    }
+  }
+}
+
+// out[k] = sum of loc[(c % E) * nb2 + c / E] over the plan's contributions
+// c = perm[j], j in [seg[k], seg[k+1]) (element-major local matrices).
+__global__ void __launch_bounds__(256)
+segment_sum_em_kernel(const double* __restrict__ loc, const int32_t* __restrict__ perm,
+                      const int32_t* __restrict__ seg, int64_t nseg, int64_t E, int64_t nb2,
+                      double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= nseg) return;
+  double acc = 0.0;
+  for (int32_t j = seg[k]; j < seg[k + 1]; ++j) {
+    const int64_t c = perm[j];
+    const int64_t e = c % E, entry = c / E;
+    acc += loc[e * nb2 + entry];
+  }
+  out[k] = acc;
+}
+
+}  // namespace
+}  // namespace femhip
+
+using namespace femhip;
+
+extern "C" int femhip_hex2_elasticity_mfma(const femhip_geom_t* g, const femhip_space_t* sp,
+                                           double lam, double mu, const double* G,
+                                           double* out, int32_t groups, femhip_stream_t s) {
+  FEMHIP_REQUIRE(g && sp && G && out, FEMHIP_EINVAL, "hex2 mfma: NULL argument");
+  FEMHIP_REQUIRE(g->kind == FEMHIP_HEX && g->nq == kQ && sp->nbs == kNB && sp->vector,
+                 FEMHIP_EUNSUPPORTED,
+                 "hex2 mfma: ElementVector(ElementHex2) with 27 quadrature points only");
+  if (g->nelems == 0) return FEMHIP_OK;
+  const int64_t grid = groups > 0 ? groups : (g->nelems < 4096 ? g->nelems : 4096);
+  hex2_elasticity_mfma_kernel<<<(unsigned)grid, kThreads, 0,
+                                reinterpret_cast<hipStream_t>(s)>>>(
+      G, g->W, sp->dphi, g->nelems, lam, mu, out);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+extern "C" int femhip_segment_sum_em(const double* loc, const int32_t* perm, const int32_t* seg,
+                                     int64_t nseg, int64_t nelems, int64_t nb2, double* out,
+                                     femhip_stream_t s) {
+  FEMHIP_REQUIRE(nseg >= 0 && nelems > 0 && nb2 > 0, FEMHIP_EINVAL, "bad sizes");
+  if (nseg == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(loc && perm && seg && out, FEMHIP_EINVAL, "NULL argument");
+  segment_sum_em_kernel<<<(unsigned)ceil_div(nseg, 256), 256, 0,
+                          reinterpret_cast<hipStream_t>(s)>>>(loc, perm, seg, nseg, nelems,
+                                                              nb2, out);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}

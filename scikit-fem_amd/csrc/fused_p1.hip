@@ -32,10 +32,12 @@ struct FusedArgs {
   const double* p;
   const double* kappa;
   double* out;
+  double* load;      // optional: unit-load vector, one entry per node
   int64_t nnodes;
   int32_t nparts;
   int32_t max_slots;
   int32_t max_nodes;
+  int32_t max_rows;
 };
 
 template <int POS, int NB, int RW>
@@ -64,18 +66,21 @@ __device__ __forceinline__ int xcd_remap(int i, int n) {
   return x * q + (x < r ? x : r) + j;
 }
 
-// Local matrix of a P1 simplex from its vertex coordinates in LDS (SoA,
-// stride ld): Laplace K_ab = coef * (n_a . n_b) / (d! |det|) with n_a the
+// Local matrix of a P1 simplex from its vertex coordinates in LDS: Laplace K_ab = coef * (n_a . n_b) / (d! |det|) with n_a the
 // cross-product (2-D: rotated-edge) gradients scaled by det; mass
 // |det| / (d! (d+1)(d+2)) * (1 + delta_ab).
 template <int D, int FORM>
-__device__ __forceinline__ void element_matrix(const double* X, int ld, const int (&lv)[D + 1],
-                                               double coef, double (&K)[D + 1][D + 1]) {
+__device__ __forceinline__ double element_matrix(const double* X, const int (&lv)[D + 1],
+                                                 double coef, double (&K)[D + 1][D + 1]) {
+  // coordinates are staged AoS, 32 B (3-D: x y z pad) or 16 B per node
   double x[D + 1][D];
 #pragma unroll
-  for (int v = 0; v < D + 1; ++v)
-#pragma unroll
-    for (int d = 0; d < D; ++d) x[v][d] = X[d * ld + lv[v]];
+  for (int v = 0; v < D + 1; ++v) {
+    const double2 xy = reinterpret_cast<const double2*>(X)[(D == 3 ? 2 : 1) * lv[v]];
+    x[v][0] = xy.x;
+    x[v][1] = xy.y;
+    if constexpr (D == 3) x[v][2] = X[4 * lv[v] + 2];
+  }
   if constexpr (D == 3) {
     double c1[3], c2[3], c3[3];
 #pragma unroll
@@ -114,6 +119,7 @@ __device__ __forceinline__ void element_matrix(const double* X, int ld, const in
           K[j][i] = v;
         }
     }
+    return fabs(det);
   } else {
     const double c1x = x[1][0] - x[0][0], c1y = x[1][1] - x[0][1];
     const double c2x = x[2][0] - x[0][0], c2y = x[2][1] - x[0][1];
@@ -138,32 +144,40 @@ __device__ __forceinline__ void element_matrix(const double* X, int ld, const in
           K[j][i] = v;
         }
     }
+    return fabs(det);
   }
 }
 
-template <int D, int RW, int IDB, int OFB, int FORM>
+template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>
 __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
   constexpr int NV = D + 1;
   constexpr uint32_t SENT = (1u << OFB) - 1u;
   extern __shared__ double smem[];
   double* acc = smem;
-  double* X = acc + a.max_slots + 2;
-  uint32_t* rinfo = reinterpret_cast<uint32_t*>(X + D * a.max_nodes);
+  double* X = acc + ((a.max_slots + 3) & ~1);  // 16-byte aligned for b128 loads
+  double* bacc = X + (D == 3 ? 4 : 2) * a.max_nodes;
+  uint32_t* rinfo = reinterpret_cast<uint32_t*>(bacc + (LOAD ? a.max_rows : 0));
 
-  const int part = xcd_remap(blockIdx.x, a.nparts);
+  // persistent: each workgroup walks partitions blockIdx.x, +gridDim.x, ...
+  for (int pi = blockIdx.x; pi < a.nparts; pi += gridDim.x) {
+  if (pi != (int)blockIdx.x) __syncthreads();  // LDS reuse
+  const int part = xcd_remap(pi, a.nparts);
   const int32_t* hdr = a.parts + 8 * (int64_t)part;
   const int node_start = hdr[0], L = hdr[1], row_start = hdr[2], R = hdr[3];
   const int nslots = hdr[4], chunk_start = hdr[5], nchunks = hdr[6];
   const int tid = threadIdx.x;
 
   for (int i = tid; i < nslots; i += kFusedBlock) acc[i] = 0.0;
+  constexpr int CS = (D == 3) ? 4 : 2;
   for (int l = tid; l < L; l += kFusedBlock) {
     const int64_t g = a.nodemap[node_start + l];
 #pragma unroll
-    for (int d = 0; d < D; ++d) X[d * a.max_nodes + l] = a.p[d * a.nnodes + g];
+    for (int d = 0; d < D; ++d) X[CS * l + d] = a.p[d * a.nnodes + g];
   }
-  for (int r = tid; r < R; r += kFusedBlock)
+  for (int r = tid; r < R; r += kFusedBlock) {
     rinfo[r] = (uint32_t)a.rows[2 * (int64_t)(row_start + r) + 1];
+    if constexpr (LOAD) bacc[r] = 0.0;
+  }
   __syncthreads();  // staged coordinates are read before the first round
 
   // records of the next chunk are fetched while the current one is summed
@@ -194,6 +208,7 @@ __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
     int lv[NV];
     uint32_t offs[NV * (NV - 1)];
     double K[NV][NV];
+    double vol = 0.0;
     int rc = -1;
     if (active) {
       lv[0] = bits<0, IDB, RW>(w);
@@ -202,39 +217,37 @@ __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
       if constexpr (NV == 4) lv[3] = bits<3 * IDB, IDB, RW>(w);
       rc = (int)bits<NV * IDB + NV * (NV - 1) * OFB, 4, RW>(w);
       decode_offsets<NV * IDB, OFB, RW>(w, offs, std::make_integer_sequence<int, NV * (NV - 1)>{});
-      element_matrix<D, FORM>(X, a.max_nodes, lv, FORM == FEMHIP_KAPPA_LAPLACE ? a.kappa[a.rec_elem[rec0 + tid]] : 1.0, K);
+      vol = element_matrix<D, FORM>(X, lv, FORM == FEMHIP_KAPPA_LAPLACE ? a.kappa[a.rec_elem[rec0 + tid]] : 1.0, K);
     }
     for (int r = 0; r < nrounds; ++r) {
       __syncthreads();
       if (rc == r) {
-        // all slot addresses first, then all loads, then all stores: the
-        // slots of one element are distinct, so nothing aliases
-        double* ptr[NV * NV];
+        // per owned vertex: its row's slots (all distinct), loads first,
+        // then adds and stores; rows of halo vertices are skipped
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
-          const bool own = lv[i] < R;
-          const uint32_t info = own ? rinfo[lv[i]] : 0u;
+          if (lv[i] >= R) continue;
+          // unit load: int phi_i = |det| / (d! (d+1))
+          if constexpr (LOAD) bacc[lv[i]] += vol * (D == 3 ? 1.0 / 24.0 : 1.0 / 6.0);
+          const uint32_t info = rinfo[lv[i]];
           double* row = acc + (info & 0xffffu);
           const uint32_t dg = (info >> 16) & 0xffu;
-          ptr[i * NV + i] = (own && dg != 0xffu) ? row + dg : dummy;
+          double* ptr[NV];
+          ptr[i] = (dg != 0xffu) ? row + dg : dummy;
           int q = 0;
 #pragma unroll
           for (int j = 0; j < NV; ++j) {
             if (j == i) continue;
             const uint32_t off = offs[i * (NV - 1) + q];
             ++q;
-            ptr[i * NV + j] = (own && off != SENT) ? row + off : dummy;
+            ptr[j] = (off != SENT) ? row + off : dummy;
           }
+          double v[NV];
+#pragma unroll
+          for (int j = 0; j < NV; ++j) v[j] = *ptr[j];
+#pragma unroll
+          for (int j = 0; j < NV; ++j) *ptr[j] = v[j] + K[i][j];
         }
-        double v[NV * NV];
-#pragma unroll
-        for (int k = 0; k < NV * NV; ++k) v[k] = *ptr[k];
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
-#pragma unroll
-          for (int j = 0; j < NV; ++j) v[i * NV + j] += K[i][j];
-#pragma unroll
-        for (int k = 0; k < NV * NV; ++k) *ptr[k] = v[k];
       }
     }
   }
@@ -248,20 +261,23 @@ __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
     for (int k = lane; k < len; k += 16)
       __builtin_nontemporal_store(acc[base + k], a.out + gstart + k);
   }
+  if constexpr (LOAD)
+    for (int r = tid; r < R; r += kFusedBlock) a.load[a.nodemap[node_start + r]] = bacc[r];
+  }  // partitions
 }
 
-template <int D, int RW, int IDB, int OFB>
-int launch_form(const FusedArgs& a, int form, size_t lds, hipStream_t s) {
-  dim3 grid((unsigned)a.nparts), block(kFusedBlock);
+template <int D, int RW, int IDB, int OFB, bool LOAD>
+int launch_form(const FusedArgs& a, int form, size_t lds, int groups, hipStream_t s) {
+  dim3 grid((unsigned)groups), block(kFusedBlock);
   switch (form) {
     case FEMHIP_LAPLACE:
-      p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_LAPLACE><<<grid, block, lds, s>>>(a);
+      p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
       break;
     case FEMHIP_MASS:
-      p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_MASS><<<grid, block, lds, s>>>(a);
+      p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_MASS, LOAD><<<grid, block, lds, s>>>(a);
       break;
     case FEMHIP_KAPPA_LAPLACE:
-      p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE><<<grid, block, lds, s>>>(a);
+      p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
       break;
     default:
       set_error("fused path: laplace, mass or kappa_laplace only");
@@ -271,6 +287,21 @@ int launch_form(const FusedArgs& a, int form, size_t lds, hipStream_t s) {
   return FEMHIP_OK;
 }
 
+template <bool LOAD>
+int dispatch_fused(const FusedArgs& a, const femhip_fused_plan_info_t& I, int form, int dim,
+                   size_t lds, int groups, hipStream_t st) {
+  if (dim == 3 && I.rec_words == 3 && I.idbits == 11 && I.offbits == 4)
+    return launch_form<3, 3, 11, 4, LOAD>(a, form, lds, groups, st);
+  if (dim == 3 && I.rec_words == 4 && I.idbits == 16 && I.offbits == 5)
+    return launch_form<3, 4, 16, 5, LOAD>(a, form, lds, groups, st);
+  if (dim == 2 && I.rec_words == 2 && I.idbits == 12 && I.offbits == 4)
+    return launch_form<2, 2, 12, 4, LOAD>(a, form, lds, groups, st);
+  if (dim == 2 && I.rec_words == 3 && I.idbits == 16 && I.offbits == 5)
+    return launch_form<2, 3, 16, 5, LOAD>(a, form, lds, groups, st);
+  set_error("fused: unsupported record format");
+  return FEMHIP_EUNSUPPORTED;
+}
+
 }  // namespace
 }  // namespace femhip
 
@@ -278,28 +309,25 @@ using namespace femhip;
 
 extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t form, int32_t dim,
                                      const double* p, int64_t nnodes, const double* kappa,
-                                     double* data, femhip_stream_t s) {
+                                     double* data, double* load, int32_t groups,
+                                     femhip_stream_t s) {
   FEMHIP_REQUIRE(plan != nullptr && p != nullptr && data != nullptr, FEMHIP_EINVAL,
                  "fused: NULL argument");
   const femhip_fused_plan_info_t& I = plan->info;
   FEMHIP_REQUIRE(I.block == kFusedBlock, FEMHIP_EINVAL, "fused: plan built for another block");
   FEMHIP_REQUIRE(form != FEMHIP_KAPPA_LAPLACE || (kappa && plan->rec_elem), FEMHIP_EINVAL,
                  "fused: kappa_laplace needs kappa and rec_elem");
+  FEMHIP_REQUIRE(groups >= 0, FEMHIP_EINVAL, "fused: negative group count");
   if (I.nparts == 0) return FEMHIP_OK;
+  const int ngroups = (groups == 0 || groups > I.nparts) ? I.nparts : groups;
   FusedArgs a{plan->parts, plan->nodemap, plan->rows, plan->chunks, plan->recs,
-              plan->rec_elem, p, kappa, data, nnodes, I.nparts, I.max_slots, I.max_nodes};
-  const size_t lds = 8 * ((size_t)I.max_slots + 2) + 8 * (size_t)dim * I.max_nodes +
-                     4 * (size_t)I.max_rows;
+              plan->rec_elem, p, kappa, data, load, nnodes, I.nparts, I.max_slots,
+              I.max_nodes, I.max_rows};
+  const size_t lds = 8 * (((size_t)I.max_slots + 3) & ~(size_t)1) +
+                     8 * (size_t)(dim == 3 ? 4 : 2) * I.max_nodes +
+                     (load ? 8 * (size_t)I.max_rows : 0) + 4 * (size_t)I.max_rows;
   FEMHIP_REQUIRE(lds <= 160 * 1024, FEMHIP_EINVAL, "fused: plan exceeds LDS");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
-  if (dim == 3 && I.rec_words == 3 && I.idbits == 11 && I.offbits == 4)
-    return launch_form<3, 3, 11, 4>(a, form, lds, st);
-  if (dim == 3 && I.rec_words == 4 && I.idbits == 16 && I.offbits == 5)
-    return launch_form<3, 4, 16, 5>(a, form, lds, st);
-  if (dim == 2 && I.rec_words == 2 && I.idbits == 12 && I.offbits == 4)
-    return launch_form<2, 2, 12, 4>(a, form, lds, st);
-  if (dim == 2 && I.rec_words == 3 && I.idbits == 16 && I.offbits == 5)
-    return launch_form<2, 3, 16, 5>(a, form, lds, st);
-  set_error("fused: unsupported record format");
-  return FEMHIP_EUNSUPPORTED;
+  if (load) return dispatch_fused<true>(a, I, form, dim, lds, ngroups, st);
+  return dispatch_fused<false>(a, I, form, dim, lds, ngroups, st);
 }

@@ -2,8 +2,8 @@
 // CSR pattern).
 //
 // The fused kernel assembles P1 forms owner-computes style: rows are cut
-// into partitions of spatially close nodes (Morton order of the node
-// coordinates), one workgroup per partition accumulates the partition's
+// into partitions of spatially close nodes (recursive coordinate
+// bisection of the node coordinates), one workgroup per partition accumulates the partition's
 // CSR rows in LDS and writes them once.  Every element with at least one
 // owned vertex is visited by that partition (halo elements are recomputed
 // by each partition they touch instead of exchanging partial sums).
@@ -43,26 +43,6 @@ struct Plan {
   std::vector<uint32_t> recs;    // rec_words uint32 per record
   std::vector<int32_t> rec_elem; // global element id per record
 };
-
-uint64_t spread3(uint64_t x) {  // 21 bits -> every third bit
-  x &= 0x1fffff;
-  x = (x | x << 32) & 0x1f00000000ffffULL;
-  x = (x | x << 16) & 0x1f0000ff0000ffULL;
-  x = (x | x << 8) & 0x100f00f00f00f00fULL;
-  x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
-  x = (x | x << 2) & 0x1249249249249249ULL;
-  return x;
-}
-
-uint64_t spread2(uint64_t x) {  // 31 bits -> every second bit
-  x &= 0x7fffffff;
-  x = (x | x << 16) & 0x0000ffff0000ffffULL;
-  x = (x | x << 8) & 0x00ff00ff00ff00ffULL;
-  x = (x | x << 4) & 0x0f0f0f0f0f0f0f0fULL;
-  x = (x | x << 2) & 0x3333333333333333ULL;
-  x = (x | x << 1) & 0x5555555555555555ULL;
-  return x;
-}
 
 // Writes `nbits` of `v` at bit position `pos` of a little-endian word array.
 void put_bits(uint32_t* w, int pos, int nbits, uint32_t v) {
@@ -106,28 +86,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   if (node_cap > id_cap) node_cap = id_cap;
   const uint32_t sentinel = (1u << offbits) - 1;
 
-  // 1. Morton order of the nodes
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int d = 0; d < dim; ++d)
-    for (int64_t n = 0; n < Nv; ++n) {
-      lo[d] = std::min(lo[d], p[d * Nv + n]);
-      hi[d] = std::max(hi[d], p[d * Nv + n]);
-    }
-  std::vector<std::pair<uint64_t, int32_t>> code(Nv);
-  const double qmax = (dim == 3) ? 2097151.0 : 2147483647.0;
-  for (int64_t n = 0; n < Nv; ++n) {
-    uint64_t c = 0;
-    for (int d = 0; d < dim; ++d) {
-      const double span = hi[d] - lo[d];
-      const double u = span > 0 ? (p[d * Nv + n] - lo[d]) / span : 0.0;
-      const uint64_t q = (uint64_t)std::min(qmax, std::max(0.0, u * qmax));
-      c |= (dim == 3 ? spread3(q) : spread2(q)) << d;
-    }
-    code[n] = {c, (int32_t)n};
-  }
-  std::sort(code.begin(), code.end());
-
-  // 2. node -> element incidence
+  // node -> element incidence
   std::vector<int64_t> inc_ptr(Nv + 1, 0);
   for (int64_t e = 0; e < E; ++e)
     for (int k = 0; k < NV; ++k) {
@@ -146,6 +105,79 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
       for (int k = 0; k < NV; ++k) inc[fill[t[(int64_t)k * E + e]]++] = (int32_t)e;
   }
 
+  // 1. recursive coordinate bisection of the nodes into blocks whose CSR
+  // rows fit `slot_cap` LDS slots; every split is weighted so the leaves
+  // come out close to the cap (fewer, larger blocks = less halo work).
+  std::vector<int32_t> order(Nv);
+  for (int64_t n = 0; n < Nv; ++n) order[n] = (int32_t)n;
+  std::vector<std::pair<int64_t, int64_t>> leaves;  // [begin, end) in `order`
+  std::vector<int64_t> cstamp(Nv, -1);
+  int64_t check_id = 0;
+  // owned + halo node count of the block order[b, e)
+  auto local_nodes = [&](int64_t b, int64_t e) {
+    const int64_t id = check_id++;
+    int64_t L = 0;
+    for (int64_t k = b; k < e; ++k) {
+      const int32_t n = order[k];
+      if (cstamp[n] != id) {
+        cstamp[n] = id;
+        ++L;
+      }
+      for (int64_t q = inc_ptr[n]; q < inc_ptr[n + 1]; ++q)
+        for (int j = 0; j < NV; ++j) {
+          const int32_t v = t[(int64_t)j * E + inc[q]];
+          if (cstamp[v] != id) {
+            cstamp[v] = id;
+            ++L;
+          }
+        }
+    }
+    return L;
+  };
+  {
+    std::vector<std::pair<int64_t, int64_t>> stack{{0, Nv}};
+    while (!stack.empty()) {
+      auto [b, e] = stack.back();
+      stack.pop_back();
+      int64_t slots = 0;
+      for (int64_t k = b; k < e; ++k) slots += indptr[order[k] + 1] - indptr[order[k]];
+      int64_t L = -1;
+      if (slots <= slot_cap) {
+        L = local_nodes(b, e);
+        if (L <= node_cap) {
+          leaves.push_back({b, e});
+          continue;
+        }
+      }
+      if (e - b <= 1) {
+        set_error("fused plan: one row does not fit the LDS budget");
+        return FEMHIP_EUNSUPPORTED;
+      }
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+      for (int64_t k = b; k < e; ++k)
+        for (int d = 0; d < dim; ++d) {
+          const double v = p[d * Nv + order[k]];
+          lo[d] = std::min(lo[d], v);
+          hi[d] = std::max(hi[d], v);
+        }
+      int ax = 0;
+      for (int d = 1; d < dim; ++d)
+        if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+      const int64_t nleaf = std::max<int64_t>(
+          2, std::max<int64_t>((slots + slot_cap - 1) / slot_cap,
+                               L > 0 ? (L + node_cap - 1) / node_cap : 2));
+      const int64_t mid = b + (int64_t)((double)(e - b) * (double)(nleaf / 2) / (double)nleaf);
+      std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e,
+                       [&](int32_t x, int32_t y) {
+                         const double px = p[ax * Nv + x], py = p[ax * Nv + y];
+                         return px < py || (px == py && x < y);
+                       });
+      // depth-first, left part first: leaves come out spatially ordered
+      stack.push_back({mid, e});
+      stack.push_back({b, mid});
+    }
+  }
+
   Plan* P = new Plan();
   std::memset(&P->info, 0, sizeof(P->info));
   std::vector<int32_t> nstamp(Nv, -1), lid(Nv, -1), estamp(E, -1);
@@ -153,41 +185,23 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   std::vector<uint64_t> used;  // per owned local row: colours used
   std::vector<int32_t> color;
   int32_t pid = 0;
-  int64_t pos = 0;
   int max_slots = 0, max_nodes = 0, max_rows = 0, max_colors = 0;
   int64_t visits = 0;
 
-  while (pos < Nv) {
+  for (size_t leaf = 0; leaf < leaves.size(); ++leaf) {
     owned.clear();
     halo.clear();
     elems.clear();
-    int slots = 0, nloc = 0;
-    // grow the partition along the Morton order
-    while (pos < Nv) {
-      const int32_t n = code[pos].second;
-      const int rl = indptr[n + 1] - indptr[n];
-      // conservative count of the new local nodes this row brings
-      int extra = (nstamp[n] == pid) ? 0 : 1;
-      for (int64_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
-        const int32_t e = inc[k];
-        if (estamp[e] == pid) continue;
-        for (int j = 0; j < NV; ++j)
-          if (nstamp[t[(int64_t)j * E + e]] != pid) ++extra;
-      }
-      if (!owned.empty() &&
-          (slots + rl > slot_cap || nloc + extra > node_cap || (int)owned.size() >= 65535))
-        break;
-      if (owned.empty() && (rl > slot_cap || extra > node_cap)) {
-        delete P;
-        set_error("fused plan: one row does not fit the LDS budget");
-        return FEMHIP_EUNSUPPORTED;
-      }
+    int nloc = 0;
+    for (int64_t k = leaves[leaf].first; k < leaves[leaf].second; ++k) {
+      const int32_t n = order[k];
       owned.push_back(n);
-      slots += rl;
       if (nstamp[n] != pid) {
         nstamp[n] = pid;
         ++nloc;
       }
+    }
+    for (int32_t n : owned)
       for (int64_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
         const int32_t e = inc[k];
         if (estamp[e] == pid) continue;
@@ -201,7 +215,10 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
           }
         }
       }
-      ++pos;
+    if (nloc > node_cap || (int)owned.size() > 65535) {
+      delete P;
+      set_error("fused plan: a block exceeds the local node budget");
+      return FEMHIP_EUNSUPPORTED;
     }
     // local numbering: owned first (partition order), then halo
     const int R = (int)owned.size();
@@ -316,6 +333,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
     visits += (int64_t)elems.size();
     ++pid;
   }
+  (void)halo;
   femhip_fused_plan_info_t& I = P->info;
   I.nparts = pid;
   I.nrecs = (int64_t)P->rec_elem.size();

@@ -93,7 +93,12 @@ SIGNATURES = {
     'femhip_fused_plan_export': (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
     'femhip_fused_plan_free': (None, [_p]),
     'femhip_fused_assemble': (ctypes.c_int, [ctypes.POINTER(FusedDev), _i32, _i32, _p,
-                                             _i64, _p, _p, _p]),
+                                             _i64, _p, _p, _p, _i32, _p]),
+    'femhip_batched_kappa_values': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _p, _i64, _p,
+                                                   _p]),
+    'femhip_hex2_elasticity_mfma': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space),
+                                                   _f64, _f64, _p, _p, _i32, _p]),
+    'femhip_segment_sum_em': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
     'femhip_p1_adjoint': (ctypes.c_int, [_i32, _i64, _i64, _p, _p, _p, _p, _p, _p, _p,
                                          _p, _p]),

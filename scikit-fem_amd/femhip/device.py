@@ -198,13 +198,18 @@ class DeviceBasis:
             self.plans[sig] = plan
         return plan
 
-    def fused_plan(self, plan):
+    def fused_plan(self, plan, form='laplace'):
+        """Fused plan for a CSR pattern (the autotuner's cached LDS budget
+        for this element kind / form / size when there is one)."""
         from .fused import FusedPlan
+        from . import autotune
         fp = self.fused_plans.get(plan.signature)
         if fp is None:
             torch.cuda.current_stream().synchronize()
+            caps = autotune.lookup(self.kind, form, self.N)
+            args = () if caps is None else caps[:2]
             fp = FusedPlan(self.basis.mesh, plan.indptr.cpu().numpy(),
-                           plan.indices.cpu().numpy())
+                           plan.indices.cpu().numpy(), *args)
             self.fused_plans[plan.signature] = fp
         return fp
 
@@ -244,21 +249,42 @@ class DeviceBasis:
         from . import fused
         fusable = (path != 'generic' and fused.eligible(self.basis, name)
                    and (kappa is None or kappa.dim() == 1))
+        mfma = (path != 'generic' and name == 'linear_elasticity' and self.kind == 'hex'
+                and self.basis.elem.vector and self.basis.elem.nbs == 27 and self.Q == 27)
         vkey = (name, lam, mu, token, self.p_version)
-        if fusable and vkey in self.validated:
+        if (fusable or mfma) and vkey in self.validated:
             plan = self.validated[vkey]
-            fp = self.fused_plans[plan.signature]
             out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
                               device=self.p.device)[:plan.nnz]
-            fp.assemble(name, self.p, out, kappa)
+            if fusable:
+                self.fused_plans[plan.signature].assemble(name, self.p, out, kappa)
+            else:
+                self.elasticity_mfma(lam, mu, plan, out)
             return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N))
         contrib = self.local_bilinear(name, kappa, lam, mu)
         plan = self.plan_for(contrib)
         data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
         if fusable:
-            self.fused_plan(plan)
+            self.fused_plan(plan, name)
+            self.validated[vkey] = plan
+        elif mfma:
             self.validated[vkey] = plan
         return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N))
+
+    def elasticity_mfma(self, lam, mu, plan, out, loc=None):
+        """Numeric vector-Q2-hex elasticity: MFMA local matrices (element
+        major) + segmented sums over the plan."""
+        G = self.geometry()
+        nb2 = self.nbfun * self.nbfun
+        if loc is None:
+            loc = torch.empty(nb2 * self.E, dtype=torch.float64, device=self.p.device)
+        _lib.check(self.lib.femhip_hex2_elasticity_mfma(
+            ctypes.byref(self.geom), ctypes.byref(self.space), float(lam), float(mu),
+            _lib.ptr(G), _lib.ptr(loc), 0, _lib.stream_ptr()))
+        _lib.check(self.lib.femhip_segment_sum_em(
+            _lib.ptr(loc), _lib.ptr(plan.perm), _lib.ptr(plan.seg), plan.nnz, self.E, nb2,
+            _lib.ptr(out), _lib.stream_ptr()))
+        return out
 
     def assemble_linear(self, name='unit_load'):
         contrib = self.local_linear(name)

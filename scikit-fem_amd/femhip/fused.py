@@ -13,9 +13,12 @@ import torch
 
 from . import _lib
 
-# LDS budget per workgroup: two workgroups per CU (2 x 80 KB of 160 KB)
-DEFAULT_SLOT_CAP = 6144
-DEFAULT_NODE_CAP = 1024
+import os
+
+# LDS budget per workgroup: two workgroups per CU (2 x 80 KB of 160 KB);
+# FEMHIP_FUSED_SLOT_CAP / FEMHIP_FUSED_NODE_CAP override it (tuning).
+DEFAULT_SLOT_CAP = int(os.environ.get('FEMHIP_FUSED_SLOT_CAP', 6144))
+DEFAULT_NODE_CAP = int(os.environ.get('FEMHIP_FUSED_NODE_CAP', 1024))
 BLOCK = 512
 
 _FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS, 'kappa_laplace': _lib.KAPPA_LAPLACE}
@@ -56,18 +59,86 @@ class FusedPlan:
                 handle, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs]))
         finally:
             lib.femhip_fused_plan_free(handle)
+        self.caps = (slot_cap, node_cap)
         self.info = info
+        self.host = dict(parts=parts, nodemap=nodemap, rows=rows, chunks=chunks,
+                         recs=recs, rec_elem=rec_elem)
+        self.nnz = int(indptr[-1])
+        self.nnodes = p.shape[1]
+        self.nelems = t.shape[1]
+        self.dim = p.shape[0]
+        self._upload()
+
+    def _upload(self):
+        h = self.host
         dev = torch.device('cuda')
-        self.parts = torch.from_numpy(parts).to(dev)
-        self.nodemap = torch.from_numpy(nodemap).to(dev)
-        self.rows = torch.from_numpy(rows).to(dev)
-        self.chunks = torch.from_numpy(chunks).to(dev)
-        self.recs = torch.from_numpy(recs.view(np.int32)).to(dev)
-        self.rec_elem = torch.from_numpy(rec_elem).to(dev)
+        self.parts = torch.from_numpy(h['parts']).to(dev)
+        self.nodemap = torch.from_numpy(h['nodemap']).to(dev)
+        self.rows = torch.from_numpy(h['rows']).to(dev)
+        self.chunks = torch.from_numpy(h['chunks']).to(dev)
+        self.recs = torch.from_numpy(h['recs'].view(np.int32)).to(dev)
+        self.rec_elem = torch.from_numpy(h['rec_elem']).to(dev)
         self.dev = _lib.FusedDev(self.parts.data_ptr(), self.nodemap.data_ptr(),
                                  self.rows.data_ptr(), self.chunks.data_ptr(),
-                                 self.recs.data_ptr(), self.rec_elem.data_ptr(), info)
-        self.nnz = int(indptr[-1])
+                                 self.recs.data_ptr(), self.rec_elem.data_ptr(), self.info)
+
+    @classmethod
+    def concat(cls, plans):
+        """One plan over several meshes: node ids, CSR offsets, records and
+        element ids are shifted so that the concatenated coordinate array,
+        CSR value array and load vector of the meshes can be assembled in a
+        single launch."""
+        fmt = {(q.info.rec_words, q.info.idbits, q.info.offbits, q.dim) for q in plans}
+        if len(fmt) != 1:
+            raise NotImplementedError("meshes need the same record format to be batched")
+        out = cls.__new__(cls)
+        parts, nodemap, rows, chunks, recs, rec_elem = [], [], [], [], [], []
+        off_node = off_nnz = off_map = off_row = off_chunk = off_rec = off_el = 0
+        info = _lib.FusedInfo()
+        for f in ('max_slots', 'max_nodes', 'max_rows', 'max_colors'):
+            setattr(info, f, max(getattr(q.info, f) for q in plans))
+        q0 = plans[0].info
+        info.rec_words, info.idbits, info.offbits, info.block = (
+            q0.rec_words, q0.idbits, q0.offbits, q0.block)
+        self_offsets = []
+        for q in plans:
+            h = q.host
+            pt = h['parts'].reshape(-1, 8).copy()
+            pt[:, 0] += off_map
+            pt[:, 2] += off_row
+            pt[:, 5] += off_chunk
+            parts.append(pt.ravel())
+            nodemap.append(h['nodemap'] + off_node)
+            rw = h['rows'].reshape(-1, 2).copy()
+            rw[:, 0] += off_nnz
+            rows.append(rw.ravel())
+            ch = h['chunks'].reshape(-1, 2).copy()
+            ch[:, 0] += off_rec
+            chunks.append(ch.ravel())
+            recs.append(h['recs'])
+            rec_elem.append(h['rec_elem'] + off_el)
+            self_offsets.append((off_node, off_nnz, off_el))
+            off_node += q.nnodes
+            off_nnz += q.nnz
+            off_el += q.nelems
+            off_map += len(h['nodemap'])
+            off_row += len(h['rows']) // 2
+            off_chunk += len(h['chunks']) // 2
+            off_rec += len(h['rec_elem'])
+        out.host = dict(parts=np.concatenate(parts), nodemap=np.concatenate(nodemap),
+                        rows=np.concatenate(rows), chunks=np.concatenate(chunks),
+                        recs=np.concatenate(recs), rec_elem=np.concatenate(rec_elem))
+        info.nparts = sum(q.info.nparts for q in plans)
+        info.nchunks = off_chunk
+        info.nrows = off_row
+        info.nrecs = off_rec
+        info.nlocal = off_map
+        info.dup = float(off_rec) / max(1, off_el)
+        out.info = info
+        out.nnz, out.nnodes, out.nelems, out.dim = off_nnz, off_node, off_el, plans[0].dim
+        out.offsets = self_offsets
+        out._upload()
+        return out
 
     def stats(self):
         i = self.info
@@ -78,11 +149,16 @@ class FusedPlan:
                         self.parts, self.nodemap, self.rows, self.chunks,
                         self.recs, self.rec_elem))))
 
-    def assemble(self, name, p, out, kappa=None):
-        """out[nnz] <- CSR values of ``name`` for node coordinates ``p``."""
+    def assemble(self, name, p, out, kappa=None, load=None, groups=0):
+        """out[nnz] <- CSR values of ``name`` for node coordinates ``p``
+        (and load[nnodes] <- the unit-load vector when given)."""
         lib = _lib.load()
         assert out.numel() == self.nnz and out.dtype == torch.float64
+        assert p.shape[1] == self.nnodes and p.is_contiguous()
+        assert load is None or load.numel() == self.nnodes
+        assert kappa is None or kappa.numel() == self.nelems
         _lib.check(lib.femhip_fused_assemble(
             ctypes.byref(self.dev), _FORM[name], p.shape[0], _lib.ptr(p), p.shape[1],
-            _lib.ptr(kappa), _lib.ptr(out), _lib.stream_ptr()))
+            _lib.ptr(kappa), _lib.ptr(out), _lib.ptr(load), int(groups),
+            _lib.stream_ptr()))
         return out
