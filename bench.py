@@ -75,6 +75,149 @@ def cpu_baseline(n_sample=81):
             "host_affinity": len(os.sched_getaffinity(0))}
 
 
+def _pct(a, q):
+    return float(np.percentile(np.asarray(a), q))
+
+
+def bench_batched_kappa(S=256, n=257):
+    """256 per-element kappa samples on the P2 triangle mesh 257^2 in one
+    [S, nnz] launch vs plain per-sample GPU assemblies."""
+    import torch
+    import femhip
+    from femhip.batch import KappaSamples
+    from femhip.models import kappa_laplace
+    x = np.linspace(0, 1, n)
+    m = femhip.MeshTri.init_tensor(x, x)
+    b = femhip.Basis(m, femhip.ElementTriP2())
+    kap = np.random.default_rng(0).lognormal(0, 0.5, (S, m.nelements))
+    kt = torch.tensor(kap, device='cuda')
+    ks = KappaSamples(b)
+    out = torch.empty((S, ks.nnz), dtype=torch.float64, device='cuda')
+    stream = torch.cuda.current_stream()
+    ks.assemble(kt, out)
+    ms_batch = time_stream(lambda: ks.assemble(kt, out), 5, stream)
+    # plain: one drop-in GPU assembly per sample (exact path; kappa changes
+    # every call, so each call also validates its pattern)
+    nplain = 8
+    kappa_laplace.assemble_device(b, kappa=kap[0][:, None])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(1, nplain + 1):
+        kappa_laplace.assemble_device(b, kappa=kt[s][:, None])
+    torch.cuda.synchronize()
+    ms_plain = (time.perf_counter() - t0) * 1e3 / nplain
+    return {"samples": S, "elements": m.nelements, "dofs": b.N, "nnz": ks.nnz,
+            "batched_ms_total": ms_batch, "batched_ms_per_sample": ms_batch / S,
+            "plain_ms_per_sample": ms_plain,
+            "speedup_per_sample": ms_plain / (ms_batch / S),
+            "batched_GBps_written": S * ks.nnz * 8 / (ms_batch * 1e-3) / 1e9}
+
+
+def bench_mesh_batch(k=64):
+    """64 P1 triangle meshes, n_i ~ U[17, 257]: Laplace + unit load, one
+    persistent graph-captured launch vs per-mesh launches."""
+    import torch
+    import femhip
+    from femhip.batch import MeshBatch
+    from femhip.models import laplace
+    ns = np.random.default_rng(0).integers(17, 258, k)
+    bases = []
+    for n in ns:
+        x = np.linspace(0, 1, int(n))
+        bases.append(femhip.Basis(femhip.MeshTri.init_tensor(x, x), femhip.ElementTriP1()))
+    mb = MeshBatch(bases, groups=512)
+    mb.capture()
+    stream = torch.cuda.current_stream()
+    ms_graph = time_stream(mb.replay, 20, stream)
+    ms_launch = time_stream(mb.assemble, 20, stream)
+    # plain: per-mesh fused launches (Laplace + load), each timed alone
+    per = []
+    for b in bases:
+        d = b.device()
+        plan = next(p for kk, p in d.validated.items() if kk[0] == 'laplace')
+        fp = d.fused_plans[plan.signature]
+        out = torch.empty(plan.nnz, dtype=torch.float64, device='cuda')
+        ld = torch.empty(b.N, dtype=torch.float64, device='cuda')
+        per.append(time_stream(lambda: fp.assemble('laplace', d.p, out, load=ld), 10, stream))
+    E = sum(b.nelems for b in bases)
+    return {"meshes": k, "elements": int(E), "graph_ms": ms_graph, "launch_ms": ms_launch,
+            "per_mesh_batched_ms": ms_graph / k, "plain_sum_ms": float(np.sum(per)),
+            "plain_p50_ms": _pct(per, 50), "plain_p99_ms": _pct(per, 99),
+            "speedup_vs_plain_sum": float(np.sum(per)) / ms_graph}
+
+
+def bench_adjoint(n=65):
+    """kappa-Laplace + load on MeshTet 65^3: forward vs forward + adjoint."""
+    import torch
+    import femhip
+    from femhip.autograd import kappa_laplace_with_load, quadratic_loss
+    x = np.linspace(0, 1, n)
+    m = femhip.MeshTet.init_tensor(x, x, x)
+    b = femhip.Basis(m, femhip.ElementTetP1())
+    kap = torch.tensor(np.random.default_rng(0).lognormal(0, 1, m.nelements), device='cuda')
+    u = torch.tensor(np.random.default_rng(1).standard_normal(b.N), device='cuda')
+    f = torch.tensor(np.random.default_rng(2).standard_normal(b.N), device='cuda')
+    p = torch.tensor(m.p, device='cuda')
+    stream = torch.cuda.current_stream()
+
+    def fwd():
+        with torch.no_grad():
+            kappa_laplace_with_load(b, kap, p)
+
+    def fwdbwd():
+        k = kap.clone().requires_grad_(True)
+        pp = p.clone().requires_grad_(True)
+        data, load, (ip, ix) = kappa_laplace_with_load(b, k, pp)
+        quadratic_loss(data, ip, ix, u, f).backward()
+    fwdbwd()
+    ms_f = time_stream(fwd, 10, stream)
+    ms_fb = time_stream(fwdbwd, 10, stream)
+    return {"elements": m.nelements, "dofs": b.N, "forward_ms": ms_f,
+            "forward_backward_ms": ms_fb, "backward_over_forward": (ms_fb - ms_f) / ms_f}
+
+
+def bench_elasticity(n=41):
+    """linear_elasticity on ElementVector(ElementHex2), 27 points, MeshHex
+    (n-1)^3 jittered 0.15h: MFMA local matrices + segmented sums."""
+    import torch
+    import femhip
+    from femhip.models import linear_elasticity, lame_parameters
+    x = np.linspace(0, 1, n)
+    m = femhip.MeshHex.init_tensor(x, x, x).jittered(0.15, 1.0 / (n - 1))
+    b = femhip.Basis(m, femhip.ElementVector(femhip.ElementHex2()), intorder=5)
+    lam, mu = lame_parameters(1.0, 0.3)
+    form = linear_elasticity(lam, mu)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A = form.assemble_device(b)
+    torch.cuda.synchronize()
+    first = (time.perf_counter() - t0) * 1e3
+    d = b.device()
+    plan = next(iter(d.validated.values()))
+    out = torch.empty(plan.nnz, dtype=torch.float64, device='cuda')
+    loc = torch.empty(81 * 81 * m.nelements, dtype=torch.float64, device='cuda')
+    G = d.geometry()
+    stream = torch.cuda.current_stream()
+    from femhip import _lib
+    import ctypes
+
+    def local():
+        _lib.check(d.lib.femhip_hex2_elasticity_mfma(
+            ctypes.byref(d.geom), ctypes.byref(d.space), lam, mu, _lib.ptr(G),
+            _lib.ptr(loc), 0, _lib.stream_ptr()))
+    ms_local = time_stream(local, 5, stream)
+    ms_all = time_stream(lambda: d.elasticity_mfma(lam, mu, plan, out, loc), 5, stream)
+    err = float((out - A.data).abs().max() / A.data.abs().max())
+    E = m.nelements
+    exec_flops = E * 6 * 2 * 32 * 32 * 28
+    return {"elements": E, "dofs": b.N, "nnz": plan.nnz, "first_call_ms": first,
+            "mfma_local_ms": ms_local, "assembly_ms": ms_all,
+            "mfma_TFLOPs_executed": exec_flops / (ms_local * 1e-3) / 1e12,
+            "mfma_util_of_78.6TF": exec_flops / (ms_local * 1e-3) / 78.6e12,
+            "survey_BtDB_TFLOPs_equiv": E * 2283228 / (ms_local * 1e-3) / 1e12,
+            "max_rel_err_vs_exact": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -83,6 +226,8 @@ def main():
     ap.add_argument('--n', type=int, default=101)
     ap.add_argument('--path', default='auto', choices=['auto', 'generic', 'fused'])
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-extra', action='store_true',
+                    help='skip the batched / adjoint / elasticity measurements')
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -122,7 +267,8 @@ def main():
 
         def step():
             fp.assemble('laplace', dvb.p, out)
-        kernel_name = 'p1_fused_kernel'
+        kernel_name = ('p1_sweep_kernel' if getattr(fp, 'kind', '') == 'sweep'
+                       else 'p1_fused_kernel')
     else:
         plan = next(iter(dvb.plans.values()))
         contrib = torch.empty(16 * E, dtype=torch.float64, device=dev)
@@ -186,6 +332,18 @@ def main():
     res["copy_bw_GBps"] = 2 * src.numel() * 8 / (cms * 1e-3) / 1e9
     res["roofline"]["frac_of_copy_bw"] = achieved / 1e9 / res["copy_bw_GBps"]
     del src, dst
+    if not args.no_extra:
+        for name, fn in (("batched_kappa_p2tri257_S256", bench_batched_kappa),
+                         ("mesh_batch_64_p1tri", bench_mesh_batch),
+                         ("adjoint_kappa_tet65", bench_adjoint),
+                         ("elasticity_vhex2_40", bench_elasticity)):
+            log('extra:', name)
+            try:
+                res[name] = fn()
+            except Exception as exc:   # report, keep the headline line intact
+                res[name] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
     if not args.no_cpu:
         log('cpu baseline ...')
         res["cpu_baseline"] = cpu_baseline()

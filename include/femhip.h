@@ -226,6 +226,46 @@ int femhip_batched_kappa_values(const double* K1, const int32_t* perm, const int
                                 int64_t nsamples, double* out, femhip_stream_t s);
 
 /* ---------------------------------------------------------------------
+ * Fused P1 path, layer-sweep variant: the same computation as
+ * femhip_fused_assemble with blocks of several thousand rows assembled
+ * level by level (breadth-first levels of the node graph), two levels of
+ * rows and coordinates resident in LDS at a time.  Plan arrays:
+ *   blocks[4*nblocks] levels[6*nlevels] nodemap[nlocal] rows[2*nrows]
+ *   chunks[2*nchunks] recs[rec_words*nrecs] rec_elem[nrecs].
+ * --------------------------------------------------------------------- */
+typedef struct femhip_sweep_plan_info {
+  int32_t nblocks, nlevels, nchunks, nrows;
+  int32_t max_level_nodes, max_level_rows, max_level_slots, max_colors;
+  int32_t rec_words, offbits, chunk, reserved;
+  int64_t nrecs, nlocal;
+  double dup;
+} femhip_sweep_plan_info_t;
+
+typedef struct femhip_sweep_dev {
+  const int32_t* blocks;
+  const int32_t* levels;
+  const int32_t* nodemap;
+  const int32_t* rows;
+  const int32_t* chunks;
+  const uint32_t* recs;
+  const int32_t* rec_elem;
+  femhip_sweep_plan_info_t info;
+} femhip_sweep_dev_t;
+
+int femhip_sweep_plan_build(int32_t nverts, int32_t dim, int64_t nelems, int64_t nnodes,
+                            const int32_t* t, const double* p, const int32_t* indptr,
+                            const int32_t* indices, int32_t block_rows, int32_t chunk,
+                            void** handle, femhip_sweep_plan_info_t* info);
+int femhip_sweep_plan_export(void* handle, int32_t* blocks, int32_t* levels, int32_t* nodemap,
+                             int32_t* rows, int32_t* chunks, uint32_t* recs, int32_t* rec_elem);
+void femhip_sweep_plan_free(void* handle);
+size_t femhip_sweep_lds_bytes(const femhip_sweep_plan_info_t* info, int32_t dim,
+                              int32_t with_load);
+int femhip_sweep_assemble(const femhip_sweep_dev_t* plan, int32_t form, int32_t dim,
+                          const double* p, int64_t nnodes, const double* kappa, double* data,
+                          double* load, int32_t groups, femhip_stream_t s);
+
+/* ---------------------------------------------------------------------
  * Adjoint of the P1 (kappa-)Laplace + unit-load assembly.  The reference
  * has no adjoint (its autodiff, skfem/autodiff, differentiates with
  * respect to the solution through JAX); this replaces finite differences

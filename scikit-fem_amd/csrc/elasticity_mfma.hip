@@ -84,8 +84,9 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
         const double b = s_gb[l][q0 + k4][tj * 16 + r16];   // B[kk][j]
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
       }
+      // v_mfma_f64_16x16x4f64 output: register r of lane l is D[4r + l/16][l%16]
 #pragma unroll
-      for (int r = 0; r < 4; ++r) s_S[kl][ti * 16 + 4 * k4 + r][tj * 16 + r16] = acc[r];
+      for (int r = 0; r < 4; ++r) s_S[kl][ti * 16 + 4 * r + k4][tj * 16 + r16] = acc[r];
     }
     __syncthreads();
     // 81 x 81 entries, element-major, index j*81 + i (j trial, i test)

@@ -61,6 +61,19 @@ class FusedDev(ctypes.Structure):
                 ('recs', _p), ('rec_elem', _p), ('info', FusedInfo)]
 
 
+class SweepInfo(ctypes.Structure):
+    _fields_ = [('nblocks', _i32), ('nlevels', _i32), ('nchunks', _i32), ('nrows', _i32),
+                ('max_level_nodes', _i32), ('max_level_rows', _i32),
+                ('max_level_slots', _i32), ('max_colors', _i32), ('rec_words', _i32),
+                ('offbits', _i32), ('chunk', _i32), ('reserved', _i32), ('nrecs', _i64),
+                ('nlocal', _i64), ('dup', _f64)]
+
+
+class SweepDev(ctypes.Structure):
+    _fields_ = [('blocks', _p), ('levels', _p), ('nodemap', _p), ('rows', _p),
+                ('chunks', _p), ('recs', _p), ('rec_elem', _p), ('info', SweepInfo)]
+
+
 class FemhipError(RuntimeError):
     """HIP runtime failure reported by the library."""
 
@@ -99,6 +112,14 @@ SIGNATURES = {
     'femhip_hex2_elasticity_mfma': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space),
                                                    _f64, _f64, _p, _p, _i32, _p]),
     'femhip_segment_sum_em': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _p]),
+    'femhip_sweep_plan_build': (ctypes.c_int, [_i32, _i32, _i64, _i64, _p, _p, _p, _p,
+                                               _i32, _i32, ctypes.POINTER(ctypes.c_void_p),
+                                               ctypes.POINTER(SweepInfo)]),
+    'femhip_sweep_plan_export': (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p]),
+    'femhip_sweep_plan_free': (None, [_p]),
+    'femhip_sweep_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(SweepInfo), _i32, _i32]),
+    'femhip_sweep_assemble': (ctypes.c_int, [ctypes.POINTER(SweepDev), _i32, _i32, _p, _i64,
+                                             _p, _p, _p, _i32, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
     'femhip_p1_adjoint': (ctypes.c_int, [_i32, _i64, _i64, _p, _p, _p, _p, _p, _p, _p,
                                          _p, _p]),

@@ -57,11 +57,10 @@ class MeshBatch:
             dev = b.device()
             laplace.assemble_device(b)          # exact path: validates the pattern
             plan = next(p for k, p in dev.validated.items() if k[0] == 'laplace')
-            if caps is not None:
-                fp = _fused.FusedPlan(b.mesh, plan.indptr.cpu().numpy(),
-                                      plan.indices.cpu().numpy(), *caps)
-            else:
-                fp = dev.fused_plan(plan)
+            # the block-partition variant: plans of any number of meshes
+            # concatenate into one launch
+            fp = _fused.FusedPlan(b.mesh, plan.indptr.cpu().numpy(),
+                                  plan.indices.cpu().numpy(), *(caps or ()))
             plans.append((plan, fp))
             ps.append(dev.p)
         self.patterns = [pl for pl, _ in plans]

@@ -201,15 +201,24 @@ class DeviceBasis:
     def fused_plan(self, plan, form='laplace'):
         """Fused plan for a CSR pattern (the autotuner's cached LDS budget
         for this element kind / form / size when there is one)."""
-        from .fused import FusedPlan
+        import os
+        from .fused import FusedPlan, SweepPlan
         from . import autotune
         fp = self.fused_plans.get(plan.signature)
         if fp is None:
             torch.cuda.current_stream().synchronize()
-            caps = autotune.lookup(self.kind, form, self.N)
-            args = () if caps is None else caps[:2]
-            fp = FusedPlan(self.basis.mesh, plan.indptr.cpu().numpy(),
-                           plan.indices.cpu().numpy(), *args)
+            ip, ix = plan.indptr.cpu().numpy(), plan.indices.cpu().numpy()
+            variant = os.environ.get('FEMHIP_FUSED_VARIANT', 'sweep')
+            fp = None
+            if variant == 'sweep':
+                try:
+                    fp = SweepPlan(self.basis.mesh, ip, ix)
+                except NotImplementedError:
+                    fp = None      # mesh the sweep cannot level: block variant
+            if fp is None:
+                caps = autotune.lookup(self.kind, form, self.N)
+                args = () if caps is None else caps[:2]
+                fp = FusedPlan(self.basis.mesh, ip, ix, *args)
             self.fused_plans[plan.signature] = fp
         return fp
 

@@ -162,3 +162,79 @@ class FusedPlan:
             _lib.ptr(kappa), _lib.ptr(out), _lib.ptr(load), int(groups),
             _lib.stream_ptr()))
         return out
+
+
+DEFAULT_BLOCK_ROWS = int(os.environ.get('FEMHIP_SWEEP_BLOCK_ROWS', 4096))
+DEFAULT_SWEEP_CHUNK = int(os.environ.get('FEMHIP_SWEEP_CHUNK', 512))
+
+
+def sweep_plan_arrays(mesh, indptr, indices, block_rows=None, chunk=None):
+    """Build a layer-sweep plan on the host; returns (info, dict of arrays)."""
+    lib = _lib.load()
+    t = np.ascontiguousarray(mesh.t, dtype=np.int32)
+    p = np.ascontiguousarray(mesh.p, dtype=np.float64)
+    ip = np.ascontiguousarray(indptr, dtype=np.int32)
+    ix = np.ascontiguousarray(indices, dtype=np.int32)
+    handle = ctypes.c_void_p()
+    info = _lib.SweepInfo()
+    _lib.check(lib.femhip_sweep_plan_build(
+        t.shape[0], p.shape[0], t.shape[1], p.shape[1],
+        t.ctypes.data_as(ctypes.c_void_p), p.ctypes.data_as(ctypes.c_void_p),
+        ip.ctypes.data_as(ctypes.c_void_p), ix.ctypes.data_as(ctypes.c_void_p),
+        int(block_rows or DEFAULT_BLOCK_ROWS), int(chunk or DEFAULT_SWEEP_CHUNK),
+        ctypes.byref(handle), ctypes.byref(info)))
+    try:
+        arrs = dict(blocks=np.empty(4 * info.nblocks, np.int32),
+                    levels=np.empty(6 * info.nlevels, np.int32),
+                    nodemap=np.empty(info.nlocal, np.int32),
+                    rows=np.empty(2 * info.nrows, np.int32),
+                    chunks=np.empty(2 * info.nchunks, np.int32),
+                    recs=np.empty(info.rec_words * info.nrecs, np.uint32),
+                    rec_elem=np.empty(info.nrecs, np.int32))
+        _lib.check(lib.femhip_sweep_plan_export(
+            handle, *[arrs[k].ctypes.data_as(ctypes.c_void_p) for k in
+                      ('blocks', 'levels', 'nodemap', 'rows', 'chunks', 'recs', 'rec_elem')]))
+    finally:
+        lib.femhip_sweep_plan_free(handle)
+    return info, arrs
+
+
+class SweepPlan:
+    """Layer-sweep plan of the fused P1 kernel, in device memory."""
+
+    kind = 'sweep'
+
+    def __init__(self, mesh, indptr, indices, block_rows=None, chunk=None):
+        info, h = sweep_plan_arrays(mesh, indptr, indices, block_rows, chunk)
+        self.info, self.host = info, h
+        self.caps = (int(block_rows or DEFAULT_BLOCK_ROWS), int(chunk or DEFAULT_SWEEP_CHUNK))
+        self.nnz = int(indptr[-1])
+        self.nnodes = mesh.p.shape[1]
+        self.nelems = mesh.t.shape[1]
+        self.dim = mesh.p.shape[0]
+        dev = torch.device('cuda')
+        self.t = {k: torch.from_numpy(v.view(np.int32) if v.dtype == np.uint32 else v).to(dev)
+                  for k, v in h.items()}
+        self.dev = _lib.SweepDev(*[self.t[k].data_ptr() for k in
+                                   ('blocks', 'levels', 'nodemap', 'rows', 'chunks', 'recs',
+                                    'rec_elem')], info)
+
+    def stats(self):
+        i = self.info
+        return dict(variant='sweep', blocks=i.nblocks, levels=i.nlevels, records=int(i.nrecs),
+                    dup=i.dup, max_colors=i.max_colors, rec_bytes=4 * i.rec_words,
+                    max_level_slots=i.max_level_slots, max_level_nodes=i.max_level_nodes,
+                    lds_bytes=int(_lib.load().femhip_sweep_lds_bytes(ctypes.byref(i), self.dim, 0)),
+                    nlocal_over_nodes=float(i.nlocal) / self.nnodes,
+                    plan_bytes=int(sum(t.numel() * t.element_size() for t in self.t.values())))
+
+    def assemble(self, name, p, out, kappa=None, load=None, groups=0):
+        lib = _lib.load()
+        assert out.numel() == self.nnz and out.dtype == torch.float64
+        assert p.shape[1] == self.nnodes and p.is_contiguous()
+        assert load is None or load.numel() == self.nnodes
+        assert kappa is None or kappa.numel() == self.nelems
+        _lib.check(lib.femhip_sweep_assemble(
+            ctypes.byref(self.dev), _FORM[name], p.shape[0], _lib.ptr(p), p.shape[1],
+            _lib.ptr(kappa), _lib.ptr(out), _lib.ptr(load), int(groups), _lib.stream_ptr()))
+        return out

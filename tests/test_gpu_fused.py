@@ -107,3 +107,41 @@ def test_tet65_kappa_digest():
     for rep in range(2):
         A = laplace.assemble_device(b)
     _check_digest(A, dig['laplace'])
+
+
+@pytest.mark.parametrize('kind,n,a,br,ck,form,load', [
+    ('tet', 12, 0.2, 200, 512, 'laplace', True),
+    ('tet', 12, 0.2, 4096, 1024, 'mass', False),
+    ('tri', 40, 0.2, 150, 512, 'laplace', True),
+    ('tet', 10, 0.0, 300, 512, 'kappa_laplace', False),
+])
+def test_sweep_variant(kind, n, a, br, ck, form, load):
+    """The layer-sweep kernel with small and large blocks, both workgroup
+    sizes, with and without the fused load vector."""
+    from femhip.fused import SweepPlan
+    m = _mesh(kind, n, a)
+    el = femhip.ElementTriP1() if kind == 'tri' else femhip.ElementTetP1()
+    b = femhip.Basis(m, el)
+    kw = {}
+    kappa = None
+    if form == 'kappa_laplace':
+        kk = np.random.default_rng(2).lognormal(0, 1, m.nelements)
+        kw['kappa'] = kk[:, None]
+        kappa = torch.tensor(kk, device='cuda')
+    f = {'laplace': laplace, 'mass': mass, 'kappa_laplace': kappa_laplace}[form]
+    A = f.assemble_device(b, path='generic', **kw)
+    sp = SweepPlan(m, A.indptr.cpu().numpy(), A.indices.cpu().numpy(), br, ck)
+    d = b.device()
+    out = torch.empty(A.nnz, dtype=torch.float64, device='cuda')
+    ld = torch.empty(b.N, dtype=torch.float64, device='cuda') if load else None
+    sp.assemble(form, d.p, out, kappa=kappa, load=ld)
+    out2 = torch.empty_like(out)
+    sp.assemble(form, d.p, out2, kappa=kappa, load=None, groups=7)
+    torch.cuda.synchronize()
+    ref = A.data.cpu().numpy()
+    got = out.cpu().numpy()
+    assert np.abs(got - ref).max() <= TOL * np.abs(ref).max()
+    assert np.array_equal(got, out2.cpu().numpy())
+    if load:
+        bref = unit_load.assemble(b)
+        assert np.abs(ld.cpu().numpy() - bref).max() <= TOL * np.abs(bref).max()

@@ -281,3 +281,30 @@ def test_planted_mistake_is_caught():
     same = (A.nnz == len(z['laplace_indices'])
             and np.array_equal(A.indices, z['laplace_indices']))
     assert not same or not np.array_equal(A.data, z['laplace_data'])
+
+
+def test_elasticity_s_matrix_identity():
+    """The MFMA kernel's formulation: K[(a,c),(b,d)] = lam S_cd[a,b] +
+    mu S_dc[a,b] + mu delta_cd tr(S)[a,b], S_kl = G_k^T diag(w|J|) G_l,
+    equals the oracle's ddot(C(sym_grad u), sym_grad v) local matrices."""
+    x = np.linspace(0, 1, 3)
+    m = O.jitter(O.MeshHex.init_tensor(x, x, x), 0.15, 0.5)
+    lam, mu = O.lame_parameters(1.0, 0.3)
+    vb = O.CellBasis(m, O.ElementVector(O.ElementHex2()), intorder=5)
+    _, data, _ = O.bilinear_coo(O.form_elasticity(lam, mu), vb)
+    K = data.reshape(81, 81, -1)          # [j trial, i test, e]
+    sb = O.CellBasis(m, O.ElementHex2(), intorder=5)
+    g = np.array([sb.basis[a][1] for a in range(27)])   # (27, 3, E, Q)
+    om = sb.dx                                           # (E, Q)
+    S = np.einsum('akeq,bleq,eq->klabe', g, g, om)       # (3,3,27,27,E)
+    tr = S[0, 0] + S[1, 1] + S[2, 2]
+    Kf = np.zeros_like(K)
+    for i in range(81):
+        a, c = divmod(i, 3)
+        for j in range(81):
+            b, d = divmod(j, 3)
+            v = lam * S[c, d, a, b] + mu * S[d, c, a, b]
+            if c == d:
+                v = v + mu * tr[a, b]
+            Kf[j, i] = v
+    assert np.abs(Kf - K).max() <= 1e-13 * np.abs(K).max()
