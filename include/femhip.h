@@ -168,8 +168,11 @@ int femhip_segment_sum(const double* contrib, const int32_t* perm,
  * (bilinear_form.py:58-148, coo_data.py:28-36) once the CSR pattern is
  * known: every workgroup owns a set of rows (a Morton-ordered node
  * partition), recomputes the local matrices of the elements touching them
- * in registers, sums the contributions in LDS in a fixed colour order and
- * writes each CSR row once.
+ * in registers, sums the contributions in LDS and writes each CSR row
+ * once.  Two deterministic summation modes: colour rounds (floating point,
+ * a fixed order) or fixed-point integers (LDS integer atomics, exact
+ * associativity; error <= n * 3.6e-15 * max|A| for a slot with n
+ * contributions) -- chosen by femhip_fused_dev_t.scale.
  *
  * femhip_fused_plan_build runs on the host (host pointers: t, p, indptr,
  * indices of the pattern) and keeps the plan in library-owned host memory
@@ -202,8 +205,16 @@ typedef struct femhip_fused_dev {
   const int32_t* chunks;
   const uint32_t* recs;
   const int32_t* rec_elem;
+  /* Fixed-point accumulation state: 6*nparts int32 (an exponent pair per
+   * form and partition), every entry set to INT32_MIN before first use and
+   * then owned by the kernel; NULL selects the colour-round kernel. */
+  int32_t* scale;
   femhip_fused_plan_info_t info;
 } femhip_fused_dev_t;
+
+/* LDS bytes per workgroup that femhip_fused_assemble asks for. */
+size_t femhip_fused_lds_bytes(const femhip_fused_plan_info_t* info, int32_t dim,
+                              int32_t with_load, int32_t fixed_point);
 
 /* data[nnz] <- assembled CSR values of `form` (FEMHIP_LAPLACE, FEMHIP_MASS,
  * FEMHIP_KAPPA_LAPLACE with kappa[nelems] per element) on the pattern the
@@ -264,6 +275,16 @@ size_t femhip_sweep_lds_bytes(const femhip_sweep_plan_info_t* info, int32_t dim,
 int femhip_sweep_assemble(const femhip_sweep_dev_t* plan, int32_t form, int32_t dim,
                           const double* p, int64_t nnodes, const double* kappa, double* data,
                           double* load, int32_t groups, femhip_stream_t s);
+
+/* Dirichlet rows on a device CSR (utils.enforce, skfem/utils.py:327-400):
+ * rows D (distinct) are zeroed, their diagonal set to `diag`, b[D] = x[D]
+ * (b, x may be NULL; x NULL means 0).  `missing` (device int32, may be
+ * NULL) is set to 1 when a row of D has no diagonal slot in the pattern --
+ * the reference inserts one (setdiag); the caller then falls back to the
+ * host function, which does. */
+int femhip_enforce_rows(const int32_t* indptr, const int32_t* indices, double* data,
+                        const int32_t* D, int64_t nD, double diag, double* b, const double* x,
+                        int32_t* missing, femhip_stream_t s);
 
 /* ---------------------------------------------------------------------
  * Adjoint of the P1 (kappa-)Laplace + unit-load assembly.  The reference

@@ -33,6 +33,7 @@ struct FusedArgs {
   const double* kappa;
   double* out;
   double* load;      // optional: unit-load vector, one entry per node
+  int32_t* scale;    // fixed-point exponents per (form, partition), or NULL
   int64_t nnodes;
   int32_t nparts;
   int32_t max_slots;
@@ -66,20 +67,35 @@ __device__ __forceinline__ int xcd_remap(int i, int n) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// 1/d: hardware reciprocal estimate refined by two Newton steps
+// r <- r + r (1 - d r) (error squared per step; ~1 ulp at the end).
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __fma_rn(-d, r, 1.0);
+  r = __fma_rn(r, e, r);
+  e = __fma_rn(-d, r, 1.0);
+  return __fma_rn(r, e, r);
+}
+
 // Local matrix of a P1 simplex from its vertex coordinates in LDS: Laplace K_ab = coef * (n_a . n_b) / (d! |det|) with n_a the
 // cross-product (2-D: rotated-edge) gradients scaled by det; mass
 // |det| / (d! (d+1)(d+2)) * (1 + delta_ab).
 template <int D, int FORM>
 __device__ __forceinline__ double element_matrix(const double* X, const int (&lv)[D + 1],
                                                  double coef, double (&K)[D + 1][D + 1]) {
-  // coordinates are staged AoS, 32 B (3-D: x y z pad) or 16 B per node
+  // coordinates are staged AoS: 24 B (x y z) or 16 B (x y) per node
   double x[D + 1][D];
 #pragma unroll
   for (int v = 0; v < D + 1; ++v) {
-    const double2 xy = reinterpret_cast<const double2*>(X)[(D == 3 ? 2 : 1) * lv[v]];
-    x[v][0] = xy.x;
-    x[v][1] = xy.y;
-    if constexpr (D == 3) x[v][2] = X[4 * lv[v] + 2];
+    if constexpr (D == 3) {
+      x[v][0] = X[3 * lv[v]];
+      x[v][1] = X[3 * lv[v] + 1];
+      x[v][2] = X[3 * lv[v] + 2];
+    } else {
+      const double2 xy = reinterpret_cast<const double2*>(X)[lv[v]];
+      x[v][0] = xy.x;
+      x[v][1] = xy.y;
+    }
   }
   if constexpr (D == 3) {
     double c1[3], c2[3], c3[3];
@@ -109,7 +125,7 @@ __device__ __forceinline__ double element_matrix(const double* X, const int (&lv
 #pragma unroll
         for (int j = 0; j < 4; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
     } else {
-      const double s = coef / (6.0 * fabs(det));
+      const double s = coef * recip(6.0 * fabs(det));
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -134,7 +150,7 @@ __device__ __forceinline__ double element_matrix(const double* X, const int (&lv
 #pragma unroll
         for (int j = 0; j < 3; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
     } else {
-      const double s = coef / (2.0 * fabs(det));
+      const double s = coef * recip(2.0 * fabs(det));
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -151,11 +167,11 @@ __device__ __forceinline__ double element_matrix(const double* X, const int (&lv
 template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>
 __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
   constexpr int NV = D + 1;
-  constexpr uint32_t SENT = (1u << OFB) - 1u;
+    constexpr uint32_t SENT = (1u << OFB) - 1u;
   extern __shared__ double smem[];
   double* acc = smem;
   double* X = acc + ((a.max_slots + 3) & ~1);  // 16-byte aligned for b128 loads
-  double* bacc = X + (D == 3 ? 4 : 2) * a.max_nodes;
+  double* bacc = X + D * a.max_nodes;
   uint32_t* rinfo = reinterpret_cast<uint32_t*>(bacc + (LOAD ? a.max_rows : 0));
 
   // persistent: each workgroup walks partitions blockIdx.x, +gridDim.x, ...
@@ -168,11 +184,10 @@ __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
   const int tid = threadIdx.x;
 
   for (int i = tid; i < nslots; i += kFusedBlock) acc[i] = 0.0;
-  constexpr int CS = (D == 3) ? 4 : 2;
   for (int l = tid; l < L; l += kFusedBlock) {
     const int64_t g = a.nodemap[node_start + l];
 #pragma unroll
-    for (int d = 0; d < D; ++d) X[CS * l + d] = a.p[d * a.nnodes + g];
+    for (int d = 0; d < D; ++d) X[D * l + d] = a.p[d * a.nnodes + g];
   }
   for (int r = tid; r < R; r += kFusedBlock) {
     rinfo[r] = (uint32_t)a.rows[2 * (int64_t)(row_start + r) + 1];
@@ -194,7 +209,7 @@ __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
     }
   };
   if (nchunks > 0) fetch(0);
-  double* const dummy = acc + a.max_slots;  // sink for skipped contributions
+  double* const dummy = acc + a.max_slots;  // sink for pruned slots
 
   for (int c = 0; c < nchunks; ++c) {
     const int32_t rec0 = rec_n;
@@ -266,18 +281,257 @@ __global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
   }  // partitions
 }
 
+// ---------------------------------------------------------------------
+// Fixed-point variant.  A contribution v enters the partition's LDS row
+// accumulators as the integer round(v * 2^s) (int64, two's complement),
+// summed with LDS integer atomics: integer addition is associative, so the
+// sums -- and the values written -- do not depend on the order in which
+// lanes and waves arrive (deterministic without colour rounds or barriers).
+//
+// s is chosen per partition (and form) from M = the largest |K_aa| over the
+// partition's elements: s = 47 - ilogb(M), so every contribution
+// |K_ab| <= M (the local matrices are semidefinite) is below 2^48 after
+// scaling.  The rounding error of one contribution is at most 2^-s / 2 <=
+// M * 2^-48, and M <= max|A| (a diagonal entry of A sums same-sign K_aa),
+// so a slot with n contributions is within n * 3.6e-15 * max|A| of its
+// exact sum.  The exponent of the previous call is kept in `scale`; the
+// pass measures M while it accumulates and is repeated with the exponent
+// of the measured M when the two differ (first call, or a changed input),
+// so the values are a function of the input alone.
+// ---------------------------------------------------------------------
+constexpr int kFxUnknown = INT32_MIN;
+constexpr double kFxMagic = 6755399441055744.0;  // 1.5 * 2^52
+
+// round(v * scale) as int64 for |v * scale| < 2^51: the fma rounds the
+// scaled value to an integer held in the low mantissa bits of the sum.
+__device__ __forceinline__ unsigned long long to_fixed(double v, double scale) {
+  const double y = __fma_rn(v, scale, kFxMagic);
+  return (unsigned long long)(__double_as_longlong(y) - __double_as_longlong(kFxMagic));
+}
+
+__device__ __forceinline__ double from_fixed(unsigned long long v, double inv_scale) {
+  return (double)(long long)v * inv_scale;
+}
+
+__device__ __forceinline__ int fx_exponent(double m) {
+  if (!(m > 0.0) || !(m < INFINITY)) return 0;
+  const int s = 47 - ilogb(m);
+  return s > 1000 ? 1000 : (s < -1000 ? -1000 : s);
+}
+
+__device__ __forceinline__ void lds_add(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One pass over a partition's element records: the local matrices, the
+// largest |K_aa| and load entry (mk, mv), and -- when SCATTER -- the
+// fixed-point contributions added into the LDS rows (pruned slots go to a
+// per-lane sink).
+template <int RW>
+struct RecFetch {
+  uint32_t w[RW];
+  int32_t rec = 0;
+  int count = 0;
+  // records of chunk c of the partition's chunk list (lane tid: record tid)
+  __device__ __forceinline__ void load(const FusedArgs& a, int chunk_start, int c, int tid) {
+    rec = a.chunks[2 * (int64_t)(chunk_start + c)];
+    count = a.chunks[2 * (int64_t)(chunk_start + c) + 1] & 0xffff;
+    if (tid < count) {
+      const uint32_t* rp = a.recs + (int64_t)(rec + tid) * RW;
+#pragma unroll
+      for (int k = 0; k < RW; ++k) w[k] = __builtin_nontemporal_load(rp + k);
+    }
+  }
+};
+
+template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD, bool SCATTER>
+__device__ __forceinline__ void fx_pass(const FusedArgs& a, int chunk_start, int nchunks,
+                                        int R, const double* X, const uint32_t* rinfo,
+                                        unsigned long long* acc, unsigned long long* bacc,
+                                        double sck, double scv, RecFetch<RW>& nxt,
+                                        double& mk, double& mv) {
+  constexpr int NV = D + 1;
+  constexpr uint32_t SENT = (1u << OFB) - 1u;
+  unsigned long long* const sink = acc + a.max_slots + (threadIdx.x & 63);
+  constexpr double LOADC = (D == 3) ? 1.0 / 24.0 : 1.0 / 6.0;  // int phi = |det| LOADC
+  const int tid = threadIdx.x;
+  // nxt holds chunk 0 on entry; chunk c+1 is fetched while c is summed
+  for (int c = 0; c < nchunks; ++c) {
+    const int32_t rec0 = nxt.rec;
+    const bool active = tid < nxt.count;
+    uint32_t w[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) w[k] = nxt.w[k];
+    if (c + 1 < nchunks) nxt.load(a, chunk_start, c + 1, tid);
+    if (!active) continue;
+    int lv[NV];
+    uint32_t offs[NV * (NV - 1)];
+    double K[NV][NV];
+    lv[0] = bits<0, IDB, RW>(w);
+    lv[1] = bits<IDB, IDB, RW>(w);
+    lv[2] = bits<2 * IDB, IDB, RW>(w);
+    if constexpr (NV == 4) lv[3] = bits<3 * IDB, IDB, RW>(w);
+    decode_offsets<NV * IDB, OFB, RW>(w, offs, std::make_integer_sequence<int, NV * (NV - 1)>{});
+    const double vol = element_matrix<D, FORM>(
+        X, lv, FORM == FEMHIP_KAPPA_LAPLACE ? a.kappa[a.rec_elem[rec0 + tid]] : 1.0, K);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) mk = fmax(mk, fabs(K[i][i]));
+    if constexpr (LOAD) mv = fmax(mv, vol * LOADC);
+    if constexpr (SCATTER) {
+      unsigned long long F[NV][NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = i; j < NV; ++j) F[i][j] = F[j][i] = to_fixed(K[i][j], sck);
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        if (lv[i] >= R) continue;  // halo vertex: its row belongs elsewhere
+        const uint32_t info = rinfo[lv[i]];
+        unsigned long long* row = acc + (info & 0xffffu);
+        const uint32_t dg = (info >> 16) & 0xffu;
+        lds_add(dg != 0xffu ? row + dg : sink, F[i][i]);
+        int q = 0;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          if (j == i) continue;
+          const uint32_t off = offs[i * (NV - 1) + q];
+          ++q;
+          lds_add(off != SENT ? row + off : sink, F[i][j]);
+        }
+        if constexpr (LOAD) lds_add(bacc + lv[i], to_fixed(vol * LOADC, scv));
+      }
+    }
+  }
+}
+
+// three workgroups per CU without the load vector (at most 80 VGPRs)
+template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>
+__global__ void __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(LOAD ? 4 : 6)))
+p1_fused_fx_kernel(FusedArgs a) {
+  constexpr int NW = kFusedBlock / 64;
+  constexpr int FSLOT = FORM == FEMHIP_LAPLACE ? 0 : (FORM == FEMHIP_MASS ? 1 : 2);
+  extern __shared__ double smem[];
+  // [max_slots accumulators | 64 per-lane sinks | coordinates | load acc |
+  //  2*NW reduction | row info]
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem);
+  double* X = smem + ((a.max_slots + 64 + 1) & ~1);
+  unsigned long long* bacc = reinterpret_cast<unsigned long long*>(X + D * a.max_nodes);
+  double* red = reinterpret_cast<double*>(bacc + (LOAD ? a.max_rows : 0));
+  uint32_t* rinfo = reinterpret_cast<uint32_t*>(red + 2 * NW);
+  int32_t* rstart = reinterpret_cast<int32_t*>(rinfo + a.max_rows);  // CSR row starts
+  const int tid = threadIdx.x;
+
+  for (int pi = blockIdx.x; pi < a.nparts; pi += gridDim.x) {
+    if (pi != (int)blockIdx.x) __syncthreads();  // LDS reuse
+    const int part = xcd_remap(pi, a.nparts);
+    const int32_t* hdr = a.parts + 8 * (int64_t)part;
+    const int node_start = hdr[0], L = hdr[1], row_start = hdr[2], R = hdr[3];
+    const int nslots = hdr[4], chunk_start = hdr[5], nchunks = hdr[6];
+    int32_t* sc = a.scale + 2 * ((int64_t)FSLOT * a.nparts + part);
+    int s = sc[0], sv = sc[1];
+    bool known = s != kFxUnknown && (!LOAD || sv != kFxUnknown);
+
+    for (int l = tid; l < L; l += kFusedBlock) {
+      const int64_t g = a.nodemap[node_start + l];
+#pragma unroll
+      for (int d = 0; d < D; ++d) X[D * l + d] = a.p[d * a.nnodes + g];
+    }
+    for (int r = tid; r < R; r += kFusedBlock) {
+      const int2 rw = reinterpret_cast<const int2*>(a.rows)[row_start + r];
+      rstart[r] = rw.x;
+      rinfo[r] = (uint32_t)rw.y;
+    }
+    RecFetch<RW> nxt;
+    if (nchunks > 0) nxt.load(a, chunk_start, 0, tid);  // overlaps the staging
+
+    for (;;) {
+      if (known) {
+        for (int i = tid; i < nslots; i += kFusedBlock) acc[i] = 0ull;
+        if constexpr (LOAD)
+          for (int r = tid; r < R; r += kFusedBlock) bacc[r] = 0ull;
+      }
+      __syncthreads();  // staging / zeroing before the first contribution
+      double mk = 0.0, mv = 0.0;
+      if (known)
+        fx_pass<D, RW, IDB, OFB, FORM, LOAD, true>(a, chunk_start, nchunks, R, X, rinfo, acc,
+                                                   bacc, ldexp(1.0, s), ldexp(1.0, sv), nxt,
+                                                   mk, mv);
+      else
+        fx_pass<D, RW, IDB, OFB, FORM, LOAD, false>(a, chunk_start, nchunks, R, X, rinfo, acc,
+                                                    bacc, 1.0, 1.0, nxt, mk, mv);
+      // partition maxima -> the exponents this pass should have used
+      mk = wave_max(mk);
+      if constexpr (LOAD) mv = wave_max(mv);
+      if ((tid & 63) == 0) {
+        red[tid >> 6] = mk;
+        red[NW + (tid >> 6)] = mv;
+      }
+      __syncthreads();  // also: every contribution of the pass has landed
+      double MK = red[0], MV = red[NW];
+#pragma unroll
+      for (int k = 1; k < NW; ++k) {
+        MK = fmax(MK, red[k]);
+        MV = fmax(MV, red[NW + k]);
+      }
+      const int s2 = fx_exponent(MK), sv2 = LOAD ? fx_exponent(MV) : sv;
+      if (known && s2 == s && sv2 == sv) break;
+      s = s2;
+      sv = sv2;
+      known = true;
+      if (nchunks > 0) nxt.load(a, chunk_start, 0, tid);
+      __syncthreads();  // reductions read before the next pass reuses red
+    }
+    // write back: 16 lanes per row
+    const double inv = ldexp(1.0, -s);
+    const int lane = tid & 15, grp = tid >> 4;
+    for (int r = grp; r < R; r += kFusedBlock / 16) {
+      const int64_t gstart = rstart[r];
+      const uint32_t info = rinfo[r];
+      const int len = (int)(info >> 24), base = (int)(info & 0xffffu);
+      for (int k = lane; k < len; k += 16)
+        __builtin_nontemporal_store(from_fixed(acc[base + k], inv), a.out + gstart + k);
+    }
+    if constexpr (LOAD) {
+      const double invv = ldexp(1.0, -sv);
+      for (int r = tid; r < R; r += kFusedBlock)
+        a.load[a.nodemap[node_start + r]] = from_fixed(bacc[r], invv);
+    }
+    if (tid == 0) {
+      sc[0] = s;
+      sc[1] = sv;
+    }
+  }  // partitions
+}
+
+
 template <int D, int RW, int IDB, int OFB, bool LOAD>
 int launch_form(const FusedArgs& a, int form, size_t lds, int groups, hipStream_t s) {
   dim3 grid((unsigned)groups), block(kFusedBlock);
+  const bool fx = a.scale != nullptr;
   switch (form) {
     case FEMHIP_LAPLACE:
-      p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
+      if (fx)
+        p1_fused_fx_kernel<D, RW, IDB, OFB, FEMHIP_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
+      else
+        p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
       break;
     case FEMHIP_MASS:
-      p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_MASS, LOAD><<<grid, block, lds, s>>>(a);
+      if (fx)
+        p1_fused_fx_kernel<D, RW, IDB, OFB, FEMHIP_MASS, LOAD><<<grid, block, lds, s>>>(a);
+      else
+        p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_MASS, LOAD><<<grid, block, lds, s>>>(a);
       break;
     case FEMHIP_KAPPA_LAPLACE:
-      p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
+      if (fx)
+        p1_fused_fx_kernel<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
+      else
+        p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
       break;
     default:
       set_error("fused path: laplace, mass or kappa_laplace only");
@@ -307,6 +561,18 @@ int dispatch_fused(const FusedArgs& a, const femhip_fused_plan_info_t& I, int fo
 
 using namespace femhip;
 
+extern "C" size_t femhip_fused_lds_bytes(const femhip_fused_plan_info_t* I, int32_t dim,
+                                         int32_t with_load, int32_t fixed_point) {
+  if (!I) return 0;
+  const size_t coords = 8 * (size_t)dim * I->max_nodes;
+  const size_t load_acc = with_load ? 8 * (size_t)I->max_rows : 0;
+  const size_t rinfo = 4 * (size_t)I->max_rows;
+  if (fixed_point)  // accumulators + 64 sinks, coordinates, load, reduction, row info + starts
+    return 8 * (((size_t)I->max_slots + 64 + 1) & ~(size_t)1) + coords + load_acc +
+           8 * 2 * (kFusedBlock / 64) + 2 * rinfo;
+  return 8 * (((size_t)I->max_slots + 3) & ~(size_t)1) + coords + load_acc + rinfo;
+}
+
 extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t form, int32_t dim,
                                      const double* p, int64_t nnodes, const double* kappa,
                                      double* data, double* load, int32_t groups,
@@ -321,11 +587,9 @@ extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t for
   if (I.nparts == 0) return FEMHIP_OK;
   const int ngroups = (groups == 0 || groups > I.nparts) ? I.nparts : groups;
   FusedArgs a{plan->parts, plan->nodemap, plan->rows, plan->chunks, plan->recs,
-              plan->rec_elem, p, kappa, data, load, nnodes, I.nparts, I.max_slots,
-              I.max_nodes, I.max_rows};
-  const size_t lds = 8 * (((size_t)I.max_slots + 3) & ~(size_t)1) +
-                     8 * (size_t)(dim == 3 ? 4 : 2) * I.max_nodes +
-                     (load ? 8 * (size_t)I.max_rows : 0) + 4 * (size_t)I.max_rows;
+              plan->rec_elem, p, kappa, data, load, plan->scale, nnodes, I.nparts,
+              I.max_slots, I.max_nodes, I.max_rows};
+  const size_t lds = femhip_fused_lds_bytes(&I, dim, load != nullptr, plan->scale != nullptr);
   FEMHIP_REQUIRE(lds <= 160 * 1024, FEMHIP_EINVAL, "fused: plan exceeds LDS");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
   if (load) return dispatch_fused<true>(a, I, form, dim, lds, ngroups, st);

@@ -19,7 +19,9 @@
 // element's vertices (partition-local numbering, owned nodes first) and, for
 // every owned vertex a and other vertex b, the offset of column b inside
 // row a of the CSR pattern (a sentinel when the slot was pruned by
-// eliminate_zeros), plus the colour relative to the chunk.
+// eliminate_zeros), plus the colour relative to the chunk.  Rows are laid
+// out back to back in LDS: with the usual odd row lengths (15 in a
+// tetrahedral interior) consecutive rows start on different LDS banks.
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -74,6 +76,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   // record format: local ids and 4- or 5-bit offsets, 4-bit relative colour
   int max_row = 0;
   for (int64_t n = 0; n < Nv; ++n) max_row = std::max(max_row, indptr[n + 1] - indptr[n]);
+  // offsets 0..len-1 and the all-ones sentinel
   int offbits = max_row <= 15 ? 4 : (max_row <= 31 ? 5 : 0);
   if (offbits == 0) {
     set_error("fused plan: CSR rows longer than 31 entries");
@@ -265,18 +268,23 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
     // rows of the partition
     const int32_t row_start = (int32_t)(P->rows.size() / 2);
     int lds = 0;
-    std::vector<int> diag_off(R, 255), row_len(R, 0), row_lds(R, 0);
+    std::vector<int> row_len(R, 0);
     for (int r = 0; r < R; ++r) {
       const int32_t g = owned[r];
       const int b = indptr[g], en = indptr[g + 1];
       row_len[r] = en - b;
-      row_lds[r] = lds;
+      int diag = 255;  // pruned diagonal
       for (int k = b; k < en; ++k)
-        if (indices[k] == g) diag_off[r] = k - b;
+        if (indices[k] == g) diag = k - b;
       P->rows.push_back(b);
-      P->rows.push_back((int32_t)((uint32_t)lds | ((uint32_t)diag_off[r] << 16) |
+      P->rows.push_back((int32_t)((uint32_t)lds | ((uint32_t)diag << 16) |
                                   ((uint32_t)row_len[r] << 24)));
       lds += row_len[r];
+    }
+    if (lds > 65535) {
+      delete P;
+      set_error("fused plan: a block exceeds 65535 LDS slots");
+      return FEMHIP_EUNSUPPORTED;
     }
     // chunks and records
     const int32_t chunk_start = (int32_t)(P->chunks.size() / 2);

@@ -58,7 +58,7 @@ class FusedInfo(ctypes.Structure):
 
 class FusedDev(ctypes.Structure):
     _fields_ = [('parts', _p), ('nodemap', _p), ('rows', _p), ('chunks', _p),
-                ('recs', _p), ('rec_elem', _p), ('info', FusedInfo)]
+                ('recs', _p), ('rec_elem', _p), ('scale', _p), ('info', FusedInfo)]
 
 
 class SweepInfo(ctypes.Structure):
@@ -105,6 +105,8 @@ SIGNATURES = {
                                                ctypes.POINTER(FusedInfo)]),
     'femhip_fused_plan_export': (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
     'femhip_fused_plan_free': (None, [_p]),
+    'femhip_fused_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(FusedInfo), _i32, _i32,
+                                                 _i32]),
     'femhip_fused_assemble': (ctypes.c_int, [ctypes.POINTER(FusedDev), _i32, _i32, _p,
                                              _i64, _p, _p, _p, _i32, _p]),
     'femhip_batched_kappa_values': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _p, _i64, _p,
@@ -120,6 +122,7 @@ SIGNATURES = {
     'femhip_sweep_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(SweepInfo), _i32, _i32]),
     'femhip_sweep_assemble': (ctypes.c_int, [ctypes.POINTER(SweepDev), _i32, _i32, _p, _i64,
                                              _p, _p, _p, _i32, _p]),
+    'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
     'femhip_p1_adjoint': (ctypes.c_int, [_i32, _i64, _i64, _p, _p, _p, _p, _p, _p, _p,
                                          _p, _p]),

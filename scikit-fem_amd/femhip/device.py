@@ -208,7 +208,7 @@ class DeviceBasis:
         if fp is None:
             torch.cuda.current_stream().synchronize()
             ip, ix = plan.indptr.cpu().numpy(), plan.indices.cpu().numpy()
-            variant = os.environ.get('FEMHIP_FUSED_VARIANT', 'sweep')
+            variant = os.environ.get('FEMHIP_FUSED_VARIANT', 'blocks')
             fp = None
             if variant == 'sweep':
                 try:

@@ -15,11 +15,16 @@ from . import _lib
 
 import os
 
-# LDS budget per workgroup: two workgroups per CU (2 x 80 KB of 160 KB);
-# FEMHIP_FUSED_SLOT_CAP / FEMHIP_FUSED_NODE_CAP override it (tuning).
-DEFAULT_SLOT_CAP = int(os.environ.get('FEMHIP_FUSED_SLOT_CAP', 6144))
-DEFAULT_NODE_CAP = int(os.environ.get('FEMHIP_FUSED_NODE_CAP', 1024))
+# LDS budget per workgroup: three workgroups per CU (3 x 53 KB of 160 KB:
+# 4096 int64 row accumulators, 768 staged nodes); FEMHIP_FUSED_SLOT_CAP /
+# FEMHIP_FUSED_NODE_CAP override it (tuning).
+DEFAULT_SLOT_CAP = int(os.environ.get('FEMHIP_FUSED_SLOT_CAP', 4096))
+DEFAULT_NODE_CAP = int(os.environ.get('FEMHIP_FUSED_NODE_CAP', 768))
 BLOCK = 512
+# summation mode: 'fixed' (fixed-point LDS integer atomics) or 'rounds'
+# (floating-point colour rounds); both deterministic
+DEFAULT_ACCUM = os.environ.get('FEMHIP_FUSED_ACCUM', 'fixed')
+FX_UNKNOWN = -2 ** 31
 
 _FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS, 'kappa_laplace': _lib.KAPPA_LAPLACE}
 
@@ -34,8 +39,11 @@ class FusedPlan:
     """Device arrays of one fused plan plus its info struct."""
 
     def __init__(self, mesh, indptr, indices, slot_cap=DEFAULT_SLOT_CAP,
-                 node_cap=DEFAULT_NODE_CAP):
+                 node_cap=DEFAULT_NODE_CAP, accum=None):
         lib = _lib.load()
+        self.accum = accum or DEFAULT_ACCUM
+        if self.accum not in ('fixed', 'rounds'):
+            raise ValueError("accum must be 'fixed' or 'rounds'")
         t = np.ascontiguousarray(mesh.t, dtype=np.int32)
         p = np.ascontiguousarray(mesh.p, dtype=np.float64)
         ip = np.ascontiguousarray(indptr, dtype=np.int32)
@@ -78,9 +86,16 @@ class FusedPlan:
         self.chunks = torch.from_numpy(h['chunks']).to(dev)
         self.recs = torch.from_numpy(h['recs'].view(np.int32)).to(dev)
         self.rec_elem = torch.from_numpy(h['rec_elem']).to(dev)
+        self.scale = None
+        if self.accum == 'fixed':
+            # exponent pair per (form, partition); owned by the kernel
+            self.scale = torch.full((6 * self.info.nparts,), FX_UNKNOWN, dtype=torch.int32,
+                                    device=dev)
         self.dev = _lib.FusedDev(self.parts.data_ptr(), self.nodemap.data_ptr(),
                                  self.rows.data_ptr(), self.chunks.data_ptr(),
-                                 self.recs.data_ptr(), self.rec_elem.data_ptr(), self.info)
+                                 self.recs.data_ptr(), self.rec_elem.data_ptr(),
+                                 None if self.scale is None else self.scale.data_ptr(),
+                                 self.info)
 
     @classmethod
     def concat(cls, plans):
@@ -92,6 +107,7 @@ class FusedPlan:
         if len(fmt) != 1:
             raise NotImplementedError("meshes need the same record format to be batched")
         out = cls.__new__(cls)
+        out.accum = plans[0].accum
         parts, nodemap, rows, chunks, recs, rec_elem = [], [], [], [], [], []
         off_node = off_nnz = off_map = off_row = off_chunk = off_rec = off_el = 0
         info = _lib.FusedInfo()
@@ -140,9 +156,14 @@ class FusedPlan:
         out._upload()
         return out
 
+    def lds_bytes(self, load=False):
+        return int(_lib.load().femhip_fused_lds_bytes(ctypes.byref(self.info), self.dim,
+                                                      int(load), int(self.accum == 'fixed')))
+
     def stats(self):
         i = self.info
-        return dict(parts=i.nparts, records=int(i.nrecs), dup=i.dup,
+        return dict(accum=self.accum, lds_bytes=self.lds_bytes(),
+                    parts=i.nparts, records=int(i.nrecs), dup=i.dup,
                     max_colors=i.max_colors, rec_bytes=4 * i.rec_words,
                     max_slots=i.max_slots, max_nodes=i.max_nodes,
                     plan_bytes=int(sum(t.numel() * t.element_size() for t in (

@@ -34,3 +34,22 @@ def enforce(A, b=None, x=None, I=None, D=None, diag=1., overwrite=False):
     bout = b if overwrite else b.copy()
     bout[D] = x[D]
     return Aout, bout
+
+
+def enforce_device(A, b=None, x=None, D=None, diag=1.):
+    """``enforce`` on a ``DeviceCSR`` in place (rows of D zeroed, diagonal
+    set, b[D] = x[D]); torch tensors stay on the device.  Returns (A, b)."""
+    import torch
+    from . import _lib
+    lib = _lib.load()
+    dev = A.data.device
+    Dt = torch.as_tensor(np.unique(np.asarray(D)), dtype=torch.int32, device=dev)
+    missing = torch.zeros(1, dtype=torch.int32, device=dev)
+    xt = None if x is None else torch.as_tensor(x, dtype=torch.float64, device=dev)
+    _lib.check(lib.femhip_enforce_rows(
+        _lib.ptr(A.indptr), _lib.ptr(A.indices), _lib.ptr(A.data), _lib.ptr(Dt), Dt.numel(),
+        float(diag), _lib.ptr(b), _lib.ptr(xt), _lib.ptr(missing), _lib.stream_ptr()))
+    if int(missing.item()):
+        raise ValueError("a Dirichlet row has no diagonal slot; use femhip.enforce on the "
+                         "SciPy matrix (it inserts one)")
+    return A, b

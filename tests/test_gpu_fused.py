@@ -145,3 +145,64 @@ def test_sweep_variant(kind, n, a, br, ck, form, load):
     if load:
         bref = unit_load.assemble(b)
         assert np.abs(ld.cpu().numpy() - bref).max() <= TOL * np.abs(bref).max()
+
+
+@pytest.mark.parametrize('kind,n,a,caps', [
+    ('tet', 12, 0.2, (6144, 1024)), ('tet', 12, 0.2, (600, 120)),
+    ('tri', 40, 0.2, (6144, 1024)), ('tri', 40, 0.0, (500, 200)),
+])
+def test_fixed_point_accumulation(kind, n, a, caps):
+    """Fixed-point LDS accumulation: every form, with and without the load
+    vector, persistent workgroups; values within the tolerance of the exact
+    path, bitwise repeatable, and independent of what the plan assembled
+    before (the per-partition exponents are re-derived from the input)."""
+    from femhip.fused import FusedPlan
+    m = _mesh(kind, n, a)
+    el = femhip.ElementTriP1() if kind == 'tri' else femhip.ElementTetP1()
+    b = femhip.Basis(m, el)
+    d = b.device()
+    rng = np.random.default_rng(11)
+    k1, k2 = (rng.lognormal(0, 1, m.nelements) for _ in range(2))
+    k2 *= 1e6                         # a very different scale: exponents change
+    plans = {}
+
+    def plans_for(A):
+        # one plan pair per CSR pattern (mass keeps entries Laplace prunes)
+        key = _sha(A.indices.cpu().numpy())
+        if key not in plans:
+            ip, ix = A.indptr.cpu().numpy(), A.indices.cpu().numpy()
+            plans[key] = (FusedPlan(m, ip, ix, *caps, accum='fixed'),
+                          FusedPlan(m, ip, ix, *caps, accum='rounds'))
+            assert plans[key][0].info.nparts > 1
+        return plans[key]
+    bref = unit_load.assemble(b)
+    results = {}
+    for step, (form, kk, load, groups) in enumerate([
+            ('laplace', None, True, 0), ('mass', None, False, 5),
+            ('kappa_laplace', k1, False, 0), ('kappa_laplace', k2, True, 3),
+            ('kappa_laplace', k1, True, 0), ('laplace', None, False, 2)]):
+        print('step', step, form, flush=True)
+        kt = None if kk is None else torch.tensor(kk, device='cuda')
+        kw = {} if kk is None else {'kappa': kk[:, None]}
+        f = {'laplace': laplace, 'mass': mass, 'kappa_laplace': kappa_laplace}[form]
+        A = f.assemble_device(b, path='generic', **kw)
+        fx, rd = plans_for(A)
+        ref = A.data.cpu().numpy()
+        out = torch.empty(fx.nnz, dtype=torch.float64, device='cuda')
+        ld = torch.empty(b.N, dtype=torch.float64, device='cuda') if load else None
+        fx.assemble(form, d.p, out, kappa=kt, load=ld, groups=groups)
+        out2 = torch.empty_like(out)
+        fx.assemble(form, d.p, out2, kappa=kt, groups=groups)
+        out3 = torch.empty_like(out)
+        rd.assemble(form, d.p, out3, kappa=kt)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        assert np.abs(got - ref).max() <= TOL * np.abs(ref).max()
+        assert np.abs(out3.cpu().numpy() - ref).max() <= TOL * np.abs(ref).max()
+        assert np.array_equal(got, out2.cpu().numpy())
+        if load:
+            assert np.abs(ld.cpu().numpy() - bref).max() <= TOL * np.abs(bref).max()
+        key = (form, None if kk is None else float(kk[0]))
+        if key in results:           # same input after other inputs: same bits
+            assert np.array_equal(results[key], got)
+        results[key] = got

@@ -167,3 +167,20 @@ def test_zero_jacobian_raises():
     b = femhip.Basis(m, femhip.ElementHex1())
     with pytest.raises(Exception, match="Zero Jacobian"):
         laplace.assemble(b)
+
+
+def test_enforce_on_device():
+    import torch
+    from femhip.utils import enforce_device
+    x = np.linspace(0, 1, 33)
+    m = femhip.MeshTri.init_tensor(x, x).jittered(0.2, 1 / 32)
+    b = femhip.Basis(m, femhip.ElementTriP1())
+    A = laplace.assemble_device(b)
+    f = unit_load.assemble_device(b)
+    D = m.boundary_nodes()
+    xD = np.random.default_rng(0).standard_normal(b.N)
+    Ah, fh = femhip.enforce(A.to_scipy(), f.cpu().numpy(), x=xD, D=D)
+    enforce_device(A, f, x=xD, D=D)
+    torch.cuda.synchronize()
+    assert np.array_equal(A.data.cpu().numpy(), Ah.data)
+    assert np.array_equal(f.cpu().numpy(), fh)
