@@ -268,6 +268,7 @@ def main():
         def step():
             fp.assemble('laplace', dvb.p, out)
         kernel_name = ('p1_sweep_kernel' if getattr(fp, 'kind', '') == 'sweep'
+                       else 'p1_fused_fx_kernel' if getattr(fp, 'accum', '') == 'fixed'
                        else 'p1_fused_kernel')
     else:
         plan = next(iter(dvb.plans.values()))

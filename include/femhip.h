@@ -205,9 +205,9 @@ typedef struct femhip_fused_dev {
   const int32_t* chunks;
   const uint32_t* recs;
   const int32_t* rec_elem;
-  /* Fixed-point accumulation state: 6*nparts int32 (an exponent pair per
-   * form and partition), every entry set to INT32_MIN before first use and
-   * then owned by the kernel; NULL selects the colour-round kernel. */
+  /* Fixed-point accumulation state, owned by the kernel: 6*nparts int32,
+   * an exponent pair per form and partition, every entry set to INT32_MIN
+   * before first use.  NULL selects the colour-round kernel. */
   int32_t* scale;
   femhip_fused_plan_info_t info;
 } femhip_fused_dev_t;

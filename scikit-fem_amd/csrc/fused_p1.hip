@@ -333,43 +333,60 @@ __device__ __forceinline__ double wave_max(double v) {
 // largest |K_aa| and load entry (mk, mv), and -- when SCATTER -- the
 // fixed-point contributions added into the LDS rows (pruned slots go to a
 // per-lane sink).
+// Element records are fetched kPrefetch chunks ahead of their use (deeper
+// rings measured no faster on the 1M-DoF tetrahedral mesh: the element
+// loop is bound by LDS and issue, not by the record loads).
+#ifndef FEMHIP_FUSED_PREFETCH
+#define FEMHIP_FUSED_PREFETCH 1
+#endif
+constexpr int kPrefetch = FEMHIP_FUSED_PREFETCH;
+static_assert(kPrefetch >= 1 && kPrefetch <= 3, "record prefetch depth 1 to 3");
+
 template <int RW>
-struct RecFetch {
-  uint32_t w[RW];
-  int32_t rec = 0;
-  int count = 0;
-  // records of chunk c of the partition's chunk list (lane tid: record tid)
+struct RecRing {
+  uint32_t w[kPrefetch][RW];
+  int32_t rec[kPrefetch];
+  int count[kPrefetch];
+  // records of chunk c of the partition's chunk list into slot S (lane tid:
+  // record tid of the chunk)
+  template <int S>
   __device__ __forceinline__ void load(const FusedArgs& a, int chunk_start, int c, int tid) {
-    rec = a.chunks[2 * (int64_t)(chunk_start + c)];
-    count = a.chunks[2 * (int64_t)(chunk_start + c) + 1] & 0xffff;
-    if (tid < count) {
-      const uint32_t* rp = a.recs + (int64_t)(rec + tid) * RW;
+    rec[S] = a.chunks[2 * (int64_t)(chunk_start + c)];
+    count[S] = a.chunks[2 * (int64_t)(chunk_start + c) + 1] & 0xffff;
+    if (tid < count[S]) {
+      const uint32_t* rp = a.recs + (int64_t)(rec[S] + tid) * RW;
 #pragma unroll
-      for (int k = 0; k < RW; ++k) w[k] = __builtin_nontemporal_load(rp + k);
+      for (int k = 0; k < RW; ++k) w[S][k] = __builtin_nontemporal_load(rp + k);
     }
+  }
+  // chunks 0 .. kPrefetch-1
+  __device__ __forceinline__ void prime(const FusedArgs& a, int chunk_start, int nchunks,
+                                        int tid) {
+    if (nchunks > 0) load<0>(a, chunk_start, 0, tid);
+    if constexpr (kPrefetch > 1)
+      if (nchunks > 1) load<(kPrefetch > 1 ? 1 : 0)>(a, chunk_start, 1, tid);
+    if constexpr (kPrefetch > 2)
+      if (nchunks > 2) load<kPrefetch - 1>(a, chunk_start, 2, tid);
   }
 };
 
+// One pass over a partition's element records: the local matrices, the
+// largest |K_aa| and load entry (mk, mv), and -- when SCATTER -- the
+// fixed-point contributions added into the LDS rows (pruned slots go to a
+// per-lane sink).  `ring` holds chunks 0..kPrefetch-1 on entry.
 template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD, bool SCATTER>
 __device__ __forceinline__ void fx_pass(const FusedArgs& a, int chunk_start, int nchunks,
                                         int R, const double* X, const uint32_t* rinfo,
                                         unsigned long long* acc, unsigned long long* bacc,
-                                        double sck, double scv, RecFetch<RW>& nxt,
+                                        double sck, double scv, RecRing<RW>& ring,
                                         double& mk, double& mv) {
   constexpr int NV = D + 1;
   constexpr uint32_t SENT = (1u << OFB) - 1u;
   unsigned long long* const sink = acc + a.max_slots + (threadIdx.x & 63);
   constexpr double LOADC = (D == 3) ? 1.0 / 24.0 : 1.0 / 6.0;  // int phi = |det| LOADC
   const int tid = threadIdx.x;
-  // nxt holds chunk 0 on entry; chunk c+1 is fetched while c is summed
-  for (int c = 0; c < nchunks; ++c) {
-    const int32_t rec0 = nxt.rec;
-    const bool active = tid < nxt.count;
-    uint32_t w[RW];
-#pragma unroll
-    for (int k = 0; k < RW; ++k) w[k] = nxt.w[k];
-    if (c + 1 < nchunks) nxt.load(a, chunk_start, c + 1, tid);
-    if (!active) continue;
+
+  auto element = [&](const uint32_t (&w)[RW], int32_t rec0) {
     int lv[NV];
     uint32_t offs[NV * (NV - 1)];
     double K[NV][NV];
@@ -407,18 +424,40 @@ __device__ __forceinline__ void fx_pass(const FusedArgs& a, int chunk_start, int
         if constexpr (LOAD) lds_add(bacc + lv[i], to_fixed(vol * LOADC, scv));
       }
     }
+  };
+  // chunk c uses ring slot c % kPrefetch, which is refilled with chunk
+  // c + kPrefetch before the element work of chunk c
+  auto step = [&](auto slot, int c) {
+    constexpr int S = decltype(slot)::value;
+    if (c >= nchunks) return;
+    uint32_t w[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) w[k] = ring.w[S][k];
+    const int32_t rec0 = ring.rec[S];
+    const bool active = tid < ring.count[S];
+    if (c + kPrefetch < nchunks) ring.template load<S>(a, chunk_start, c + kPrefetch, tid);
+    if (active) element(w, rec0);
+  };
+  for (int c = 0; c < nchunks; c += kPrefetch) {
+    step(std::integral_constant<int, 0>{}, c);
+    if constexpr (kPrefetch > 1) step(std::integral_constant<int, (kPrefetch > 1 ? 1 : 0)>{}, c + 1);
+    if constexpr (kPrefetch > 2) step(std::integral_constant<int, kPrefetch - 1>{}, c + 2);
   }
 }
 
-// three workgroups per CU without the load vector (at most 80 VGPRs)
-template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>
-__global__ void __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(LOAD ? 4 : 6)))
+// Fixed-point kernel: one workgroup per partition by default (a smaller
+// grid walks the partitions persistently).
+// NT = 512: three workgroups per CU without the load vector (at most 80
+// VGPRs); NT = 1024: two workgroups per CU (at most 64 VGPRs)
+template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD, int NT>
+__global__ void __launch_bounds__(NT)
+    __attribute__((amdgpu_waves_per_eu(LOAD ? 4 : (NT == 1024 ? 8 : 6))))
 p1_fused_fx_kernel(FusedArgs a) {
-  constexpr int NW = kFusedBlock / 64;
+  constexpr int NW = NT / 64;
   constexpr int FSLOT = FORM == FEMHIP_LAPLACE ? 0 : (FORM == FEMHIP_MASS ? 1 : 2);
   extern __shared__ double smem[];
   // [max_slots accumulators | 64 per-lane sinks | coordinates | load acc |
-  //  2*NW reduction | row info]
+  //  2*NW reduction | row info | row starts]
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem);
   double* X = smem + ((a.max_slots + 64 + 1) & ~1);
   unsigned long long* bacc = reinterpret_cast<unsigned long long*>(X + D * a.max_nodes);
@@ -437,34 +476,34 @@ p1_fused_fx_kernel(FusedArgs a) {
     int s = sc[0], sv = sc[1];
     bool known = s != kFxUnknown && (!LOAD || sv != kFxUnknown);
 
-    for (int l = tid; l < L; l += kFusedBlock) {
+    for (int l = tid; l < L; l += NT) {
       const int64_t g = a.nodemap[node_start + l];
 #pragma unroll
       for (int d = 0; d < D; ++d) X[D * l + d] = a.p[d * a.nnodes + g];
     }
-    for (int r = tid; r < R; r += kFusedBlock) {
+    for (int r = tid; r < R; r += NT) {
       const int2 rw = reinterpret_cast<const int2*>(a.rows)[row_start + r];
       rstart[r] = rw.x;
       rinfo[r] = (uint32_t)rw.y;
     }
-    RecFetch<RW> nxt;
-    if (nchunks > 0) nxt.load(a, chunk_start, 0, tid);  // overlaps the staging
+    RecRing<RW> ring;
+    ring.prime(a, chunk_start, nchunks, tid);  // overlaps the staging
 
     for (;;) {
       if (known) {
-        for (int i = tid; i < nslots; i += kFusedBlock) acc[i] = 0ull;
+        for (int i = tid; i < nslots; i += NT) acc[i] = 0ull;
         if constexpr (LOAD)
-          for (int r = tid; r < R; r += kFusedBlock) bacc[r] = 0ull;
+          for (int r = tid; r < R; r += NT) bacc[r] = 0ull;
       }
       __syncthreads();  // staging / zeroing before the first contribution
       double mk = 0.0, mv = 0.0;
       if (known)
         fx_pass<D, RW, IDB, OFB, FORM, LOAD, true>(a, chunk_start, nchunks, R, X, rinfo, acc,
-                                                   bacc, ldexp(1.0, s), ldexp(1.0, sv), nxt,
+                                                   bacc, ldexp(1.0, s), ldexp(1.0, sv), ring,
                                                    mk, mv);
       else
         fx_pass<D, RW, IDB, OFB, FORM, LOAD, false>(a, chunk_start, nchunks, R, X, rinfo, acc,
-                                                    bacc, 1.0, 1.0, nxt, mk, mv);
+                                                    bacc, 1.0, 1.0, ring, mk, mv);
       // partition maxima -> the exponents this pass should have used
       mk = wave_max(mk);
       if constexpr (LOAD) mv = wave_max(mv);
@@ -484,13 +523,13 @@ p1_fused_fx_kernel(FusedArgs a) {
       s = s2;
       sv = sv2;
       known = true;
-      if (nchunks > 0) nxt.load(a, chunk_start, 0, tid);
+      ring.prime(a, chunk_start, nchunks, tid);
       __syncthreads();  // reductions read before the next pass reuses red
     }
     // write back: 16 lanes per row
     const double inv = ldexp(1.0, -s);
     const int lane = tid & 15, grp = tid >> 4;
-    for (int r = grp; r < R; r += kFusedBlock / 16) {
+    for (int r = grp; r < R; r += NT / 16) {
       const int64_t gstart = rstart[r];
       const uint32_t info = rinfo[r];
       const int len = (int)(info >> 24), base = (int)(info & 0xffffu);
@@ -499,7 +538,7 @@ p1_fused_fx_kernel(FusedArgs a) {
     }
     if constexpr (LOAD) {
       const double invv = ldexp(1.0, -sv);
-      for (int r = tid; r < R; r += kFusedBlock)
+      for (int r = tid; r < R; r += NT)
         a.load[a.nodemap[node_start + r]] = from_fixed(bacc[r], invv);
     }
     if (tid == 0) {
@@ -510,28 +549,35 @@ p1_fused_fx_kernel(FusedArgs a) {
 }
 
 
+// groups == 0: one workgroup per partition; fewer: persistent workgroups
+template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>
+void launch_one(const FusedArgs& a, int nt, size_t lds, int groups, hipStream_t s) {
+  if (a.scale == nullptr) {
+    const int g = (groups == 0 || groups > a.nparts) ? a.nparts : groups;
+    p1_fused_kernel<D, RW, IDB, OFB, FORM, LOAD><<<dim3(g), dim3(kFusedBlock), lds, s>>>(a);
+    return;
+  }
+  auto launch = [&](auto kernel, int threads) {
+    const int g = (groups == 0 || groups > a.nparts) ? a.nparts : groups;
+    kernel<<<dim3(g), dim3(threads), lds, s>>>(a);
+  };
+  if (nt == 1024)
+    launch(p1_fused_fx_kernel<D, RW, IDB, OFB, FORM, LOAD, 1024>, 1024);
+  else
+    launch(p1_fused_fx_kernel<D, RW, IDB, OFB, FORM, LOAD, 512>, 512);
+}
+
 template <int D, int RW, int IDB, int OFB, bool LOAD>
-int launch_form(const FusedArgs& a, int form, size_t lds, int groups, hipStream_t s) {
-  dim3 grid((unsigned)groups), block(kFusedBlock);
-  const bool fx = a.scale != nullptr;
+int launch_form(const FusedArgs& a, int form, int nt, size_t lds, int groups, hipStream_t s) {
   switch (form) {
     case FEMHIP_LAPLACE:
-      if (fx)
-        p1_fused_fx_kernel<D, RW, IDB, OFB, FEMHIP_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
-      else
-        p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
+      launch_one<D, RW, IDB, OFB, FEMHIP_LAPLACE, LOAD>(a, nt, lds, groups, s);
       break;
     case FEMHIP_MASS:
-      if (fx)
-        p1_fused_fx_kernel<D, RW, IDB, OFB, FEMHIP_MASS, LOAD><<<grid, block, lds, s>>>(a);
-      else
-        p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_MASS, LOAD><<<grid, block, lds, s>>>(a);
+      launch_one<D, RW, IDB, OFB, FEMHIP_MASS, LOAD>(a, nt, lds, groups, s);
       break;
     case FEMHIP_KAPPA_LAPLACE:
-      if (fx)
-        p1_fused_fx_kernel<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
-      else
-        p1_fused_kernel<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD><<<grid, block, lds, s>>>(a);
+      launch_one<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD>(a, nt, lds, groups, s);
       break;
     default:
       set_error("fused path: laplace, mass or kappa_laplace only");
@@ -545,13 +591,13 @@ template <bool LOAD>
 int dispatch_fused(const FusedArgs& a, const femhip_fused_plan_info_t& I, int form, int dim,
                    size_t lds, int groups, hipStream_t st) {
   if (dim == 3 && I.rec_words == 3 && I.idbits == 11 && I.offbits == 4)
-    return launch_form<3, 3, 11, 4, LOAD>(a, form, lds, groups, st);
+    return launch_form<3, 3, 11, 4, LOAD>(a, form, I.block, lds, groups, st);
   if (dim == 3 && I.rec_words == 4 && I.idbits == 16 && I.offbits == 5)
-    return launch_form<3, 4, 16, 5, LOAD>(a, form, lds, groups, st);
+    return launch_form<3, 4, 16, 5, LOAD>(a, form, I.block, lds, groups, st);
   if (dim == 2 && I.rec_words == 2 && I.idbits == 12 && I.offbits == 4)
-    return launch_form<2, 2, 12, 4, LOAD>(a, form, lds, groups, st);
+    return launch_form<2, 2, 12, 4, LOAD>(a, form, I.block, lds, groups, st);
   if (dim == 2 && I.rec_words == 3 && I.idbits == 16 && I.offbits == 5)
-    return launch_form<2, 3, 16, 5, LOAD>(a, form, lds, groups, st);
+    return launch_form<2, 3, 16, 5, LOAD>(a, form, I.block, lds, groups, st);
   set_error("fused: unsupported record format");
   return FEMHIP_EUNSUPPORTED;
 }
@@ -569,7 +615,7 @@ extern "C" size_t femhip_fused_lds_bytes(const femhip_fused_plan_info_t* I, int3
   const size_t rinfo = 4 * (size_t)I->max_rows;
   if (fixed_point)  // accumulators + 64 sinks, coordinates, load, reduction, row info + starts
     return 8 * (((size_t)I->max_slots + 64 + 1) & ~(size_t)1) + coords + load_acc +
-           8 * 2 * (kFusedBlock / 64) + 2 * rinfo;
+           8 * 2 * (I->block / 64) + 2 * rinfo;
   return 8 * (((size_t)I->max_slots + 3) & ~(size_t)1) + coords + load_acc + rinfo;
 }
 
@@ -580,12 +626,14 @@ extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t for
   FEMHIP_REQUIRE(plan != nullptr && p != nullptr && data != nullptr, FEMHIP_EINVAL,
                  "fused: NULL argument");
   const femhip_fused_plan_info_t& I = plan->info;
-  FEMHIP_REQUIRE(I.block == kFusedBlock, FEMHIP_EINVAL, "fused: plan built for another block");
+  FEMHIP_REQUIRE(I.block == kFusedBlock || (plan->scale != nullptr && I.block == 1024),
+                 FEMHIP_EINVAL,
+                 "fused: plan block must be 512 (or 1024 with fixed-point accumulation)");
   FEMHIP_REQUIRE(form != FEMHIP_KAPPA_LAPLACE || (kappa && plan->rec_elem), FEMHIP_EINVAL,
                  "fused: kappa_laplace needs kappa and rec_elem");
   FEMHIP_REQUIRE(groups >= 0, FEMHIP_EINVAL, "fused: negative group count");
   if (I.nparts == 0) return FEMHIP_OK;
-  const int ngroups = (groups == 0 || groups > I.nparts) ? I.nparts : groups;
+  const int ngroups = groups;  // resolved per kernel in launch_one
   FusedArgs a{plan->parts, plan->nodemap, plan->rows, plan->chunks, plan->recs,
               plan->rec_elem, p, kappa, data, load, plan->scale, nnodes, I.nparts,
               I.max_slots, I.max_nodes, I.max_rows};

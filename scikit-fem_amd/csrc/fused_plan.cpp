@@ -223,19 +223,25 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
       set_error("fused plan: a block exceeds the local node budget");
       return FEMHIP_EUNSUPPORTED;
     }
-    // local numbering: owned first (partition order), then halo
+    // local numbering: owned rows first, then halo nodes, each in global
+    // order -- consecutive owned rows are then mostly consecutive CSR rows
+    // (the write-back of neighbouring lanes lands in the same cache lines)
+    // and the coordinate gathers of the staging step are coalesced
+    std::sort(owned.begin(), owned.end());
     const int R = (int)owned.size();
     for (int r = 0; r < R; ++r) lid[owned[r]] = r;
-    int L = R;
     std::vector<int32_t> locals(owned.begin(), owned.end());
     for (int32_t e : elems)
       for (int j = 0; j < NV; ++j) {
         const int32_t v = t[(int64_t)j * E + e];
-        if (lid[v] < 0) {
-          lid[v] = L++;
+        if (lid[v] == -1) {
+          lid[v] = -2;  // halo, numbered below
           locals.push_back(v);
         }
       }
+    std::sort(locals.begin() + R, locals.end());
+    const int L = (int)locals.size();
+    for (int l = R; l < L; ++l) lid[locals[l]] = l;
     // colouring: no two elements of a colour share an owned vertex
     used.assign(R, 0);
     color.assign(elems.size(), 0);

@@ -20,7 +20,8 @@ import os
 # FEMHIP_FUSED_NODE_CAP override it (tuning).
 DEFAULT_SLOT_CAP = int(os.environ.get('FEMHIP_FUSED_SLOT_CAP', 4096))
 DEFAULT_NODE_CAP = int(os.environ.get('FEMHIP_FUSED_NODE_CAP', 768))
-BLOCK = 512
+# workgroup size: 512, or 1024 (fixed-point accumulation only)
+BLOCK = int(os.environ.get('FEMHIP_FUSED_BLOCK', 512))
 # summation mode: 'fixed' (fixed-point LDS integer atomics) or 'rounds'
 # (floating-point colour rounds); both deterministic
 DEFAULT_ACCUM = os.environ.get('FEMHIP_FUSED_ACCUM', 'fixed')
@@ -39,11 +40,14 @@ class FusedPlan:
     """Device arrays of one fused plan plus its info struct."""
 
     def __init__(self, mesh, indptr, indices, slot_cap=DEFAULT_SLOT_CAP,
-                 node_cap=DEFAULT_NODE_CAP, accum=None):
+                 node_cap=DEFAULT_NODE_CAP, accum=None, block=None):
         lib = _lib.load()
         self.accum = accum or DEFAULT_ACCUM
         if self.accum not in ('fixed', 'rounds'):
             raise ValueError("accum must be 'fixed' or 'rounds'")
+        block = int(block or BLOCK)
+        if block not in (512, 1024) or (block == 1024 and self.accum != 'fixed'):
+            raise ValueError("block must be 512, or 1024 with fixed-point accumulation")
         t = np.ascontiguousarray(mesh.t, dtype=np.int32)
         p = np.ascontiguousarray(mesh.p, dtype=np.float64)
         ip = np.ascontiguousarray(indptr, dtype=np.int32)
@@ -54,7 +58,7 @@ class FusedPlan:
             t.shape[0], p.shape[0], t.shape[1], p.shape[1],
             t.ctypes.data_as(ctypes.c_void_p), p.ctypes.data_as(ctypes.c_void_p),
             ip.ctypes.data_as(ctypes.c_void_p), ix.ctypes.data_as(ctypes.c_void_p),
-            slot_cap, node_cap, BLOCK, ctypes.byref(handle), ctypes.byref(info)))
+            slot_cap, node_cap, block, ctypes.byref(handle), ctypes.byref(info)))
         try:
             parts = np.empty(8 * info.nparts, np.int32)
             nodemap = np.empty(info.nlocal, np.int32)

@@ -147,11 +147,12 @@ def test_sweep_variant(kind, n, a, br, ck, form, load):
         assert np.abs(ld.cpu().numpy() - bref).max() <= TOL * np.abs(bref).max()
 
 
-@pytest.mark.parametrize('kind,n,a,caps', [
-    ('tet', 12, 0.2, (6144, 1024)), ('tet', 12, 0.2, (600, 120)),
-    ('tri', 40, 0.2, (6144, 1024)), ('tri', 40, 0.0, (500, 200)),
+@pytest.mark.parametrize('kind,n,a,caps,block', [
+    ('tet', 12, 0.2, (6144, 1024), 512), ('tet', 12, 0.2, (600, 120), 512),
+    ('tri', 40, 0.2, (6144, 1024), 512), ('tri', 40, 0.0, (500, 200), 512),
+    ('tet', 12, 0.2, (6144, 1024), 1024), ('tri', 40, 0.2, (700, 150), 1024),
 ])
-def test_fixed_point_accumulation(kind, n, a, caps):
+def test_fixed_point_accumulation(kind, n, a, caps, block):
     """Fixed-point LDS accumulation: every form, with and without the load
     vector, persistent workgroups; values within the tolerance of the exact
     path, bitwise repeatable, and independent of what the plan assembled
@@ -171,7 +172,7 @@ def test_fixed_point_accumulation(kind, n, a, caps):
         key = _sha(A.indices.cpu().numpy())
         if key not in plans:
             ip, ix = A.indptr.cpu().numpy(), A.indices.cpu().numpy()
-            plans[key] = (FusedPlan(m, ip, ix, *caps, accum='fixed'),
+            plans[key] = (FusedPlan(m, ip, ix, *caps, accum='fixed', block=block),
                           FusedPlan(m, ip, ix, *caps, accum='rounds'))
             assert plans[key][0].info.nparts > 1
         return plans[key]
