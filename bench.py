@@ -54,6 +54,23 @@ def time_stream(fn, steps, stream):
     return start.elapsed_time(end) / steps
 
 
+def measured_traffic(kernel_name, n):
+    """HBM bytes per launch of the headline kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain
+    copy), or (None, reason) when there is none for this kernel/workload."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                        '%s_tet%d_laplace.md' % (kernel_name.split(' ')[0], n))
+    try:
+        with open(path) as f:
+            m = re.search(r'traffic_bytes_per_launch:\s*(\d+)', f.read())
+    except OSError:
+        return None, 'no counter profile for this kernel/workload'
+    if not m:
+        return None, 'no traffic line in ' + os.path.basename(path)
+    return int(m.group(1)), 'profiles/' + os.path.basename(path)
+
+
 def cpu_baseline(n_sample=81):
     """The oracle (NumPy restatement of the reference algorithm) on a
     bounded sample: Basis + laplace.assemble on the 81^3 jittered mesh."""
@@ -301,6 +318,7 @@ def main():
 
     alg_bytes = 16 * E + 24 * Nv + 8 * nnz
     achieved = alg_bytes / (ms * 1e-3)
+    traffic, traffic_src = measured_traffic(kernel_name, args.n)
     res = {
         "metric": METRIC,
         "value": E / (ms * 1e-3),
@@ -318,7 +336,8 @@ def main():
                    % (E, N, nnz), "path": path, "elements": E, "dofs": N,
                    "nnz": nnz},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "alg_bytes_per_launch": alg_bytes, "kernel": kernel_name},
         "dofs_per_s": N / (ms * 1e-3),
         "first_call_ms": first_call_ms,
