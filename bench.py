@@ -164,7 +164,9 @@ def bench_mesh_batch(k=64):
 
 
 def bench_adjoint(n=65):
-    """kappa-Laplace + load on MeshTet 65^3: forward vs forward + adjoint."""
+    """kappa-Laplace + load on MeshTet 65^3: forward, the adjoint alone
+    (gradients of given upstream dL/dA, dL/db w.r.t. kappa and the node
+    coordinates), and a full forward + loss + backward step."""
     import torch
     import femhip
     from femhip.autograd import kappa_laplace_with_load, quadratic_loss
@@ -176,21 +178,32 @@ def bench_adjoint(n=65):
     f = torch.tensor(np.random.default_rng(2).standard_normal(b.N), device='cuda')
     p = torch.tensor(m.p, device='cuda')
     stream = torch.cuda.current_stream()
+    k = kap.clone().requires_grad_(True)
+    pp = p.clone().requires_grad_(True)
+    data, load, (ip, ix) = kappa_laplace_with_load(b, k, pp)
+    gd = torch.randn_like(data)
+    gl = torch.randn_like(load)
 
     def fwd():
         with torch.no_grad():
             kappa_laplace_with_load(b, kap, p)
 
+    def bwd():
+        torch.autograd.grad((data, load), (k, pp), grad_outputs=(gd, gl), retain_graph=True)
+
     def fwdbwd():
-        k = kap.clone().requires_grad_(True)
-        pp = p.clone().requires_grad_(True)
-        data, load, (ip, ix) = kappa_laplace_with_load(b, k, pp)
-        quadratic_loss(data, ip, ix, u, f).backward()
+        k2 = kap.clone().requires_grad_(True)
+        p2 = p.clone().requires_grad_(True)
+        d2, l2, _ = kappa_laplace_with_load(b, k2, p2)
+        quadratic_loss(d2, ip, ix, u, f).backward()
     fwdbwd()
+    bwd()
     ms_f = time_stream(fwd, 10, stream)
+    ms_b = time_stream(bwd, 10, stream)
     ms_fb = time_stream(fwdbwd, 10, stream)
-    return {"elements": m.nelements, "dofs": b.N, "forward_ms": ms_f,
-            "forward_backward_ms": ms_fb, "backward_over_forward": (ms_fb - ms_f) / ms_f}
+    return {"elements": m.nelements, "dofs": b.N, "nnz": int(data.numel()),
+            "forward_ms": ms_f, "adjoint_ms": ms_b, "adjoint_over_forward": ms_b / ms_f,
+            "forward_loss_backward_ms": ms_fb}
 
 
 def bench_elasticity(n=41):

@@ -309,6 +309,15 @@ int femhip_p1_adjoint(int32_t dim, int64_t nelems, int64_t nnodes, const double*
                       const double* grad_data, const double* grad_load, double* dkappa,
                       double* gX, femhip_stream_t s);
 
+/* Node sums of the adjoint's per-vertex contributions, up to three
+ * components in one pass: out[c*out_stride + s] = sum_{k in [seg[s],
+ * seg[s+1])} contrib[c*in_stride + perm[k]] (perm order, deterministic).
+ * Multi-column counterpart of femhip_segment_sum; no reference counterpart
+ * (it accumulates dL/dx per mesh node). */
+int femhip_segment_sum_multi(const double* contrib, int32_t ncomp, int64_t in_stride,
+                             const int32_t* perm, const int32_t* seg, int64_t nseg,
+                             double* out, int64_t out_stride, femhip_stream_t s);
+
 /* ---------------------------------------------------------------------
  * Vector Q2 hexahedron linear elasticity on FP64 matrix cores.  Numeric
  * counterpart of BilinearForm._assemble for linear_elasticity(lam, mu) on

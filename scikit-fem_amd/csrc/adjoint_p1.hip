@@ -87,24 +87,33 @@ p1_adjoint_kernel(int64_t E, int64_t N, const double* __restrict__ p,
     for (int j = 0; j < D; ++j) H[i][j] = 0.0;
   double dk = 0.0;
   if (gdata != nullptr) {
+    // upstream values of the element's entries (j = b trial, i = a test),
+    // each slot read once; Gs is symmetric, so the pairs (a, b) and (b, a)
+    // are taken together
+    double G[NV][NV];
 #pragma unroll
     for (int a = 0; a < NV; ++a)
 #pragma unroll
       for (int b = 0; b < NV; ++b) {
-        // contribution (j = b trial, i = a test) of element e
-        const int32_t s1 = slot_of[((int64_t)b * NV + a) * E + e];
-        const int32_t s2 = slot_of[((int64_t)a * NV + b) * E + e];
-        const double G1 = s1 >= 0 ? gdata[s1] : 0.0;
-        const double G2 = s2 >= 0 ? gdata[s2] : 0.0;
-        const double Gs = 0.5 * (G1 + G2);
+        const int32_t sl = slot_of[((int64_t)b * NV + a) * E + e];
+        G[a][b] = sl >= 0 ? gdata[sl] : 0.0;
+      }
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+#pragma unroll
+      for (int b = a; b < NV; ++b) {
+        const double w = (a == b) ? G[a][a] : G[a][b] + G[b][a];  // 2 Gs off the diagonal
         double gg = 0.0;
 #pragma unroll
         for (int d = 0; d < D; ++d) gg += g[a][d] * g[b][d];
-        dk += Gs * gg;
+        dk += w * gg;
+        const double h = (a == b) ? w : 0.5 * w;
 #pragma unroll
         for (int i = 0; i < D; ++i)
 #pragma unroll
-          for (int j = 0; j < D; ++j) H[i][j] += Gs * g[a][i] * g[b][j];
+          for (int j = 0; j < D; ++j)
+            H[i][j] += (a == b) ? h * g[a][i] * g[a][j]
+                                : h * (g[a][i] * g[b][j] + g[b][i] * g[a][j]);
       }
   }
   if (dkappa != nullptr) dkappa[e] = V * dk;
@@ -133,6 +142,26 @@ p1_adjoint_kernel(int64_t E, int64_t N, const double* __restrict__ p,
       for (int j = 0; j < D; ++j) s += M[i][j] * g[a][j];
       gX[(int64_t)i * CE + (int64_t)a * E + e] = s;
     }
+}
+
+// out[c * out_stride + s] = sum over k in [seg[s], seg[s+1]) of
+// contrib[c * in_stride + perm[k]], c < ncomp, summed in perm order.
+__global__ void __launch_bounds__(kBlock)
+segment_sum_multi_kernel(const double* __restrict__ contrib, int32_t ncomp, int64_t in_stride,
+                         const int32_t* __restrict__ perm, const int32_t* __restrict__ seg,
+                         int64_t nseg, double* __restrict__ out, int64_t out_stride) {
+  const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (s >= nseg) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int32_t k = seg[s]; k < seg[s + 1]; ++k) {
+    const int64_t c = perm[k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (i < ncomp) acc[i] += contrib[i * in_stride + c];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (i < ncomp) out[i * out_stride + s] = acc[i];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -178,6 +207,21 @@ extern "C" int femhip_p1_adjoint(int32_t dim, int64_t nelems, int64_t nnodes, co
   else
     p1_adjoint_kernel<2><<<grid, kBlock, 0, st>>>(nelems, nnodes, p, t, kappa, slot_of,
                                                   grad_data, grad_load, dkappa, gX);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+extern "C" int femhip_segment_sum_multi(const double* contrib, int32_t ncomp, int64_t in_stride,
+                                        const int32_t* perm, const int32_t* seg, int64_t nseg,
+                                        double* out, int64_t out_stride, femhip_stream_t s) {
+  FEMHIP_REQUIRE(ncomp >= 1 && ncomp <= 3, FEMHIP_EINVAL, "segment_sum_multi: 1 to 3 components");
+  FEMHIP_REQUIRE(nseg >= 0 && in_stride >= 0 && out_stride >= nseg, FEMHIP_EINVAL,
+                 "segment_sum_multi: bad sizes");
+  if (nseg == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(contrib && perm && seg && out, FEMHIP_EINVAL, "segment_sum_multi: NULL argument");
+  segment_sum_multi_kernel<<<(unsigned)ceil_div(nseg, kBlock), kBlock, 0,
+                             reinterpret_cast<hipStream_t>(s)>>>(contrib, ncomp, in_stride, perm,
+                                                                 seg, nseg, out, out_stride);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }

@@ -124,6 +124,7 @@ SIGNATURES = {
                                              _p, _p, _p, _i32, _p]),
     'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
+    'femhip_segment_sum_multi': (ctypes.c_int, [_p, _i32, _i64, _p, _p, _i64, _p, _i64, _p]),
     'femhip_p1_adjoint': (ctypes.c_int, [_i32, _i64, _i64, _p, _p, _p, _p, _p, _p, _p,
                                          _p, _p]),
 }

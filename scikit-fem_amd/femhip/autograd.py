@@ -54,11 +54,9 @@ class _KappaLaplaceLoad(torch.autograd.Function):
         kappa = kappa.detach().contiguous()
         p = p.detach().contiguous()
         out = torch.empty(c.plan.nnz, dtype=torch.float64, device=p.device)
-        c.fp.assemble('kappa_laplace', p, out, kappa)
-        # unit load on the current coordinates (exact path, deterministic)
-        if p.data_ptr() != d.p.data_ptr():
-            d.set_coordinates(p)
-        b = d.assemble_linear('unit_load')
+        b = torch.empty(p.shape[1], dtype=torch.float64, device=p.device)
+        # CSR values and the unit-load vector in one fused pass
+        c.fp.assemble('kappa_laplace', p, out, kappa, load=b)
         ctx.save_for_backward(kappa, p)
         ctx.basis = basis
         return out, b
@@ -80,8 +78,9 @@ class _KappaLaplaceLoad(torch.autograd.Function):
             _lib.ptr(gX), _lib.stream_ptr()))
         perm, seg = c.rows
         dX = torch.empty_like(p)
-        for i in range(dim):
-            d.segment_sum(gX[i * nv * E:(i + 1) * nv * E], perm, seg, p.shape[1], out=dX[i])
+        _lib.check(d.lib.femhip_segment_sum_multi(
+            _lib.ptr(gX), dim, nv * E, _lib.ptr(perm), _lib.ptr(seg), p.shape[1],
+            _lib.ptr(dX), p.shape[1], _lib.stream_ptr()))
         return dk if ctx.needs_input_grad[0] else None, \
             dX if ctx.needs_input_grad[1] else None, None
 
