@@ -206,6 +206,36 @@ def bench_adjoint(n=65):
             "forward_loss_backward_ms": ms_fb}
 
 
+def bench_device_solve(basis, A):
+    """The assembled 101^3 Laplace kept on the device: condense (Dirichlet
+    boundary) + Jacobi-PCG to rtol 1e-8, beside the host hand-off a SciPy
+    solve would start with (D2H copy + csr_matrix)."""
+    import torch
+    from femhip.models import unit_load
+    from femhip.utils import condense_device, solve_device
+    f = unit_load.assemble(basis)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    AII, bI, xx, I = condense_device(A, f, D=basis.mesh.boundary_nodes())
+    torch.cuda.synchronize()
+    t_cond = (time.perf_counter() - t0) * 1e3
+    info = {}
+    t0 = time.perf_counter()
+    u = solve_device(AII, bI, xx, I, rtol=1e-8, info=info)
+    torch.cuda.synchronize()
+    t_solve = (time.perf_counter() - t0) * 1e3
+    from scipy.sparse import csr_matrix
+    t0 = time.perf_counter()
+    _ = csr_matrix((A.data.cpu().numpy(), A.indices.cpu().numpy(), A.indptr.cpu().numpy()),
+                   shape=A.shape)
+    t_host = (time.perf_counter() - t0) * 1e3
+    return {"dofs": int(A.shape[0]), "interior": int(AII.shape[0]), "condense_ms": t_cond,
+            "pcg_ms": t_solve, "pcg_iterations": info["iterations"],
+            "pcg_ms_per_iteration": t_solve / max(1, info["iterations"]),
+            "final_residual": info["residual"], "umax": float(u.max()),
+            "host_handoff_ms": t_host}
+
+
 def bench_elasticity(n=41):
     """linear_elasticity on ElementVector(ElementHex2), 27 points, MeshHex
     (n-1)^3 jittered 0.15h: MFMA local matrices + segmented sums."""
@@ -366,6 +396,12 @@ def main():
     res["roofline"]["frac_of_copy_bw"] = achieved / 1e9 / res["copy_bw_GBps"]
     del src, dst
     if not args.no_extra:
+        log('extra: device_solve')
+        try:
+            res["device_solve_tet101"] = bench_device_solve(basis, A)
+        except Exception as exc:
+            res["device_solve_tet101"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+        torch.cuda.synchronize()
         for name, fn in (("batched_kappa_p2tri257_S256", bench_batched_kappa),
                          ("mesh_batch_64_p1tri", bench_mesh_batch),
                          ("adjoint_kappa_tet65", bench_adjoint),

@@ -319,6 +319,44 @@ int femhip_segment_sum_multi(const double* contrib, int32_t ncomp, int64_t in_st
                              double* out, int64_t out_stride, femhip_stream_t s);
 
 /* ---------------------------------------------------------------------
+ * The assembled system on the device, for the solve.
+ *
+ * Row/column selection (utils.condense, utils.py:540-603: A[I][:, I] and
+ * b[I] - A[I][:, D] @ x[D]): rows[nsel] are the kept rows, colmap[ncols]
+ * the new index of every kept column (increasing along the kept set) or -1.
+ * femhip_csr_select_pattern writes out_indptr[nsel+1] and returns the nnz
+ * (synchronizes); femhip_csr_select_fill writes out_indices/out_data and,
+ * when b is given, b_out[r] = b[rows[r]] - sum_{j dropped} A[rows[r], j] x[j]
+ * (x may be NULL: zero).
+ * --------------------------------------------------------------------- */
+int femhip_csr_select_workspace(int64_t nsel, size_t* bytes);
+int femhip_csr_select_pattern(const int32_t* indptr, const int32_t* indices, const int32_t* rows,
+                              int64_t nsel, const int32_t* colmap, int32_t* out_indptr,
+                              int64_t* nnz_out, void* workspace, size_t ws_bytes,
+                              femhip_stream_t s);
+int femhip_csr_select_fill(const int32_t* indptr, const int32_t* indices, const double* data,
+                           const int32_t* rows, int64_t nsel, const int32_t* colmap,
+                           const int32_t* out_indptr, int32_t* out_indices, double* out_data,
+                           const double* b, const double* x, double* b_out, femhip_stream_t s);
+
+/* y[n] <- A x for a square CSR matrix (16 lanes per row, fixed summation
+ * order); the product scipy's cg applies (utils.py:121-169). */
+int femhip_spmv(const int32_t* indptr, const int32_t* indices, const double* data, int64_t n,
+                const double* x, double* y, femhip_stream_t s);
+
+/* Jacobi-preconditioned conjugate gradients on the device: the iteration of
+ * utils.solver_iter_pcg (scipy.sparse.linalg.cg with M = build_pc_diag(A),
+ * utils.py:56-58,121-169).  x holds the initial guess and receives the
+ * iterate; stops when ||r|| <= max(rtol ||b||, atol) (checked every
+ * check_every iterations, one host synchronization each) or after maxiter
+ * iterations; *iters and *resid report them.  Needs a nonzero diagonal. */
+size_t femhip_pcg_workspace_size(int64_t n);
+int femhip_pcg(const int32_t* indptr, const int32_t* indices, const double* data, int64_t n,
+               const double* b, double* x, double rtol, double atol, int64_t maxiter,
+               int32_t check_every, void* workspace, size_t ws_bytes, int64_t* iters,
+               double* resid, femhip_stream_t s);
+
+/* ---------------------------------------------------------------------
  * Vector Q2 hexahedron linear elasticity on FP64 matrix cores.  Numeric
  * counterpart of BilinearForm._assemble for linear_elasticity(lam, mu) on
  * ElementVector(ElementHex2) with 27 Gauss points

@@ -872,6 +872,26 @@ def enforce(A, b=None, x=None, D=None, diag=1.):
     return Aout, bout
 
 
+def condense(A, b=None, x=None, I=None, D=None):
+    """``utils.condense`` with ``_init_bc`` (``utils.py:295-320,540-603``)
+    for a vector ``b``: ``(A[I][:, I], b[I] - A[I][:, D] @ x[D], x, I)``."""
+    n = A.shape[0]
+    if (I is None) == (D is None):
+        raise Exception("Either I or D must be given!" if I is None else "Give only I or only D!")
+    if I is None:
+        I = np.setdiff1d(np.arange(n, dtype=np.int32), D)
+    else:
+        D = np.setdiff1d(np.arange(n, dtype=np.int32), I)
+    if x is None:
+        x = np.zeros(n, dtype=A.dtype)
+    elif b is None:
+        b = np.zeros_like(x)
+    AII = A[I][:, I]
+    if b is None:
+        return AII, x, I
+    return AII, b[I] - A[I][:, D] @ x[D], x, I
+
+
 # ---------------------------------------------------------------------------
 # convenience: the built-in forms by name
 # ---------------------------------------------------------------------------

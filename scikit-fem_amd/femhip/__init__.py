@@ -17,7 +17,7 @@ from .element import (ElementTriP1, ElementTriP2, ElementTetP1, ElementTetP2,
                       ElementHex1, ElementHex2, ElementVector)
 from .basis import CellBasis, Basis
 from .forms import BilinearForm, LinearForm, asm
-from .utils import enforce
+from .utils import enforce, enforce_device, condense_device, solve_device
 from .device import DeviceCSR
 
 __version__ = "0.1.0"

@@ -125,6 +125,17 @@ SIGNATURES = {
     'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
     'femhip_segment_sum_multi': (ctypes.c_int, [_p, _i32, _i64, _p, _p, _i64, _p, _i64, _p]),
+    'femhip_csr_select_workspace': (ctypes.c_int, [_i64, ctypes.POINTER(ctypes.c_size_t)]),
+    'femhip_csr_select_pattern': (ctypes.c_int, [_p, _p, _p, _i64, _p, _p,
+                                                 ctypes.POINTER(_i64), _p, ctypes.c_size_t,
+                                                 _p]),
+    'femhip_csr_select_fill': (ctypes.c_int, [_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p,
+                                              _p, _p]),
+    'femhip_spmv': (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _p]),
+    'femhip_pcg_workspace_size': (ctypes.c_size_t, [_i64]),
+    'femhip_pcg': (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _f64, _f64, _i64, _i32, _p,
+                                  ctypes.c_size_t, ctypes.POINTER(_i64),
+                                  ctypes.POINTER(_f64), _p]),
     'femhip_p1_adjoint': (ctypes.c_int, [_i32, _i64, _i64, _p, _p, _p, _p, _p, _p, _p,
                                          _p, _p]),
 }

@@ -104,6 +104,41 @@ def test_ex01_max_oracle():
     assert abs(u.max() - 0.073657185490792) < 1e-7
 
 
+def test_condense_oracle():
+    """condense(D=boundary) and enforce give the same solution, also with
+    nonzero boundary values x[D] (the condensed right-hand side carries
+    -A_ID x_D); the ex01 maximum is reproduced through condense."""
+    from scipy.sparse.linalg import spsolve
+    x = np.linspace(0, 1, 33)
+    m = O.MeshTri.init_tensor(x, x)
+    b = O.CellBasis(m, O.ElementTriP1())
+    A = O.bilinear_assemble(O.form_laplace, b)
+    f = O.linear_assemble(O.form_unit_load, b)
+    D = m.boundary_nodes()
+    xb = np.zeros(A.shape[0])
+    xb[D] = np.sin(3 * m.p[0, D]) + m.p[1, D]
+    AII, bI, xx, I = O.condense(A, f, x=xb, D=D)
+    assert AII.shape == (len(I), len(I)) and AII.has_sorted_indices
+    u1 = xx.copy()
+    u1[I] = spsolve(AII.tocsc(), bI)
+    Ae, fe = O.enforce(A, f, x=xb, D=D)
+    u2 = spsolve(Ae.tocsc(), fe)
+    assert np.abs(u1 - u2).max() < 1e-12
+    # I given instead of D: the same system
+    AII2, bI2, _, I2 = O.condense(A, f, x=xb, I=I)
+    assert np.array_equal(I2, I) and (AII2 != AII).nnz == 0 and np.array_equal(bI2, bI)
+    # ex01 through condense
+    x = np.linspace(0, 1, 65)
+    m = O.MeshTri.init_tensor(x, x)
+    b = O.CellBasis(m, O.ElementTriP1())
+    A = O.bilinear_assemble(O.form_laplace, b)
+    f = O.linear_assemble(O.form_unit_load, b)
+    AII, bI, xx, I = O.condense(A, f, I=m.interior_nodes())
+    u = xx.copy()
+    u[I] = spsolve(AII.tocsc(), bI)
+    assert abs(u.max() - 0.073657185490792) < 1e-7
+
+
 # -- quadrature ---------------------------------------------------------------
 @pytest.mark.parametrize('kind,order', [('tri', 2), ('tri', 3), ('tri', 4),
                                         ('tet', 1), ('tet', 2), ('tet', 3),
