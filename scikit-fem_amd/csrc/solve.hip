@@ -21,6 +21,7 @@ namespace {
 constexpr int kBlock = 256;           // threads per block
 constexpr int kRowLanes = 16;         // lanes per CSR row
 constexpr int kRowsPerBlock = kBlock / kRowLanes;
+constexpr int kPcgGrid = 1024;        // blocks of the PCG kernels
 
 // sum over the 16 lanes of a row group, fixed order (xor tree)
 __device__ __forceinline__ double row_sum(double v) {
@@ -114,27 +115,30 @@ pcg_init_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ 
                 double* __restrict__ dinv, double* __restrict__ p, double* __restrict__ part) {
   // r = b - A x, dinv = 1 / diag(A), z = dinv r, p = 0; partials of r.z, r.r, b.b
   __shared__ double red[kBlock / 64];
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x / kRowLanes);
   const int lane = threadIdx.x % kRowLanes;
-  double s = 0.0, dg = 0.0;
-  if (r0 < n)
-    for (int32_t k = indptr[r0] + lane; k < indptr[r0 + 1]; k += kRowLanes) {
-      s += data[k] * x[indices[k]];
-      if (indices[k] == r0) dg = data[k];
-    }
-  s = row_sum(s);
-  dg = row_sum(dg);  // one lane holds the diagonal, the others 0
   double rz = 0.0, rr = 0.0, bb = 0.0;
-  if (r0 < n && lane == 0) {
-    const double ri = b[r0] - s;
-    const double di = 1.0 / dg;
-    r[r0] = ri;
-    dinv[r0] = di;
-    z[r0] = di * ri;
-    p[r0] = 0.0;
-    rz = ri * di * ri;
-    rr = ri * ri;
-    bb = b[r0] * b[r0];
+  for (int64_t base = (int64_t)blockIdx.x * kRowsPerBlock; base < n;
+       base += (int64_t)gridDim.x * kRowsPerBlock) {
+    const int64_t r0 = base + threadIdx.x / kRowLanes;
+    double s = 0.0, dg = 0.0;
+    if (r0 < n)
+      for (int32_t k = indptr[r0] + lane; k < indptr[r0 + 1]; k += kRowLanes) {
+        s += data[k] * x[indices[k]];
+        if (indices[k] == r0) dg = data[k];
+      }
+    s = row_sum(s);
+    dg = row_sum(dg);  // one lane holds the diagonal, the others 0
+    if (r0 < n && lane == 0) {
+      const double ri = b[r0] - s;
+      const double di = 1.0 / dg;
+      r[r0] = ri;
+      dinv[r0] = di;
+      z[r0] = di * ri;
+      p[r0] = 0.0;
+      rz += ri * di * ri;
+      rr += ri * ri;
+      bb += b[r0] * b[r0];
+    }
   }
   rz = block_sum(rz, red);
   rr = block_sum(rr, red);
@@ -187,21 +191,24 @@ pcg_spmv_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ 
                 double* __restrict__ part) {
   __shared__ double red[kBlock / 64];
   const double beta = sc[2];
-  const int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x / kRowLanes);
   const int lane = threadIdx.x % kRowLanes;
-  double s = 0.0;
-  if (r0 < n)
-    for (int32_t k = indptr[r0] + lane; k < indptr[r0 + 1]; k += kRowLanes) {
-      const int32_t j = indices[k];
-      s += data[k] * (z[j] + beta * p_old[j]);
-    }
-  s = row_sum(s);
   double pq = 0.0;
-  if (r0 < n && lane == 0) {
-    const double pi = z[r0] + beta * p_old[r0];
-    p_new[r0] = pi;
-    q[r0] = s;
-    pq = pi * s;
+  for (int64_t base = (int64_t)blockIdx.x * kRowsPerBlock; base < n;
+       base += (int64_t)gridDim.x * kRowsPerBlock) {
+    const int64_t r0 = base + threadIdx.x / kRowLanes;
+    double s = 0.0;
+    if (r0 < n)
+      for (int32_t k = indptr[r0] + lane; k < indptr[r0 + 1]; k += kRowLanes) {
+        const int32_t j = indices[k];
+        s += data[k] * (z[j] + beta * p_old[j]);
+      }
+    s = row_sum(s);
+    if (r0 < n && lane == 0) {
+      const double pi = z[r0] + beta * p_old[r0];
+      p_new[r0] = pi;
+      q[r0] = s;
+      pq += pi * s;
+    }
   }
   pq = block_sum(pq, red);
   if (threadIdx.x == 0) part[blockIdx.x] = pq;
@@ -215,16 +222,16 @@ pcg_update_kernel(int64_t n, const double* __restrict__ p, const double* __restr
                   const double* __restrict__ sc, double* __restrict__ part) {
   __shared__ double red[kBlock / 64];
   const double alpha = sc[1];
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double rz = 0.0, rr = 0.0;
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     const double zi = dinv[i] * ri;
     r[i] = ri;
     z[i] = zi;
-    rz = ri * zi;
-    rr = ri * ri;
+    rz += ri * zi;
+    rr += ri * ri;
   }
   rz = block_sum(rz, red);
   rr = block_sum(rr, red);
@@ -363,8 +370,10 @@ extern "C" int femhip_pcg(const int32_t* indptr, const int32_t* indices, const d
                  "pcg: workspace too small");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
   PcgWork w = carve(workspace, n);
-  const unsigned gr = (unsigned)ceil_div(n, kRowsPerBlock);  // row-group blocks
-  const unsigned gv = (unsigned)ceil_div(n, kBlock);         // vector blocks
+  // fixed grids with grid-stride loops: at most kPcgGrid partial sums per
+  // reduction (the grid depends on n only, so the summation order does too)
+  const unsigned gr = (unsigned)std::min<int64_t>(ceil_div(n, kRowsPerBlock), kPcgGrid);
+  const unsigned gv = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), kPcgGrid);
   pcg_init_kernel<<<gr, kBlock, 0, st>>>(indptr, indices, data, n, b, x, w.r, w.z, w.dinv, w.p0,
                                          w.part);
   FEMHIP_CHECK_LAUNCH();
