@@ -236,6 +236,22 @@ def bench_device_solve(basis, A):
             "host_handoff_ms": t_host}
 
 
+def bench_interpolate(basis):
+    """Basis.interpolate of a DOF vector (values and gradients at the 4
+    quadrature points) on the 101^3 P1-tet basis; bytes = DOF map + w +
+    mapping read, values + gradients written."""
+    import torch
+    d = basis.device()
+    w = torch.randn(basis.N, dtype=torch.float64, device='cuda')
+    d.geometry()
+    basis.interpolate_device(w)
+    ms = time_stream(lambda: basis.interpolate_device(w), 20, torch.cuda.current_stream())
+    E, Q, D = d.E, d.Q, d.dim
+    nbytes = E * (4 * d.nbfun + 8 * (D * D + 1)) + 8 * basis.N + 8 * E * Q * (1 + D)
+    return {"elements": E, "points": Q, "ms": ms, "GBps": nbytes / (ms * 1e-3) / 1e9,
+            "frac_of_8TBps": nbytes / (ms * 1e-3) / HBM_PEAK}
+
+
 def bench_elasticity(n=41):
     """linear_elasticity on ElementVector(ElementHex2), 27 points, MeshHex
     (n-1)^3 jittered 0.15h: MFMA local matrices + segmented sums."""
@@ -396,6 +412,11 @@ def main():
     res["roofline"]["frac_of_copy_bw"] = achieved / 1e9 / res["copy_bw_GBps"]
     del src, dst
     if not args.no_extra:
+        log('extra: interpolate')
+        try:
+            res["interpolate_tet101"] = bench_interpolate(basis)
+        except Exception as exc:
+            res["interpolate_tet101"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
         log('extra: device_solve')
         try:
             res["device_solve_tet101"] = bench_device_solve(basis, A)

@@ -121,6 +121,14 @@ int femhip_local_linear(const femhip_geom_t* g, const femhip_space_t* sp,
                         const femhip_form_t* f, const double* G,
                         double* contrib, femhip_stream_t s);
 
+/* Basis.interpolate (abstract_basis.py:271-322) of a DOF vector w[ndofs]
+ * for a scalar element: value[nelems][nq] and, when grad != NULL,
+ * grad[dim][nelems][nq] (the global gradients of ElementH1.gbasis,
+ * element_h1.py:10-24, G from femhip_geometry), bit-identical to the
+ * reference's values. */
+int femhip_interpolate(const femhip_geom_t* g, const femhip_space_t* sp, const double* G,
+                       const double* w, double* value, double* grad, femhip_stream_t s);
+
 /* ---------------------------------------------------------------------
  * Symbolic phase.  Replaces coo_matrix -> eliminate_zeros -> tocsr
  * (skfem/assembly/form/coo_data.py:28-36) for the pattern, and

@@ -872,6 +872,20 @@ def enforce(A, b=None, x=None, D=None, diag=1.):
     return Aout, bout
 
 
+def interpolate(basis, w):
+    """``AbstractBasis.interpolate`` (``abstract_basis.py:271-322``) for a
+    scalar element: (value (E, Q), grad (dim, E, Q)), one einsum term per
+    local basis function after the ``0. * (...)`` start."""
+    dofs = basis.element_dofs
+    fields = []
+    for n in range(2):
+        out = 0. * np.einsum('...,...j->...j', w[dofs[0]], basis.basis[0][n])
+        for i in range(basis.Nbfun):
+            out += np.einsum('...,...j->...j', w[dofs[i]], basis.basis[i][n])
+        fields.append(out)
+    return tuple(fields)
+
+
 def condense(A, b=None, x=None, I=None, D=None):
     """``utils.condense`` with ``_init_bc`` (``utils.py:295-320,540-603``)
     for a vector ``b``: ``(A[I][:, I], b[I] - A[I][:, D] @ x[D], x, I)``."""

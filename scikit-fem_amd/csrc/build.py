@@ -27,6 +27,7 @@ PER_FILE = {
     'exact_tri.hip': ['-ffp-contract=off'],
     'exact_tet.hip': ['-ffp-contract=off'],
     'exact_hex.hip': ['-ffp-contract=off'],
+    'interpolate.hip': ['-ffp-contract=off'],
 }
 
 

@@ -15,7 +15,7 @@ The arithmetic runs in hand-written HIP kernels for gfx950 (libfemhip.so);
 from .mesh import MeshTri, MeshTet, MeshHex, MeshTri1, MeshTet1, MeshHex1
 from .element import (ElementTriP1, ElementTriP2, ElementTetP1, ElementTetP2,
                       ElementHex1, ElementHex2, ElementVector)
-from .basis import CellBasis, Basis
+from .basis import CellBasis, Basis, DiscreteField
 from .forms import BilinearForm, LinearForm, asm
 from .utils import enforce, enforce_device, condense_device, solve_device
 from .device import DeviceCSR

@@ -124,6 +124,8 @@ SIGNATURES = {
                                              _p, _p, _p, _i32, _p]),
     'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
+    'femhip_interpolate': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space), _p, _p,
+                                          _p, _p, _p]),
     'femhip_segment_sum_multi': (ctypes.c_int, [_p, _i32, _i64, _p, _p, _i64, _p, _i64, _p]),
     'femhip_csr_select_workspace': (ctypes.c_int, [_i64, ctypes.POINTER(ctypes.c_size_t)]),
     'femhip_csr_select_pattern': (ctypes.c_int, [_p, _p, _p, _i64, _p, _p,

@@ -7,11 +7,36 @@ and the reference tables at the quadrature points.  The device copies
 (int32 connectivity, SoA float64 coordinates, tables) are made once per
 basis on first use and reused by every assembly on it.
 """
+import ctypes
+
 import numpy as np
 
 from .dofs import Dofs
 from .quadrature import get_quadrature
 from .element import ElementHex1
+
+
+class DiscreteField:
+    """A field at the quadrature points (``skfem.element.DiscreteField``):
+    ``value`` (E, Q) and ``grad`` (dim, E, Q); NumPy arrays from
+    ``CellBasis.interpolate``, torch tensors on the device from
+    ``CellBasis.interpolate_device``."""
+
+    def __init__(self, value, grad=None):
+        self.value, self.grad = value, grad
+
+    @property
+    def shape(self):
+        return self.value.shape
+
+    def astuple(self):
+        return (self.value, self.grad)
+
+    def __array__(self, dtype=None, copy=None):
+        v = self.value
+        if not isinstance(v, np.ndarray):
+            v = v.cpu().numpy()
+        return np.asarray(v, dtype=dtype)
 
 
 class CellBasis:
@@ -65,6 +90,33 @@ class CellBasis:
 
     def zeros(self):
         return np.zeros(self.N)
+
+    def interpolate_device(self, w, grad=True):
+        """``interpolate`` with the result left on the device: a
+        ``DiscreteField`` of torch tensors (value (E, Q), grad (dim, E, Q))."""
+        import torch
+        from . import _lib
+        if self.elem.vector:
+            raise NotImplementedError("interpolate: scalar elements only")
+        if w.shape[0] != self.N:
+            raise ValueError("Input array has wrong size.")
+        d = self.device()
+        wt = torch.as_tensor(w, dtype=torch.float64, device=d.p.device).contiguous()
+        E, Q, D = d.E, d.Q, d.dim
+        value = torch.empty((E, Q), dtype=torch.float64, device=d.p.device)
+        gr = torch.empty((D, E, Q), dtype=torch.float64, device=d.p.device) if grad else None
+        G = d.geometry() if grad else None
+        _lib.check(d.lib.femhip_interpolate(
+            ctypes.byref(d.geom), ctypes.byref(d.space), _lib.ptr(G), _lib.ptr(wt),
+            _lib.ptr(value), _lib.ptr(gr), _lib.stream_ptr()))
+        return DiscreteField(value, gr)
+
+    def interpolate(self, w):
+        """A DOF vector at the quadrature points (``abstract_basis.py:271-322``):
+        ``DiscreteField`` with ``value`` (E, Q) and ``grad`` (dim, E, Q), equal
+        to the reference's bit for bit (computed on the device)."""
+        f = self.interpolate_device(np.asarray(w, dtype=np.float64))
+        return DiscreteField(f.value.cpu().numpy(), f.grad.cpu().numpy())
 
     def with_element(self, elem):
         return type(self)(self.mesh, elem, quadrature=(self.X, self.W))

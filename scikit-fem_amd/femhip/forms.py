@@ -51,7 +51,7 @@ _KAPPA_CACHE = {}
 
 def _kappa_tensor(w, basis, name='kappa'):
     """Coefficient array of ``w[name]`` as a device tensor: (E,) per element
-    or (E*Q,) per quadrature point (the reference accepts an (E, Q) array or
+    or (E, Q) per quadrature point (the reference accepts an (E, Q) array or
     a DiscreteField, ``form.py:92-121``), and a token identifying its
     contents (content hash for host arrays, storage and version counter for
     device tensors).  An (E, Q) array with identical columns is treated as
@@ -73,10 +73,18 @@ def _kappa_tensor_impl(w, basis, name):
     else:
         k = np.asarray(k, dtype=np.float64)
         if k.ndim == 1:
-            raise NotImplementedError(
-                "1-D coefficient arrays are DOF vectors in the reference "
-                "(Basis.interpolate); pass a per-element (E, 1) or per-point "
-                "(E, Q) array")
+            # a DOF vector: interpolated at the quadrature points
+            # (form.py:106-108)
+            if k.shape[0] != basis.N:
+                raise ValueError("Input array has wrong size.")
+            import hashlib
+            token = ('dofs', k.shape, hashlib.blake2b(k.tobytes(), digest_size=16).hexdigest())
+            kt = _KAPPA_CACHE.get(token)
+            if kt is None:
+                kt = basis.interpolate_device(k, grad=False).value
+                _KAPPA_CACHE.clear()
+                _KAPPA_CACHE[token] = kt
+            return kt.reshape(E, Q).contiguous(), token
         if k.ndim == 2 and k.shape == (E, Q) and Q > 1 and np.all(k == k[:, :1]):
             k = k[:, :1]
         k = np.ascontiguousarray(k)
@@ -90,7 +98,7 @@ def _kappa_tensor_impl(w, basis, name):
     if kt.dim() == 2 and kt.shape == (E, 1):
         return kt.reshape(E).contiguous(), token
     if kt.dim() == 2 and kt.shape == (E, Q):
-        return kt.reshape(E * Q).contiguous(), token
+        return kt.reshape(E, Q).contiguous(), token
     if kt.dim() == 1 and kt.shape[0] == E:
         return kt.contiguous(), token
     raise ValueError("Quadrature mismatch: '%s' should have shape (E, Q) = "

@@ -104,6 +104,74 @@ def test_ex01_max_oracle():
     assert abs(u.max() - 0.073657185490792) < 1e-7
 
 
+_DOFLOCS = {
+    'ElementTriP1': [(0, 0), (1, 0), (0, 1)],
+    'ElementTriP2': [(0, 0), (1, 0), (0, 1), (.5, 0), (.5, .5), (0, .5)],
+    'ElementTetP1': [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)],
+    'ElementTetP2': [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (.5, 0, 0), (.5, .5, 0),
+                     (0, .5, 0), (0, 0, .5), (.5, 0, .5), (0, .5, .5)],
+    'ElementHex1': [tuple(r) for r in O.HEX2_DOFLOCS[:8]],
+    'ElementHex2': [tuple(r) for r in O.HEX2_DOFLOCS],
+}
+
+
+@pytest.mark.parametrize('kind,elem,deg', [
+    ('tri', 'ElementTriP1', 1), ('tri', 'ElementTriP2', 2), ('tet', 'ElementTetP1', 1),
+    ('tet', 'ElementTetP2', 2), ('hex', 'ElementHex1', 1), ('hex', 'ElementHex2', 2),
+])
+def test_interpolate_reproduces_polynomials(kind, elem, deg):
+    """interpolate of the nodal values of a polynomial the element contains
+    gives its values and gradients at the mapped quadrature points (hex: a
+    trilinear / triquadratic field on an undistorted mesh, where the mapping
+    is linear); the DOF values come from the DOF locations."""
+    n = {'tri': 6, 'tet': 4, 'hex': 3}[kind]
+    x = np.linspace(0, 1, n)
+    if kind == 'tri':
+        m = O.MeshTri.init_tensor(x, x)
+    elif kind == 'tet':
+        m = O.MeshTet.init_tensor(x, x, x)
+    else:
+        m = O.MeshHex.init_tensor(x, x, x)
+    if kind != 'hex':
+        m = O.jitter(m, 0.2, 1.0 / (n - 1))
+    b = O.CellBasis(m, getattr(O, elem)())
+    D = m.p.shape[0]
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal(D + 1)
+    C2 = rng.standard_normal((D, D)) if deg == 2 else np.zeros((D, D))
+    if kind == 'hex' and deg == 1:
+        C2 = np.zeros((D, D))
+
+    def f(X):
+        return c[0] + np.einsum('i,i...->...', c[1:], X) + np.einsum('ij,i...,j...->...', C2, X, X)
+
+    def gradf(X):
+        return c[1:].reshape((D,) + (1,) * (X.ndim - 1)) + np.einsum('ij,j...->i...', C2 + C2.T, X)
+
+    # DOF locations on the reference cell, checked to be nodal, mapped to
+    # every element; w holds f at each DOF (consistent across elements)
+    Xloc = np.array(_DOFLOCS[elem], dtype=float).T
+    for i in range(b.Nbfun):
+        phi_i, _ = b.elem.lbasis(Xloc, i)
+        assert np.allclose(phi_i, np.eye(b.Nbfun)[i], atol=1e-14)
+    pts = b.mapping.F(Xloc)
+    w = np.zeros(b.N)
+    for j in range(b.Nbfun):
+        w[b.element_dofs[j]] = f(pts[:, :, j])
+    val, grad = O.interpolate(b, w)
+    xq = b.mapping.F(b.X)
+    assert np.abs(val - f(xq)).max() < 1e-12
+    assert np.abs(grad - gradf(xq)).max() < 1e-10
+
+
+def test_interpolate_partition_of_unity():
+    x = np.linspace(0, 1, 5)
+    m = O.jitter(O.MeshTet.init_tensor(x, x, x), 0.2, 0.25)
+    b = O.CellBasis(m, O.ElementTetP2())
+    val, grad = O.interpolate(b, np.ones(b.N))
+    assert np.abs(val - 1).max() < 1e-14 and np.abs(grad).max() < 1e-12
+
+
 def test_condense_oracle():
     """condense(D=boundary) and enforce give the same solution, also with
     nonzero boundary values x[D] (the condensed right-hand side carries
