@@ -372,8 +372,8 @@ struct RecRing {
 
 // One pass over a partition's element records: the local matrices, the
 // largest |K_aa| and load entry (mk, mv), and -- when SCATTER -- the
-// fixed-point contributions added into the LDS rows (pruned slots go to a
-// per-lane sink).  `ring` holds chunks 0..kPrefetch-1 on entry.
+// fixed-point contributions added into the LDS rows.  `ring` holds chunks
+// 0..kPrefetch-1 on entry.
 template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD, bool SCATTER>
 __device__ __forceinline__ void fx_pass(const FusedArgs& a, int chunk_start, int nchunks,
                                         int R, const double* X, const uint32_t* rinfo,
@@ -381,8 +381,6 @@ __device__ __forceinline__ void fx_pass(const FusedArgs& a, int chunk_start, int
                                         double sck, double scv, RecRing<RW>& ring,
                                         double& mk, double& mv) {
   constexpr int NV = D + 1;
-  constexpr uint32_t SENT = (1u << OFB) - 1u;
-  unsigned long long* const sink = acc + a.max_slots + (threadIdx.x & 63);
   constexpr double LOADC = (D == 3) ? 1.0 / 24.0 : 1.0 / 6.0;  // int phi = |det| LOADC
   const int tid = threadIdx.x;
 
@@ -409,17 +407,17 @@ __device__ __forceinline__ void fx_pass(const FusedArgs& a, int chunk_start, int
 #pragma unroll
       for (int i = 0; i < NV; ++i) {
         if (lv[i] >= R) continue;  // halo vertex: its row belongs elsewhere
+        // pruned entries (a pruned diagonal too) carry the offset of the
+        // row's spare slot
         const uint32_t info = rinfo[lv[i]];
         unsigned long long* row = acc + (info & 0xffffu);
-        const uint32_t dg = (info >> 16) & 0xffu;
-        lds_add(dg != 0xffu ? row + dg : sink, F[i][i]);
+        lds_add(row + ((info >> 16) & 0xffu), F[i][i]);
         int q = 0;
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
           if (j == i) continue;
-          const uint32_t off = offs[i * (NV - 1) + q];
+          lds_add(row + offs[i * (NV - 1) + q], F[i][j]);
           ++q;
-          lds_add(off != SENT ? row + off : sink, F[i][j]);
         }
         if constexpr (LOAD) lds_add(bacc + lv[i], to_fixed(vol * LOADC, scv));
       }
@@ -456,10 +454,10 @@ p1_fused_fx_kernel(FusedArgs a) {
   constexpr int NW = NT / 64;
   constexpr int FSLOT = FORM == FEMHIP_LAPLACE ? 0 : (FORM == FEMHIP_MASS ? 1 : 2);
   extern __shared__ double smem[];
-  // [max_slots accumulators | 64 per-lane sinks | coordinates | load acc |
-  //  2*NW reduction | row info | row starts]
+  // [max_slots accumulators | coordinates | load acc | 2*NW reduction |
+  //  row info | row starts]
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem);
-  double* X = smem + ((a.max_slots + 64 + 1) & ~1);
+  double* X = smem + ((a.max_slots + 1) & ~1);
   unsigned long long* bacc = reinterpret_cast<unsigned long long*>(X + D * a.max_nodes);
   double* red = reinterpret_cast<double*>(bacc + (LOAD ? a.max_rows : 0));
   uint32_t* rinfo = reinterpret_cast<uint32_t*>(red + 2 * NW);
@@ -613,8 +611,8 @@ extern "C" size_t femhip_fused_lds_bytes(const femhip_fused_plan_info_t* I, int3
   const size_t coords = 8 * (size_t)dim * I->max_nodes;
   const size_t load_acc = with_load ? 8 * (size_t)I->max_rows : 0;
   const size_t rinfo = 4 * (size_t)I->max_rows;
-  if (fixed_point)  // accumulators + 64 sinks, coordinates, load, reduction, row info + starts
-    return 8 * (((size_t)I->max_slots + 64 + 1) & ~(size_t)1) + coords + load_acc +
+  if (fixed_point)  // accumulators, coordinates, load, reduction, row info + starts
+    return 8 * (((size_t)I->max_slots + 1) & ~(size_t)1) + coords + load_acc +
            8 * 2 * (I->block / 64) + 2 * rinfo;
   return 8 * (((size_t)I->max_slots + 3) & ~(size_t)1) + coords + load_acc + rinfo;
 }

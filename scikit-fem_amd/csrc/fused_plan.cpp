@@ -18,10 +18,12 @@
 // Per element visit the plan stores one packed record: the local ids of the
 // element's vertices (partition-local numbering, owned nodes first) and, for
 // every owned vertex a and other vertex b, the offset of column b inside
-// row a of the CSR pattern (a sentinel when the slot was pruned by
-// eliminate_zeros), plus the colour relative to the chunk.  Rows are laid
-// out back to back in LDS: with the usual odd row lengths (15 in a
-// tetrahedral interior) consecutive rows start on different LDS banks.
+// row a of the CSR pattern, plus the colour relative to the chunk.  Rows are
+// laid out back to back in LDS: with the usual odd row lengths (15 in a
+// tetrahedral interior) consecutive rows start on different LDS banks.  A
+// row that lost entries to eliminate_zeros has one spare slot after its CSR
+// slots; its pruned entries (a pruned diagonal too) carry the offset
+// len = the spare slot, which is never written out.
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -76,7 +78,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   // record format: local ids and 4- or 5-bit offsets, 4-bit relative colour
   int max_row = 0;
   for (int64_t n = 0; n < Nv; ++n) max_row = std::max(max_row, indptr[n + 1] - indptr[n]);
-  // offsets 0..len-1 and the all-ones sentinel
+  // offsets 0..len (len: the spare slot of a row with pruned entries)
   int offbits = max_row <= 15 ? 4 : (max_row <= 31 ? 5 : 0);
   if (offbits == 0) {
     set_error("fused plan: CSR rows longer than 31 entries");
@@ -87,7 +89,6 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   const int rec_words = (total_bits + 31) / 32;
   const int id_cap = 1 << idbits;
   if (node_cap > id_cap) node_cap = id_cap;
-  const uint32_t sentinel = (1u << offbits) - 1;
 
   // node -> element incidence
   std::vector<int64_t> inc_ptr(Nv + 1, 0);
@@ -271,7 +272,25 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t a, int32_t b) { return color[a] < color[b]; });
-    // rows of the partition
+    // rows of the partition; a row that loses an entry to eliminate_zeros
+    // (a pruned off-diagonal pair of one of its elements, or its diagonal)
+    // gets one spare LDS slot after its CSR slots, which the pruned entries
+    // address (offset = row length), so the kernel adds without a test
+    auto in_row = [&](int32_t g, int32_t col) {
+      const int32_t* s0 = indices + indptr[g];
+      const int32_t* f0 = indices + indptr[g + 1];
+      const int32_t* it = std::lower_bound(s0, f0, col);
+      return it != f0 && *it == col;
+    };
+    std::vector<char> spare(R, 0);
+    for (int32_t e : elems)
+      for (int a = 0; a < NV; ++a) {
+        const int32_t ga = t[(int64_t)a * E + e];
+        const int la = lid[ga];
+        if (la >= R || spare[la]) continue;
+        for (int bb = 0; bb < NV; ++bb)
+          if (!in_row(ga, t[(int64_t)bb * E + e])) spare[la] = 1;
+      }
     const int32_t row_start = (int32_t)(P->rows.size() / 2);
     int lds = 0;
     std::vector<int> row_len(R, 0);
@@ -279,13 +298,13 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
       const int32_t g = owned[r];
       const int b = indptr[g], en = indptr[g + 1];
       row_len[r] = en - b;
-      int diag = 255;  // pruned diagonal
+      int diag = row_len[r];  // pruned diagonal: the spare slot
       for (int k = b; k < en; ++k)
         if (indices[k] == g) diag = k - b;
       P->rows.push_back(b);
       P->rows.push_back((int32_t)((uint32_t)lds | ((uint32_t)diag << 16) |
                                   ((uint32_t)row_len[r] << 24)));
-      lds += row_len[r];
+      lds += row_len[r] + spare[r];
     }
     if (lds > 65535) {
       delete P;
@@ -317,13 +336,14 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
         for (int a = 0; a < NV; ++a)
           for (int b = 0; b < NV; ++b) {
             if (b == a) continue;
-            uint32_t off = sentinel;
+            uint32_t off = 0;  // halo row: never read
             if (lv[a] < R) {
               const int32_t g = gv[a];
               const int32_t* s = indices + indptr[g];
               const int32_t* f = indices + indptr[g + 1];
               const int32_t* it = std::lower_bound(s, f, gv[b]);
-              if (it != f && *it == gv[b]) off = (uint32_t)(it - s);
+              // pruned pair: the row's spare slot (offset = row length)
+              off = (it != f && *it == gv[b]) ? (uint32_t)(it - s) : (uint32_t)(f - s);
             }
             put_bits(w, bitpos, offbits, off);
             bitpos += offbits;

@@ -51,7 +51,6 @@ def replay(mesh, indptr, indices, Kloc, **caps):
     info, parts, nodemap, rows, chunks, recs, rec_elem = _build(mesh, indptr, indices, **caps)
     nv = mesh.t.shape[0]
     idb, ofb = info.idbits, info.offbits
-    sent = (1 << ofb) - 1
     nnz = int(indptr[-1])
     out = np.full(nnz, np.nan)
     written = np.zeros(nnz, int)
@@ -90,17 +89,18 @@ def replay(mesh, indptr, indices, Kloc, **caps):
                         base, dg, ln = info_i & 0xffff, (info_i >> 16) & 0xff, info_i >> 24
                         assert lv[i] not in touched, "two elements of a round share a row"
                         touched.add(lv[i])
-                        if dg != 0xff:
-                            acc[base + dg] += Kloc[e, i, i]
+                        # a pruned entry (diagonal too) addresses the row's
+                        # spare slot base + ln, which is never written out
+                        assert dg <= ln
+                        acc[base + dg] += Kloc[e, i, i]
                         q = 0
                         for j in range(nv):
                             if j == i:
                                 continue
                             off = offs[i * (nv - 1) + q]
                             q += 1
-                            if off != sent:
-                                assert off < ln
-                                acc[base + off] += Kloc[e, i, j]
+                            assert off <= ln
+                            acc[base + off] += Kloc[e, i, j]
         for r in range(R):
             gstart, info_r = rows[rs + r]
             info_r = int(np.uint32(info_r))
