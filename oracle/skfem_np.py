@@ -799,21 +799,37 @@ def form_unit_load(v, w):
 # ---------------------------------------------------------------------------
 # assembly (skfem/assembly/form/*.py)
 # ---------------------------------------------------------------------------
-def bilinear_coo(form, basis, **w):
+def bilinear_coo(form, basis, nthreads=0, **w):
     """``BilinearForm._assemble``, ``bilinear_form.py:58-128``: the data is
     laid out ``(j, i, e)`` C-order, rows from the test (i), cols from the
-    trial (j) basis."""
+    trial (j) basis.  ``nthreads > 0``: the (j, i) entries are split over
+    Python threads as the reference does (``bilinear_form.py:100-119``); the
+    values do not change."""
     nt, nb = basis.nelems, basis.Nbfun
     data = np.zeros((nb, nb, nt))
     rows = np.zeros(nb * nb * nt, dtype=np.int32)
     cols = np.zeros(nb * nb * nt, dtype=np.int32)
+
+    def entry(j, i):
+        data[j, i, :] = np.sum(form(basis.basis[j], basis.basis[i], w)
+                               * basis.dx, axis=1)
+
     for j in range(nb):
         for i in range(nb):
             ixs = slice(nt * (nb * j + i), nt * (nb * j + i + 1))
             rows[ixs] = basis.element_dofs[i]
             cols[ixs] = basis.element_dofs[j]
-            data[j, i, :] = np.sum(form(basis.basis[j], basis.basis[i], w)
-                                   * basis.dx, axis=1)
+            if nthreads <= 0:
+                entry(j, i)
+    if nthreads > 0:
+        import threading
+        pairs = np.array([[j, i] for j in range(nb) for i in range(nb)])
+        threads = [threading.Thread(target=lambda ix=ix: [entry(j, i) for j, i in ix])
+                   for ix in np.array_split(pairs, nthreads, axis=0)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
     return np.array([rows, cols]), data.flatten('C'), (basis.N, basis.N)
 
 
@@ -824,9 +840,9 @@ def coo_to_csr(indices, data, shape):
     return K.tocsr()
 
 
-def bilinear_assemble(form, basis, **w):
+def bilinear_assemble(form, basis, nthreads=0, **w):
     """``BilinearForm.assemble``, ``bilinear_form.py:130-148``."""
-    return coo_to_csr(*bilinear_coo(form, basis, **w))
+    return coo_to_csr(*bilinear_coo(form, basis, nthreads=nthreads, **w))
 
 
 def linear_assemble(form, basis, **w):

@@ -411,3 +411,15 @@ def test_elasticity_s_matrix_identity():
                 v = v + mu * tr[a, b]
             Kf[j, i] = v
     assert np.abs(Kf - K).max() <= 1e-13 * np.abs(K).max()
+
+
+def test_threaded_assembly_identical():
+    """nthreads splits the (j, i) entries over threads (the reference's
+    knob); the matrix is the same bit for bit."""
+    x = np.linspace(0, 1, 9)
+    m = O.jitter(O.MeshTet.init_tensor(x, x, x), 0.2, 1 / 8)
+    b = O.CellBasis(m, O.ElementTetP2())
+    A1 = O.bilinear_assemble(O.form_laplace, b)
+    A2 = O.bilinear_assemble(O.form_laplace, b, nthreads=3)
+    assert np.array_equal(A1.indptr, A2.indptr) and np.array_equal(A1.indices, A2.indices)
+    assert np.array_equal(A1.data, A2.data)
