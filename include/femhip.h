@@ -121,6 +121,41 @@ int femhip_local_linear(const femhip_geom_t* g, const femhip_space_t* sp,
                         const femhip_form_t* f, const double* G,
                         double* contrib, femhip_stream_t s);
 
+/* Boundary facets of a P1/P2 simplex mesh (FacetBasis, facet_basis.py:23-110):
+ * facets[dim][nfacets] their vertices (Mesh.facets order), tind[nfacets]
+ * the owning element, X[dim-1][nq] / W[nq] the facet quadrature rule. */
+typedef struct femhip_facet_geom {
+  int32_t kind;         /* FEMHIP_TRI or FEMHIP_TET                       */
+  int32_t dim;          /* 2 or 3                                         */
+  int32_t degree;       /* 1 (P1) or 2 (P2)                               */
+  int32_t nq;
+  int64_t nfacets;
+  int64_t nnodes;
+  int64_t nelems;
+  const double* p;      /* [dim][nnodes]                                  */
+  const int32_t* facets;
+  const int32_t* t;     /* [dim+1][nelems]                                */
+  const int32_t* tind;
+  const double* X;
+  const double* W;
+} femhip_facet_geom_t;
+
+/* Local entries of a boundary form on the facets, in the reference's
+ * arithmetic (mapping_affine.py:195-203,234-281 for G / invF / detDG):
+ * form FEMHIP_MASS -> out[(j*nb + i)*nfacets + f] (BilinearForm._assemble
+ * layout, bilinear_form.py:58-128), FEMHIP_UNIT_LOAD -> out[i*nfacets + f]
+ * (linear_form.py:18-49).  G is the cell mapping of femhip_geometry. */
+int femhip_facet_local(const femhip_facet_geom_t* g, int32_t form, const double* G,
+                       double* out, femhip_stream_t s);
+
+/* Outward unit normals of the facets seen from tind_normals[f] (= f2t[0]),
+ * whose local facet (RefTri/RefTet order) is floc[f]
+ * (MappingAffine.normals, mapping_affine.py:255-281; FacetBasis.normals,
+ * facet_basis.py:96-104): out[(j*nfacets + f)*nq + q]. */
+int femhip_facet_normals(const femhip_facet_geom_t* g, const int32_t* tind_normals,
+                         const int32_t* floc, const double* G, double* out,
+                         femhip_stream_t s);
+
 /* Basis.interpolate (abstract_basis.py:271-322) of a DOF vector w[ndofs]
  * for a scalar element: value[nelems][nq] and, when grad != NULL,
  * grad[dim][nelems][nq] (the global gradients of ElementH1.gbasis,

@@ -737,6 +737,85 @@ class CellBasis:
                    * np.broadcast_to(self.W, (self.nelems, self.W.shape[-1])))
 
 
+def facet_mapping(mesh):
+    """``MappingAffine._init_boundary_mapping`` (``mapping_affine.py:155-181``):
+    B (dim, dim-1, nf), c (dim, nf), detB (nf,) for every facet."""
+    dim = mesh.p.shape[0]
+    f, p = mesh.facets, mesh.p
+    nf = f.shape[1]
+    B = np.empty((dim, dim - 1, nf))
+    c = np.empty((dim, nf))
+    for i in range(dim):
+        c[i] = p[i, f[0]]
+        for j in range(dim - 1):
+            B[i, j] = p[i, f[j + 1]] - p[i, f[0]]
+    if dim == 2:
+        detB = np.sqrt(B[0, 0] ** 2 + B[1, 0] ** 2)
+    else:
+        detB = np.sqrt((B[1, 0] * B[2, 1] - B[2, 0] * B[1, 1]) ** 2
+                       + (-B[0, 0] * B[2, 1] + B[2, 0] * B[0, 1]) ** 2
+                       + (B[0, 0] * B[1, 1] - B[1, 0] * B[0, 1]) ** 2)
+    return B, c, detB
+
+
+# outward unit normals of the reference simplices' facets in RefTri/RefTet
+# facet order (mapping_affine.py:248-262)
+_NREF = {2: np.array([[0., -1.], [1., 1.], [-1., 0.]]),
+         3: np.array([[0., 0., -1.], [0., -1., 0.], [-1., 0., 0.], [1., 1., 1.]])}
+
+
+class FacetBasis:
+    """``FacetBasis.__init__`` (``facet_basis.py:23-110``) on an affine
+    simplex mesh: quadrature on the facet reference domain (line / triangle),
+    facet points mapped by ``G`` (``mapping_affine.py:234-245``) and pulled
+    back into the owning element by ``invF`` (``:195-203``), values and
+    gradients by ``ElementH1.gbasis``'s 3-D branch (``element_h1.py:10-24``),
+    ``dx = |detB| W`` (``:247-253``), normals by ``normals`` (``:255-281``),
+    and ``element_dofs[:, tind]`` (``abstract_basis.py:110-117``)."""
+
+    def __init__(self, mesh, elem, intorder=None, facets=None, side=0):
+        if not mesh.affine:
+            raise NotImplementedError("affine (simplex) meshes only")
+        self.mesh, self.elem = mesh, elem
+        mp = MappingAffine(mesh)
+        ed, self.N = element_dofs(mesh, elem)
+        self.find = (mesh.boundary_facets() if facets is None
+                     else np.asarray(facets, dtype=np.int32))
+        self.tind = mesh.f2t[side, self.find]
+        tn = mesh.f2t[0, self.find]
+        self.element_dofs = ed[:, self.tind]
+        self.Nbfun = ed.shape[0]
+        self.nelems = len(self.find)
+        order = intorder if intorder is not None else 2 * elem.maxdeg
+        self.X, self.W = {'tri': quadrature_line,
+                          'tet': quadrature_tri}[mesh.kind](order)
+        B, c, detB = facet_mapping(mesh)
+        B, c = B[:, :, self.find], c[:, self.find]
+        x = (np.einsum('ijk,jl', B, self.X).T + c.T).T
+        invA, b = mp.invA[:, :, self.tind], mp.b[:, self.tind]
+        Y = np.einsum('ijk,jkl->ikl', invA, (x.T - b.T).T)
+        invDF = np.einsum('ijk,l->ijkl', invA, np.ones(self.X.shape[-1]))
+        self.basis = []
+        for j in range(self.Nbfun):
+            phi, dphi = elem.lbasis(Y, j)
+            self.basis.append((np.broadcast_to(phi, (invDF.shape[2], invDF.shape[3])),
+                               np.einsum('ijkl,ikl->jkl', invDF, dphi)))
+        self.dx = (np.abs(np.tile(detB[self.find], (self.X.shape[-1], 1)).T)
+                   * np.broadcast_to(self.W, (self.nelems, self.W.shape[-1])))
+        # normals, seen from f2t[0] (tind_normals)
+        dim = mesh.p.shape[0]
+        Nr = _NREF[dim]
+        invDFn = np.einsum('ijk,l->ijkl', mp.invA[:, :, tn], np.ones(self.X.shape[-1]))
+        N = np.empty((dim, self.nelems))
+        for itr in range(Nr.shape[0]):
+            ix = np.nonzero(mesh.t2f[itr, tn] == self.find)[0].astype(np.int32)
+            for jtr in range(Nr.shape[1]):
+                N[jtr, ix] = Nr[itr, jtr]
+        n = np.einsum('ijkl,ik->jkl', invDFn, N)
+        nlength = np.sqrt(np.sum(n ** 2, axis=0))
+        self.normals = np.einsum('ijk,jk->ijk', n, 1. / nlength)
+
+
 # ---------------------------------------------------------------------------
 # helpers and models (skfem/helpers.py, skfem/models/*.py)
 # ---------------------------------------------------------------------------

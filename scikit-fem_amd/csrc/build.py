@@ -28,6 +28,7 @@ PER_FILE = {
     'exact_tet.hip': ['-ffp-contract=off'],
     'exact_hex.hip': ['-ffp-contract=off'],
     'interpolate.hip': ['-ffp-contract=off'],
+    'facet.hip': ['-ffp-contract=off'],
 }
 
 

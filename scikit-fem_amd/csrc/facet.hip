@@ -1,0 +1,295 @@
+// Local entries of boundary forms on a FacetBasis (facet_basis.py:23-110)
+// for P1/P2 triangles and tetrahedra: facet mass u*v and facet load 1*v
+// (models/poisson.py), in the reference's arithmetic so that the exact
+// zeros -- the basis functions of the vertex opposite a facet, evaluated
+// at the facet's quadrature points -- come out as the reference's do.
+//
+// Per boundary facet f (vertices facets[:, f], owning element e = tind[f])
+// and facet quadrature point X_q:
+//   x   = B X_q + c              MappingAffine.G, mapping_affine.py:234-245
+//         (B[:, j] = p[f_{j+1}] - p[f_0], c = p[f_0]; einsum in index order)
+//   Y   = invA_e (x - b_e)       MappingAffine.invF, mapping_affine.py:195-203
+//         (invA_e from the element mapping, femhip_geometry's affine rows)
+//   dx  = |detB| W_q             detB: mapping_affine.py:158-181
+//   phi_i(Y) by the element's polynomial in the reference's operation order
+//         (element_tri_p1.py, element_tri_p2.py, element_tet_p1.py,
+//         element_tet_p2.py)
+// and, as BilinearForm._kernel / LinearForm._kernel do, the entries are
+// np.sum((phi_j phi_i) dx, axis=1) and np.sum((1.0 phi_i) dx, axis=1).
+// Compiled with -ffp-contract=off.
+#include "common.h"
+
+namespace femhip {
+namespace {
+
+constexpr int kBlock = 256;
+
+// P1 / P2 simplex basis values at Y (reference expressions, left to right)
+template <int D, int DEG>
+__device__ __forceinline__ double lbasis(int i, const double (&Y)[D]) {
+#pragma clang fp contract(off)
+  if constexpr (D == 2) {
+    const double x = Y[0], y = Y[1];
+    if constexpr (DEG == 1) {
+      if (i == 0) return 1. - x - y;
+      return i == 1 ? x : y;
+    } else {
+      switch (i) {
+        case 0: return 1. - 3. * x - 3. * y + 2. * (x * x) + 4. * x * y + 2. * (y * y);
+        case 1: return 2. * (x * x) - x;
+        case 2: return 2. * (y * y) - y;
+        case 3: return 4. * x - 4. * (x * x) - 4. * x * y;
+        case 4: return 4. * x * y;
+        default: return 4. * y - 4. * x * y - 4. * (y * y);
+      }
+    }
+  } else {
+    const double x = Y[0], y = Y[1], z = Y[2];
+    if constexpr (DEG == 1) {
+      switch (i) {
+        case 0: return 1. - x - y - z;
+        case 1: return x;
+        case 2: return y;
+        default: return z;
+      }
+    } else {
+      switch (i) {
+        case 0:
+          return 1. - 3. * x + 2. * (x * x) - 3. * y + 4. * x * y + 2. * (y * y) - 3. * z +
+                 4. * x * z + 4. * y * z + 2. * (z * z);
+        case 1: return -1. * x + 2. * (x * x);
+        case 2: return -1. * y + 2. * (y * y);
+        case 3: return -1. * z + 2. * (z * z);
+        case 4: return 4. * x - 4. * (x * x) - 4. * x * y - 4. * x * z;
+        case 5: return 4. * x * y;
+        case 6: return 0. + 4. * y - 4. * x * y - 4. * (y * y) - 4. * y * z;
+        case 7: return 0. + 4. * z - 4. * x * z - 4. * y * z - 4. * (z * z);
+        case 8: return 0. + 4. * x * z;
+        default: return 0. + 4. * y * z;
+      }
+    }
+  }
+}
+
+struct FacetArgs {
+  const double* p;        // [D][nnodes]
+  const int32_t* facets;  // [D][nf] vertices of the boundary facets
+  const int32_t* t;       // [D+1][E] element vertices
+  const int32_t* tind;    // [nf] owning element
+  const double* G;        // element mapping: [D*D+1][E] (invA rows, |det|)
+  const double* X;        // [D-1][Q] facet quadrature points
+  const double* W;        // [Q]
+  double* out;            // mass [(j nb + i) nf + f], load [i nf + f]
+  int64_t nnodes, E, nf;
+};
+
+// the facet's geometry: Y (element reference coordinates of the facet
+// quadrature points) and dx
+template <int D, int Q>
+struct FacetPoints {
+  double Y[Q][D];
+  double dx[Q];
+  __device__ __forceinline__ void init(const FacetArgs& a, int64_t f) {
+#pragma clang fp contract(off)
+    const int64_t N = a.nnodes;
+    int32_t fv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) fv[k] = a.facets[(int64_t)k * a.nf + f];
+    double B[D][D - 1], c[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      c[i] = a.p[i * N + fv[0]];
+#pragma unroll
+      for (int j = 0; j < D - 1; ++j) B[i][j] = a.p[i * N + fv[j + 1]] - a.p[i * N + fv[0]];
+    }
+    double detB;
+    if constexpr (D == 2) {
+      detB = sqrt(B[0][0] * B[0][0] + B[1][0] * B[1][0]);
+    } else {
+      const double t0 = B[1][0] * B[2][1] - B[2][0] * B[1][1];
+      const double t1 = -B[0][0] * B[2][1] + B[2][0] * B[0][1];
+      const double t2 = B[0][0] * B[1][1] - B[1][0] * B[0][1];
+      detB = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    }
+    const int64_t e = a.tind[f];
+    double inv[D][D], b[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      b[i] = a.p[i * N + a.t[e]];
+#pragma unroll
+      for (int j = 0; j < D; ++j) inv[i][j] = a.G[(int64_t)(i * D + j) * a.E + e];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      double y[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < D - 1; ++j) s = s + B[i][j] * a.X[j * Q + q];
+        y[i] = (s + c[i]) - b[i];
+      }
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s = s + inv[i][j] * y[j];
+        Y[q][i] = s;
+      }
+      dx[q] = fabs(detB) * a.W[q];
+    }
+  }
+};
+
+template <int D, int DEG, int Q>
+__global__ void __launch_bounds__(kBlock) facet_mass_kernel(FacetArgs a) {
+#pragma clang fp contract(off)
+  constexpr int NB = D == 2 ? (DEG == 1 ? 3 : 6) : (DEG == 1 ? 4 : 10);
+  const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (f >= a.nf) return;
+  const int pair = blockIdx.y;  // j * NB + i (j trial, i test)
+  const int j = pair / NB, i = pair % NB;
+  FacetPoints<D, Q> fp;
+  fp.init(a, f);
+  auto term = [&](int q) -> double {
+#pragma clang fp contract(off)
+    return (lbasis<D, DEG>(j, fp.Y[q]) * lbasis<D, DEG>(i, fp.Y[q])) * fp.dx[q];
+  };
+  a.out[(int64_t)pair * a.nf + f] = 0.0 + np_pairwise<0, Q>(term);
+}
+
+template <int D, int DEG, int Q>
+__global__ void __launch_bounds__(kBlock) facet_load_kernel(FacetArgs a) {
+#pragma clang fp contract(off)
+  const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (f >= a.nf) return;
+  const int i = blockIdx.y;
+  FacetPoints<D, Q> fp;
+  fp.init(a, f);
+  auto term = [&](int q) -> double {
+#pragma clang fp contract(off)
+    return (1.0 * lbasis<D, DEG>(i, fp.Y[q])) * fp.dx[q];
+  };
+  a.out[(int64_t)i * a.nf + f] = 0.0 + np_pairwise<0, Q>(term);
+}
+
+// outward unit normals seen from element tn[f] through its local facet
+// floc[f] (MappingAffine.normals, mapping_affine.py:255-281): the reference
+// cell's facet normal N (RefTri/RefTet facet order), n_j = sum_i invA[i][j] N_i
+// (einsum 'ijkl,ik->jkl', i in order), |n|^2 summed over j in order, and
+// n_j * (1 / |n|); the same value at every quadrature point.
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+facet_normals_kernel(const int32_t* __restrict__ tn, const int32_t* __restrict__ floc,
+                     const double* __restrict__ G, int64_t E, int64_t nf, int Q,
+                     double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (f >= nf) return;
+  const int64_t e = tn[f];
+  const int k = floc[f];
+  double N[D];
+  if constexpr (D == 2) {
+    // facets (0,1), (1,2), (0,2)
+    N[0] = k == 0 ? 0. : (k == 1 ? 1. : -1.);
+    N[1] = k == 0 ? -1. : (k == 1 ? 1. : 0.);
+  } else {
+    // facets (0,1,2), (0,1,3), (0,2,3), (1,2,3)
+    N[0] = k == 2 ? -1. : (k == 3 ? 1. : 0.);
+    N[1] = k == 1 ? -1. : (k == 3 ? 1. : 0.);
+    N[2] = k == 0 ? -1. : (k == 3 ? 1. : 0.);
+  }
+  double n[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) acc = acc + G[(int64_t)(i * D + j) * E + e] * N[i];
+    n[j] = acc;
+  }
+  double l2 = n[0] * n[0];
+#pragma unroll
+  for (int j = 1; j < D; ++j) l2 = l2 + n[j] * n[j];
+  const double r = 1. / sqrt(l2);
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const double v = n[j] * r;
+    for (int q = 0; q < Q; ++q) out[((int64_t)j * nf + f) * Q + q] = v;
+  }
+}
+
+template <int D, int DEG, int Q>
+int launch(const FacetArgs& a, int form, hipStream_t s) {
+  constexpr int NB = D == 2 ? (DEG == 1 ? 3 : 6) : (DEG == 1 ? 4 : 10);
+  const unsigned gx = (unsigned)ceil_div(a.nf, kBlock);
+  if (form == FEMHIP_MASS)
+    facet_mass_kernel<D, DEG, Q><<<dim3(gx, NB * NB), kBlock, 0, s>>>(a);
+  else
+    facet_load_kernel<D, DEG, Q><<<dim3(gx, NB), kBlock, 0, s>>>(a);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+template <int D, int DEG>
+int dispatch_q(const FacetArgs& a, int form, int Q, hipStream_t s) {
+  switch (Q) {
+    case 1: return launch<D, DEG, 1>(a, form, s);
+    case 2: return launch<D, DEG, 2>(a, form, s);
+    case 3: return launch<D, DEG, 3>(a, form, s);
+    case 4: return launch<D, DEG, 4>(a, form, s);
+    case 5: return launch<D, DEG, 5>(a, form, s);
+    case 6: return launch<D, DEG, 6>(a, form, s);
+    case 7: return launch<D, DEG, 7>(a, form, s);
+    default:
+      set_error("facet forms: unsupported number of facet quadrature points");
+      return FEMHIP_EUNSUPPORTED;
+  }
+}
+
+}  // namespace
+}  // namespace femhip
+
+using namespace femhip;
+
+extern "C" int femhip_facet_local(const femhip_facet_geom_t* g, int32_t form, const double* G,
+                                  double* out, femhip_stream_t s) {
+  FEMHIP_REQUIRE(g != nullptr && G != nullptr && out != nullptr, FEMHIP_EINVAL,
+                 "facet forms: NULL argument");
+  FEMHIP_REQUIRE(form == FEMHIP_MASS || form == FEMHIP_UNIT_LOAD, FEMHIP_EUNSUPPORTED,
+                 "facet forms: mass and unit_load only");
+  FEMHIP_REQUIRE((g->kind == FEMHIP_TRI && g->dim == 2) || (g->kind == FEMHIP_TET && g->dim == 3),
+                 FEMHIP_EUNSUPPORTED, "facet forms: triangles or tetrahedra only");
+  FEMHIP_REQUIRE(g->degree == 1 || g->degree == 2, FEMHIP_EUNSUPPORTED,
+                 "facet forms: P1 or P2 only");
+  FEMHIP_REQUIRE(g->p && g->facets && g->t && g->tind && g->X && g->W, FEMHIP_EINVAL,
+                 "facet forms: NULL array");
+  if (g->nfacets == 0) return FEMHIP_OK;
+  FacetArgs a{g->p, g->facets, g->t, g->tind, G, g->X, g->W, out, g->nnodes, g->nelems,
+              g->nfacets};
+  hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  if (g->dim == 2)
+    return g->degree == 1 ? dispatch_q<2, 1>(a, form, g->nq, st) : dispatch_q<2, 2>(a, form, g->nq, st);
+  return g->degree == 1 ? dispatch_q<3, 1>(a, form, g->nq, st) : dispatch_q<3, 2>(a, form, g->nq, st);
+}
+
+extern "C" int femhip_facet_normals(const femhip_facet_geom_t* g, const int32_t* tind_normals,
+                                    const int32_t* floc, const double* G, double* out,
+                                    femhip_stream_t s) {
+  FEMHIP_REQUIRE(g != nullptr && tind_normals != nullptr && floc != nullptr && G != nullptr &&
+                     out != nullptr,
+                 FEMHIP_EINVAL, "facet normals: NULL argument");
+  FEMHIP_REQUIRE((g->kind == FEMHIP_TRI && g->dim == 2) || (g->kind == FEMHIP_TET && g->dim == 3),
+                 FEMHIP_EUNSUPPORTED, "facet normals: triangles or tetrahedra only");
+  FEMHIP_REQUIRE(g->nq >= 1, FEMHIP_EINVAL, "facet normals: no quadrature points");
+  if (g->nfacets == 0) return FEMHIP_OK;
+  hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  const unsigned grid = (unsigned)ceil_div(g->nfacets, kBlock);
+  if (g->dim == 2)
+    facet_normals_kernel<2><<<grid, kBlock, 0, st>>>(tind_normals, floc, G, g->nelems, g->nfacets,
+                                                     g->nq, out);
+  else
+    facet_normals_kernel<3><<<grid, kBlock, 0, st>>>(tind_normals, floc, G, g->nelems, g->nfacets,
+                                                     g->nq, out);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}

@@ -15,7 +15,7 @@ The arithmetic runs in hand-written HIP kernels for gfx950 (libfemhip.so);
 from .mesh import MeshTri, MeshTet, MeshHex, MeshTri1, MeshTet1, MeshHex1
 from .element import (ElementTriP1, ElementTriP2, ElementTetP1, ElementTetP2,
                       ElementHex1, ElementHex2, ElementVector)
-from .basis import CellBasis, Basis, DiscreteField
+from .basis import CellBasis, Basis, FacetBasis, DiscreteField
 from .forms import BilinearForm, LinearForm, asm
 from .utils import enforce, enforce_device, condense_device, solve_device
 from .device import DeviceCSR
@@ -25,6 +25,6 @@ __version__ = "0.1.0"
 __all__ = [
     'MeshTri', 'MeshTet', 'MeshHex', 'MeshTri1', 'MeshTet1', 'MeshHex1',
     'ElementTriP1', 'ElementTriP2', 'ElementTetP1', 'ElementTetP2',
-    'ElementHex1', 'ElementHex2', 'ElementVector', 'CellBasis', 'Basis',
+    'ElementHex1', 'ElementHex2', 'ElementVector', 'CellBasis', 'Basis', 'FacetBasis',
     'BilinearForm', 'LinearForm', 'asm', 'enforce', 'DeviceCSR',
 ]

@@ -48,6 +48,12 @@ class Form(ctypes.Structure):
                 ('lam', _f64), ('two_mu', _f64)]
 
 
+class FacetGeom(ctypes.Structure):
+    _fields_ = [('kind', _i32), ('dim', _i32), ('degree', _i32), ('nq', _i32),
+                ('nfacets', _i64), ('nnodes', _i64), ('nelems', _i64), ('p', _p),
+                ('facets', _p), ('t', _p), ('tind', _p), ('X', _p), ('W', _p)]
+
+
 class FusedInfo(ctypes.Structure):
     _fields_ = [('nparts', _i32), ('nchunks', _i32), ('nrows', _i32), ('max_slots', _i32),
                 ('max_nodes', _i32), ('max_rows', _i32), ('rec_words', _i32),
@@ -124,6 +130,8 @@ SIGNATURES = {
                                              _p, _p, _p, _i32, _p]),
     'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
+    'femhip_facet_local': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _i32, _p, _p, _p]),
+    'femhip_facet_normals': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p, _p]),
     'femhip_interpolate': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space), _p, _p,
                                           _p, _p, _p]),
     'femhip_segment_sum_multi': (ctypes.c_int, [_p, _i32, _i64, _p, _p, _i64, _p, _i64, _p]),

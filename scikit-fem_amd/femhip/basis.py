@@ -134,3 +134,96 @@ class CellBasis:
 
 
 Basis = CellBasis
+
+
+class FacetBasis:
+    """Basis on a set of mesh facets, by default the boundary
+    (``facet_basis.py:23-110``), for boundary forms on triangles and
+    tetrahedra with P1/P2 elements.
+
+    As ``CellBasis`` it keeps the index data only: ``find`` (the facets),
+    ``tind`` (the element on the chosen ``side``, ``f2t[side, find]``),
+    ``element_dofs`` (the owning elements' DOFs, ``abstract_basis.py:110-117``)
+    and the facet quadrature on ``mesh.brefdom``.  The facet points, their
+    pull-back into the owning elements and the local entries are computed on
+    the device (``femhip_facet_local``).  A facet without an element on
+    ``side`` (side 1 of a boundary facet) raises ``ValueError``.
+    """
+
+    def __init__(self, mesh, elem, mapping=None, intorder=None, quadrature=None,
+                 facets=None, dofs=None, side=0, disable_doflocs=False):
+        if mesh.kind != elem.kind:
+            raise ValueError("Incompatible Mesh and Element.")
+        if mapping is not None:
+            raise NotImplementedError("custom mappings are not accelerated")
+        if mesh.kind not in ('tri', 'tet') or elem.vector or elem.maxdeg not in (1, 2):
+            raise NotImplementedError("facet bases: P1/P2 triangles and tetrahedra")
+        if side not in (0, 1):
+            raise ValueError("side is 0 or 1")
+        self.mesh = mesh
+        self.elem = elem
+        self.dofs = Dofs(mesh, elem) if dofs is None else dofs
+        if quadrature is not None:
+            self.X, self.W = (np.asarray(quadrature[0], dtype=np.float64),
+                              np.asarray(quadrature[1], dtype=np.float64))
+        else:
+            order = intorder if intorder is not None else 2 * elem.maxdeg
+            self.X, self.W = get_quadrature(mesh.brefdom, order)
+        find = mesh.normalize_facets(facets)
+        if find.dtype == bool:
+            find = np.nonzero(find)[0]
+        self.find = np.ascontiguousarray(find, dtype=np.int32)
+        self.side = side
+        f2t = mesh.f2t
+        self.tind = f2t[side, self.find]
+        self.tind_normals = f2t[0, self.find]
+        if np.any(self.tind < 0):
+            raise ValueError("facets without an element on side %d" % side)
+        self.nelems = len(self.find)
+        self._element_dofs = None
+        self._device = None
+
+    @property
+    def N(self):
+        return self.dofs.N
+
+    @property
+    def Nbfun(self):
+        return self.dofs.element_dofs.shape[0]
+
+    @property
+    def element_dofs(self):
+        if self._element_dofs is None:
+            self._element_dofs = self.dofs.element_dofs[:, self.tind]
+        return self._element_dofs
+
+    @property
+    def nodal_dofs(self):
+        return self.dofs.nodal_dofs
+
+    def facet_local_index(self):
+        """Local index (``RefTri``/``RefTet`` facet order) of every facet in
+        its ``tind_normals`` element."""
+        t2f = self.mesh.t2f[:, self.tind_normals]
+        return np.argmax(t2f == self.find[None, :], axis=0).astype(np.int32)
+
+    @property
+    def normals(self):
+        """Outward unit normals seen from ``f2t[0]``
+        (``mapping_affine.py:255-281``), a ``DiscreteField`` with ``value``
+        (dim, nfacets, Q), computed on the device."""
+        n = self.device().normals()
+        return DiscreteField(n.cpu().numpy())
+
+    def zeros(self):
+        return np.zeros(self.N)
+
+    def device(self):
+        if self._device is None:
+            from .device import FacetDeviceBasis
+            self._device = FacetDeviceBasis(self)
+        return self._device
+
+    def __repr__(self):
+        return "FacetBasis(%r, %r, %d facets, %d quadrature points)" % (
+            self.mesh, self.elem, self.nelems, self.W.shape[0])

@@ -299,3 +299,115 @@ class DeviceBasis:
         contrib = self.local_linear(name)
         perm, seg = self.rows_plan()
         return self.segment_sum(contrib, perm, seg, self.N)
+
+
+class FacetDeviceBasis(DeviceBasis):
+    """Device side of a ``FacetBasis``: the cell mesh (for the element
+    mapping of the owning elements), the facets, ``tind`` and the facet
+    quadrature.  Local entries come from ``femhip_facet_local``; the pattern
+    and the sums go through the generic path of ``DeviceBasis`` with one
+    "element" per facet (``element_dofs[:, tind]``)."""
+
+    def __init__(self, basis):
+        lib = _lib.load()
+        self.lib = lib
+        self.basis = basis
+        mesh, elem = basis.mesh, basis.elem
+        self.kind = mesh.kind
+        self.dim = mesh.dim
+        self.E = basis.nelems              # one local block per facet
+        self.Q = basis.W.shape[0]
+        self.nbfun = basis.Nbfun
+        self.N = basis.N
+        self.p = to_dev(mesh.p, np.float64)
+        self.t = to_dev(mesh.t, np.int32)
+        self.dofs = to_dev(basis.element_dofs, np.int32)
+        self.facets = to_dev(mesh.facets[:, basis.find], np.int32)
+        self.tind = to_dev(basis.tind, np.int32)
+        self.X = to_dev(basis.X, np.float64)
+        self.W = to_dev(basis.W, np.float64)
+        # the element mapping of the whole cell mesh (affine: one entry per
+        # element, independent of the quadrature)
+        self.geom = _lib.Geom(_KIND[self.kind], self.dim, mesh.nverts, 1,
+                              mesh.nelements, mesh.p.shape[1], self.p.data_ptr(),
+                              self.t.data_ptr(), self.W.data_ptr(), 0)
+        self.fgeom = _lib.FacetGeom(_KIND[self.kind], self.dim, elem.maxdeg, self.Q,
+                                    self.E, mesh.p.shape[1], mesh.nelements,
+                                    self.p.data_ptr(), self.facets.data_ptr(),
+                                    self.t.data_ptr(), self.tind.data_ptr(),
+                                    self.X.data_ptr(), self.W.data_ptr())
+        self._G = None
+        self._normals = None
+        self.p_version = 0
+        self.plans = {}
+        self.validated = {}
+        self.fused_plans = {}
+        self.row_plan = None
+
+    def set_coordinates(self, p):
+        super().set_coordinates(p)
+        self.fgeom.p = self.p.data_ptr()
+        self._normals = None
+
+    def local_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, out=None):
+        if name != 'mass':
+            raise NotImplementedError("facet bilinear forms: mass only")
+        if out is None:
+            out = torch.empty(self.nbfun * self.nbfun * self.E, dtype=torch.float64,
+                              device=self.p.device)
+        if self.E == 0:
+            return out
+        G = self.geometry()
+        _lib.check(self.lib.femhip_facet_local(ctypes.byref(self.fgeom), _FORM[name],
+                                               _lib.ptr(G), _lib.ptr(out), _lib.stream_ptr()))
+        return out
+
+    def local_linear(self, name='unit_load'):
+        if name != 'unit_load':
+            raise NotImplementedError("facet linear forms: unit_load only")
+        out = torch.empty(self.nbfun * self.E, dtype=torch.float64, device=self.p.device)
+        if self.E == 0:
+            return out
+        G = self.geometry()
+        _lib.check(self.lib.femhip_facet_local(ctypes.byref(self.fgeom), _FORM[name],
+                                               _lib.ptr(G), _lib.ptr(out), _lib.stream_ptr()))
+        return out
+
+    def normals(self):
+        """(dim, nfacets, Q) outward unit normals seen from ``f2t[0]``."""
+        if self._normals is None and self.E == 0:
+            self._normals = torch.zeros((self.dim, 0, self.Q), dtype=torch.float64,
+                                        device=self.p.device)
+        if self._normals is None:
+            G = self.geometry()
+            tn = to_dev(self.basis.tind_normals, np.int32)
+            floc = to_dev(self.basis.facet_local_index(), np.int32)
+            out = torch.empty((self.dim, self.E, self.Q), dtype=torch.float64,
+                              device=self.p.device)
+            _lib.check(self.lib.femhip_facet_normals(
+                ctypes.byref(self.fgeom), _lib.ptr(tn), _lib.ptr(floc), _lib.ptr(G),
+                _lib.ptr(out), _lib.stream_ptr()))
+            torch.cuda.current_stream().synchronize()
+            self._normals = out
+        return self._normals
+
+    def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, token=None,
+                          path='auto'):
+        """Exact path: local entries -> pattern (cached by zero signature)
+        -> segmented sums."""
+        if self.E == 0:
+            self.local_bilinear(name)          # form check only
+            dev = self.p.device
+            return DeviceCSR(torch.zeros(self.N + 1, dtype=torch.int32, device=dev),
+                             torch.zeros(0, dtype=torch.int32, device=dev),
+                             torch.zeros(0, dtype=torch.float64, device=dev), (self.N, self.N))
+        contrib = self.local_bilinear(name)
+        plan = self.plan_for(contrib)
+        data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
+        return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N))
+
+    def assemble_linear(self, name='unit_load'):
+        if self.E == 0:
+            self.local_linear(name)
+            return torch.zeros(self.N, dtype=torch.float64, device=self.p.device)
+        return super().assemble_linear(name)

@@ -137,6 +137,63 @@ class Mesh:
             self._init_edges()
         return self._t2e
 
+    @property
+    def f2t(self):
+        """(2, nfacets) int32: the elements on either side of every facet
+        (``Mesh.build_inverse``, ``mesh.py:1085-1100``).  Reading ``t2f`` in
+        C order (local facet, then element), row 0 holds the element of the
+        facet's first occurrence and row 1 that of its last; a facet seen
+        once (a boundary facet) has -1 in row 1."""
+        if not hasattr(self, '_f2t'):
+            t2f = np.asarray(self.t2f)
+            E = t2f.shape[1]
+            flat = t2f.ravel()
+            order = np.argsort(flat, kind='stable')   # positions by facet, C order kept
+            cnt = np.bincount(flat, minlength=self.nfacets)
+            end = np.cumsum(cnt)
+            f2t = np.empty((2, self.nfacets), dtype=np.int32)
+            f2t[0] = order[end - cnt] % E
+            f2t[1] = order[end - 1] % E
+            f2t[1, cnt == 1] = -1
+            self._f2t = f2t
+        return self._f2t
+
+    @property
+    def brefdom(self):
+        """Reference domain of the facets (``mesh.py:57-59``)."""
+        return {'tri': 'line', 'tet': 'tri', 'hex': 'quad'}[self.kind]
+
+    def facets_satisfying(self, test, boundaries_only=False):
+        """Facets whose midpoints satisfy ``test`` (``mesh.py:426-456``,
+        without the ``normal`` orientation)."""
+        midp = self.p[:, self.facets].mean(axis=1)
+        facets = np.nonzero(test(midp))[0].astype(np.int32)
+        if boundaries_only:
+            facets = np.intersect1d(facets, self.boundary_facets())
+        return facets
+
+    def normalize_facets(self, facets):
+        """Facet indices from a selection (``mesh.py:1291-1329``): None (the
+        boundary), an index or index array, a list/tuple/set of selections
+        (their sorted union), a callable on facet midpoints, or the name of
+        a boundary in ``self.boundaries``."""
+        if isinstance(facets, (int, np.integer)):
+            return np.array([facets])
+        if isinstance(facets, np.ndarray):
+            return facets
+        if facets is None:
+            return self.boundary_facets()
+        if isinstance(facets, (tuple, list, set)):
+            return np.unique(np.concatenate([self.normalize_facets(f) for f in facets]))
+        if callable(facets):
+            return self.facets_satisfying(facets)
+        if isinstance(facets, str):
+            bnd = getattr(self, 'boundaries', None)
+            if bnd is not None and facets in bnd:
+                return bnd[facets]
+            raise ValueError("Boundary '{}' not found.".format(facets))
+        raise NotImplementedError
+
     def boundary_facets(self):
         """Facets that belong to exactly one element (``mesh.py:197-199``)."""
         counts = np.bincount(self.t2f.ravel(), minlength=self.nfacets)

@@ -92,5 +92,7 @@ def get_quadrature(kind, norder):
         return quadrature_tet(norder)
     if kind == 'hex':
         return quadrature_hex(norder)
+    if kind == 'line':
+        return quadrature_line(norder)
     raise NotImplementedError("The given reference domain type '%s' "
                               "is not supported!" % kind)

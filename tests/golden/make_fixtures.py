@@ -16,6 +16,10 @@ What is written (all under tests/golden/):
 * ``digest_*.json`` (``--big``) -- SHA-256 of indptr/indices, nnz, max|A|,
   sum(data) and 4096 seeded (slot, value) samples of the large cases, plus
   the SHA-256 of the jittered node array the outputs were computed on.
+* ``facet_*.npz`` -- boundary forms on a ``FacetBasis``: the facets and
+  their owning elements, the facet quadrature, dx, the outward normals, the
+  CSR facet mass matrix and the facet load vector (default boundary facets,
+  and a set of interior facets seen from either side).
 * ``grad_kappa65.npz`` (``--big``) -- the reference gradient of
   L = 1/2 u^T A(kappa, X) u - f^T u on the 65^3 tet mesh: dL/dkappa_e from
   single-element assemblies (unit coefficient) and dL/dX from central
@@ -154,6 +158,43 @@ def write_small():
                               for f in forms})
 
 
+FACET = [
+    # name, mesh kind, n, jitter a, element, facet selection, side
+    ('tri1_n9j', 'tri', 9, 0.2, 'tri1', None, 0),
+    ('tri2_n9j', 'tri', 9, 0.2, 'tri2', None, 0),
+    ('tri1_n9j_mid1', 'tri', 9, 0.2, 'tri1', 'mid', 1),
+    ('tet1_n6j', 'tet', 6, 0.2, 'tet1', None, 0),
+    ('tet2_n5j', 'tet', 5, 0.2, 'tet2', None, 0),
+    ('tet2_n5j_mid0', 'tet', 5, 0.2, 'tet2', 'mid', 0),
+    ('tet1_n6', 'tet', 6, 0.0, 'tet1', None, 0),
+]
+
+
+def write_facet():
+    from skfem import FacetBasis
+    for name, kind, n, a, elem, sel, side in FACET:
+        m = mk_mesh(kind, n, a)
+        facets = None
+        if sel == 'mid':
+            # interior facets on the grid plane x = 0.5 (n odd), selected on
+            # the undistorted mesh (same t, hence same facet numbering)
+            m0 = mk_mesh(kind, n)
+            facets = m0.facets_satisfying(lambda x: np.abs(x[0] - 0.5) < 1e-9)
+        fb = FacetBasis(m, ELEMS[elem](), facets=facets, side=side)
+        A = mass.assemble(fb).tocsr()
+        f = unit_load.assemble(fb)
+        out = dict(p=m.p, t=m.t, find=fb.find, tind=fb.tind,
+                   element_dofs=fb.element_dofs, N=np.array(fb.N), X=fb.X, W=fb.W,
+                   dx=fb.dx, normals=fb.normals.value,
+                   mass_indptr=A.indptr, mass_indices=A.indices, mass_data=A.data,
+                   unit_load_b=f)
+        meta = dict(kind=kind, n=n, jitter=a, element=elem,
+                    facets='boundary' if sel is None else sel, side=side)
+        np.savez_compressed(os.path.join(HERE, 'facet_%s.npz' % name),
+                            meta=json.dumps(meta), **out)
+        print('wrote facet', name, len(fb.find), 'facets, nnz', A.nnz)
+
+
 def write_config1():
     m = mk_mesh('tri', 129)
     b = Basis(m, ElementTriP1())
@@ -261,6 +302,8 @@ if __name__ == '__main__':
     if args.only in (None, 'small'):
         write_small()
         write_config1()
+    if args.only in (None, 'facet'):
+        write_facet()
     if args.big:
         big = {
             'tet1_n101j': ('tet', 101, 0.2, 'tet1', False, None,
