@@ -259,6 +259,53 @@ def bench_interpolate(basis):
             "frac_of_8TBps": nbytes / (ms * 1e-3) / HBM_PEAK}
 
 
+def bench_facet(m):
+    """Boundary forms on the boundary facets of the 101^3 jittered tet mesh
+    with P1 (FacetBasis): facet mass (local entries + segmented sums over the
+    cached pattern) and facet load.  Local-kernel bytes per facet: facet
+    vertices, tind and t[0, e] (4 B each), their coordinates (8 B x dim x
+    (dim + 1)), the element mapping rows (8 B x dim^2) read, nbfun^2 doubles
+    written."""
+    import torch
+    import femhip
+    from femhip.models import mass, unit_load
+    t0 = time.perf_counter()
+    fb = femhip.FacetBasis(m, femhip.ElementTetP1())
+    d = fb.device()
+    A = mass.assemble_device(fb)
+    b_first = unit_load.assemble_device(fb)
+    torch.cuda.synchronize()
+    first = (time.perf_counter() - t0) * 1e3
+    plan = next(iter(d.plans.values()))
+    nf, nb, D = d.E, d.nbfun, d.dim
+    stream = torch.cuda.current_stream()
+    contrib = torch.empty(nb * nb * nf, dtype=torch.float64, device='cuda')
+    out = torch.empty(max(plan.nnz, 1), dtype=torch.float64, device='cuda')[:plan.nnz]
+
+    def step():
+        d.local_bilinear('mass', out=contrib)
+        d.segment_sum(contrib, plan.perm, plan.seg, plan.nnz, out=out)
+    step()
+    ms = time_stream(step, 20, stream)
+    local_ms = time_stream(lambda: d.local_bilinear('mass', out=contrib), 20, stream)
+    perm, seg = d.rows_plan()
+    bout = torch.empty(fb.N, dtype=torch.float64, device='cuda')
+
+    def lstep():
+        d.segment_sum(d.local_linear('unit_load'), perm, seg, fb.N, out=bout)
+    lstep()
+    load_ms = time_stream(lstep, 20, stream)
+    torch.cuda.synchronize()
+    err = float((out - A.data).abs().max() / A.data.abs().max())
+    errb = float((bout - b_first).abs().max() / b_first.abs().max())
+    local_bytes = nf * (4 * (D + 2) + 8 * D * (D + 1) + 8 * D * D + 8 * nb * nb)
+    return {"facets": nf, "nnz": plan.nnz, "first_call_ms": first,
+            "mass_ms": ms, "mass_facets_per_s": nf / (ms * 1e-3),
+            "local_kernel_ms": local_ms,
+            "local_kernel_GBps": local_bytes / (local_ms * 1e-3) / 1e9,
+            "load_ms": load_ms, "max_rel_err_vs_first_call": max(err, errb)}
+
+
 def bench_elasticity(n=41):
     """linear_elasticity on ElementVector(ElementHex2), 27 points, MeshHex
     (n-1)^3 jittered 0.15h: MFMA local matrices + segmented sums."""
@@ -311,6 +358,10 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-extra', action='store_true',
                     help='skip the batched / adjoint / elasticity measurements')
+    ap.add_argument('--extra', default=None,
+                    help='comma-separated subset of the extra measurements '
+                         '(interpolate, facet, device_solve, batched_kappa, '
+                         'mesh_batch, adjoint, elasticity)')
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -419,21 +470,19 @@ def main():
     res["roofline"]["frac_of_copy_bw"] = achieved / 1e9 / res["copy_bw_GBps"]
     del src, dst
     if not args.no_extra:
-        log('extra: interpolate')
-        try:
-            res["interpolate_tet101"] = bench_interpolate(basis)
-        except Exception as exc:
-            res["interpolate_tet101"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
-        log('extra: device_solve')
-        try:
-            res["device_solve_tet101"] = bench_device_solve(basis, A)
-        except Exception as exc:
-            res["device_solve_tet101"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
-        torch.cuda.synchronize()
-        for name, fn in (("batched_kappa_p2tri257_S256", bench_batched_kappa),
-                         ("mesh_batch_64_p1tri", bench_mesh_batch),
-                         ("adjoint_kappa_tet65", bench_adjoint),
-                         ("elasticity_vhex2_40", bench_elasticity)):
+        extras = (
+            ('interpolate', "interpolate_tet101", lambda: bench_interpolate(basis)),
+            ('facet', "facet_mass_tet101_boundary", lambda: bench_facet(m)),
+            ('device_solve', "device_solve_tet101", lambda: bench_device_solve(basis, A)),
+            ('batched_kappa', "batched_kappa_p2tri257_S256", bench_batched_kappa),
+            ('mesh_batch', "mesh_batch_64_p1tri", bench_mesh_batch),
+            ('adjoint', "adjoint_kappa_tet65", bench_adjoint),
+            ('elasticity', "elasticity_vhex2_40", bench_elasticity),
+        )
+        chosen = None if args.extra is None else set(args.extra.split(','))
+        for short, name, fn in extras:
+            if chosen is not None and short not in chosen:
+                continue
             log('extra:', name)
             try:
                 res[name] = fn()

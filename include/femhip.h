@@ -141,20 +141,21 @@ typedef struct femhip_facet_geom {
 } femhip_facet_geom_t;
 
 /* Local entries of a boundary form on the facets, in the reference's
- * arithmetic (mapping_affine.py:195-203,234-281 for G / invF / detDG):
+ * arithmetic (FacetBasis.__init__, facet_basis.py:93-110: MappingAffine.G
+ * mapping_affine.py:234-245, invF :195-203, detDG :247-253; the owning
+ * element's invA from its vertices, :55-131):
  * form FEMHIP_MASS -> out[(j*nb + i)*nfacets + f] (BilinearForm._assemble
  * layout, bilinear_form.py:58-128), FEMHIP_UNIT_LOAD -> out[i*nfacets + f]
- * (linear_form.py:18-49).  G is the cell mapping of femhip_geometry. */
-int femhip_facet_local(const femhip_facet_geom_t* g, int32_t form, const double* G,
-                       double* out, femhip_stream_t s);
+ * (linear_form.py:18-49).  nq is 1..7. */
+int femhip_facet_local(const femhip_facet_geom_t* g, int32_t form, double* out,
+                       femhip_stream_t s);
 
 /* Outward unit normals of the facets seen from tind_normals[f] (= f2t[0]),
  * whose local facet (RefTri/RefTet order) is floc[f]
  * (MappingAffine.normals, mapping_affine.py:255-281; FacetBasis.normals,
  * facet_basis.py:96-104): out[(j*nfacets + f)*nq + q]. */
 int femhip_facet_normals(const femhip_facet_geom_t* g, const int32_t* tind_normals,
-                         const int32_t* floc, const double* G, double* out,
-                         femhip_stream_t s);
+                         const int32_t* floc, double* out, femhip_stream_t s);
 
 /* Basis.interpolate (abstract_basis.py:271-322) of a DOF vector w[ndofs]
  * for a scalar element: value[nelems][nq] and, when grad != NULL,

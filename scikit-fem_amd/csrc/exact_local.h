@@ -47,6 +47,48 @@ struct MapAt {
 
 // grad[j] = sum_i invDF[i, j] * dphi[i]   (einsum 'ijkl,il->jkl',
 // element_h1.py:17), accumulated from 0 in index order.
+// MappingAffine._init_Ab/_init_invA (mapping_affine.py:55-131) for element e:
+// A[:, j] = p[t[j+1, e]] - p[t[0, e]], the determinant by cofactor expansion
+// along the first row and every entry of invA as the reference writes it.
+// Returns the signed determinant.  Used by every kernel that needs the affine
+// mapping of an element (the geometry pass and the facet kernels).
+template <int D>
+__device__ __forceinline__ double affine_inverse(const double* __restrict__ p,
+                                                 const int32_t* __restrict__ t, int64_t N,
+                                                 int64_t E, int64_t e, double (&inv)[D][D]) {
+#pragma clang fp contract(off)
+  const int32_t n0 = t[e];
+  double A[D][D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const int32_t nj = t[(int64_t)(j + 1) * E + e];
+#pragma unroll
+    for (int i = 0; i < D; ++i) A[i][j] = p[i * N + nj] - p[i * N + n0];
+  }
+  if constexpr (D == 2) {
+    const double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    inv[0][0] = A[1][1] / det;
+    inv[0][1] = -A[0][1] / det;
+    inv[1][0] = -A[1][0] / det;
+    inv[1][1] = A[0][0] / det;
+    return det;
+  } else {
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                       A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                       A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    inv[0][0] = (-A[1][2] * A[2][1] + A[1][1] * A[2][2]) / det;
+    inv[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+    inv[2][0] = (-A[1][1] * A[2][0] + A[1][0] * A[2][1]) / det;
+    inv[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+    inv[1][1] = (-A[0][2] * A[2][0] + A[0][0] * A[2][2]) / det;
+    inv[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+    inv[0][2] = (-A[0][2] * A[1][1] + A[0][1] * A[1][2]) / det;
+    inv[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+    inv[2][2] = (-A[0][1] * A[1][0] + A[0][0] * A[1][1]) / det;
+    return det;
+  }
+}
+
 template <int D>
 __device__ __forceinline__ void global_grad(const double (&inv)[D][D],
                                             const double* __restrict__ dphi,

@@ -34,36 +34,13 @@ affine_map_kernel(const double* __restrict__ p, const int32_t* __restrict__ t,
 #pragma clang fp contract(off)
   const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (e >= E) return;
-  const int32_t n0 = t[e];
-  double A[D][D];
+  double inv[D][D];
+  const double det = exact::affine_inverse<D>(p, t, N, E, e, inv);
 #pragma unroll
-  for (int j = 0; j < D; ++j) {
-    const int32_t nj = t[(int64_t)(j + 1) * E + e];
+  for (int i = 0; i < D; ++i)
 #pragma unroll
-    for (int i = 0; i < D; ++i) A[i][j] = p[i * N + nj] - p[i * N + n0];
-  }
-  if constexpr (D == 2) {
-    const double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
-    G[0 * E + e] = A[1][1] / det;
-    G[1 * E + e] = -A[0][1] / det;
-    G[2 * E + e] = -A[1][0] / det;
-    G[3 * E + e] = A[0][0] / det;
-    G[4 * E + e] = fabs(det);
-  } else {
-    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
-                       A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
-                       A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
-    G[0 * E + e] = (-A[1][2] * A[2][1] + A[1][1] * A[2][2]) / det;  // [0,0]
-    G[3 * E + e] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;   // [1,0]
-    G[6 * E + e] = (-A[1][1] * A[2][0] + A[1][0] * A[2][1]) / det;  // [2,0]
-    G[1 * E + e] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;   // [0,1]
-    G[4 * E + e] = (-A[0][2] * A[2][0] + A[0][0] * A[2][2]) / det;  // [1,1]
-    G[7 * E + e] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;   // [2,1]
-    G[2 * E + e] = (-A[0][2] * A[1][1] + A[0][1] * A[1][2]) / det;  // [0,2]
-    G[5 * E + e] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;   // [1,2]
-    G[8 * E + e] = (-A[0][1] * A[1][0] + A[0][0] * A[1][1]) / det;  // [2,2]
-    G[9 * E + e] = fabs(det);
-  }
+    for (int j = 0; j < D; ++j) G[(int64_t)(i * D + j) * E + e] = inv[i][j];
+  G[(int64_t)D * D * E + e] = fabs(det);
 }
 
 // MappingIsoparametric._J/detDF/invDF, mapping_isoparametric.py:112-226,

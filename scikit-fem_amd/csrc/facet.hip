@@ -1,28 +1,39 @@
-// Local entries of boundary forms on a FacetBasis (facet_basis.py:23-110)
-// for P1/P2 triangles and tetrahedra: facet mass u*v and facet load 1*v
-// (models/poisson.py), in the reference's arithmetic so that the exact
-// zeros -- the basis functions of the vertex opposite a facet, evaluated
-// at the facet's quadrature points -- come out as the reference's do.
+// Boundary forms on a FacetBasis (facet_basis.py:23-110) for P1/P2
+// triangles and tetrahedra: the local entries of the facet mass u*v and the
+// facet load 1*v (models/poisson.py), and the facet normals, in the
+// reference's arithmetic, so that the exact zeros -- the basis functions of
+// the vertex opposite a facet, evaluated at the facet's quadrature points --
+// come out as the reference's do.
 //
-// Per boundary facet f (vertices facets[:, f], owning element e = tind[f])
-// and facet quadrature point X_q:
+// Per facet f (vertices facets[:, f], owning element e = tind[f]) and facet
+// quadrature point X_q:
 //   x   = B X_q + c              MappingAffine.G, mapping_affine.py:234-245
 //         (B[:, j] = p[f_{j+1}] - p[f_0], c = p[f_0]; einsum in index order)
 //   Y   = invA_e (x - b_e)       MappingAffine.invF, mapping_affine.py:195-203
-//         (invA_e from the element mapping, femhip_geometry's affine rows)
+//         (invA_e, b_e = p[t[0, e]] from the element's vertices,
+//         mapping_affine.py:55-131)
 //   dx  = |detB| W_q             detB: mapping_affine.py:158-181
 //   phi_i(Y) by the element's polynomial in the reference's operation order
 //         (element_tri_p1.py, element_tri_p2.py, element_tet_p1.py,
 //         element_tet_p2.py)
 // and, as BilinearForm._kernel / LinearForm._kernel do, the entries are
 // np.sum((phi_j phi_i) dx, axis=1) and np.sum((1.0 phi_i) dx, axis=1).
-// Compiled with -ffp-contract=off.
-#include "common.h"
+//
+// One thread per facet computes the facet's geometry once and all of its
+// entries; the output is facet-minor (entry k of facet f at k * nf + f), so
+// every store of a wave is one contiguous run.  Compiled with
+// -ffp-contract=off.
+#include "exact_local.h"
 
 namespace femhip {
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kFacetBlock = 256;
+
+template <int D, int DEG>
+constexpr int nbasis() {
+  return D == 2 ? (DEG == 1 ? 3 : 6) : (DEG == 1 ? 4 : 10);
+}
 
 // P1 / P2 simplex basis values at Y (reference expressions, left to right)
 template <int D, int DEG>
@@ -73,18 +84,17 @@ __device__ __forceinline__ double lbasis(int i, const double (&Y)[D]) {
 
 struct FacetArgs {
   const double* p;        // [D][nnodes]
-  const int32_t* facets;  // [D][nf] vertices of the boundary facets
+  const int32_t* facets;  // [D][nf] vertices of the facets
   const int32_t* t;       // [D+1][E] element vertices
   const int32_t* tind;    // [nf] owning element
-  const double* G;        // element mapping: [D*D+1][E] (invA rows, |det|)
   const double* X;        // [D-1][Q] facet quadrature points
   const double* W;        // [Q]
   double* out;            // mass [(j nb + i) nf + f], load [i nf + f]
   int64_t nnodes, E, nf;
 };
 
-// the facet's geometry: Y (element reference coordinates of the facet
-// quadrature points) and dx
+// the facet's geometry: Y (reference coordinates of the facet quadrature
+// points in the owning element) and dx
 template <int D, int Q>
 struct FacetPoints {
   double Y[Q][D];
@@ -113,12 +123,10 @@ struct FacetPoints {
     }
     const int64_t e = a.tind[f];
     double inv[D][D], b[D];
+    exact::affine_inverse<D>(a.p, a.t, N, a.E, e, inv);
+    const int32_t n0 = a.t[e];
 #pragma unroll
-    for (int i = 0; i < D; ++i) {
-      b[i] = a.p[i * N + a.t[e]];
-#pragma unroll
-      for (int j = 0; j < D; ++j) inv[i][j] = a.G[(int64_t)(i * D + j) * a.E + e];
-    }
+    for (int i = 0; i < D; ++i) b[i] = a.p[i * N + n0];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       double y[D];
@@ -141,36 +149,44 @@ struct FacetPoints {
   }
 };
 
-template <int D, int DEG, int Q>
-__global__ void __launch_bounds__(kBlock) facet_mass_kernel(FacetArgs a) {
+template <int D, int DEG, int Q, bool MASS>
+__global__ void __launch_bounds__(kFacetBlock) facet_local_kernel(FacetArgs a) {
 #pragma clang fp contract(off)
-  constexpr int NB = D == 2 ? (DEG == 1 ? 3 : 6) : (DEG == 1 ? 4 : 10);
-  const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  constexpr int NB = nbasis<D, DEG>();
+  const int64_t f = (int64_t)blockIdx.x * kFacetBlock + threadIdx.x;
   if (f >= a.nf) return;
-  const int pair = blockIdx.y;  // j * NB + i (j trial, i test)
-  const int j = pair / NB, i = pair % NB;
   FacetPoints<D, Q> fp;
   fp.init(a, f);
-  auto term = [&](int q) -> double {
+  double phi[NB][Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+#pragma unroll
+    for (int i = 0; i < NB; ++i) phi[i][q] = lbasis<D, DEG>(i, fp.Y[q]);
+  const int64_t nf = a.nf;
+  if constexpr (MASS) {
+    // (phi_j phi_i) dx == (phi_i phi_j) dx exactly: each unordered pair once
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int i = 0; i <= j; ++i) {
+        auto term = [&](int q) -> double {
 #pragma clang fp contract(off)
-    return (lbasis<D, DEG>(j, fp.Y[q]) * lbasis<D, DEG>(i, fp.Y[q])) * fp.dx[q];
-  };
-  a.out[(int64_t)pair * a.nf + f] = 0.0 + np_pairwise<0, Q>(term);
-}
-
-template <int D, int DEG, int Q>
-__global__ void __launch_bounds__(kBlock) facet_load_kernel(FacetArgs a) {
+          return (phi[j][q] * phi[i][q]) * fp.dx[q];
+        };
+        const double v = 0.0 + np_pairwise<0, Q>(term);
+        a.out[(int64_t)(j * NB + i) * nf + f] = v;
+        if (i != j) a.out[(int64_t)(i * NB + j) * nf + f] = v;
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      auto term = [&](int q) -> double {
 #pragma clang fp contract(off)
-  const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (f >= a.nf) return;
-  const int i = blockIdx.y;
-  FacetPoints<D, Q> fp;
-  fp.init(a, f);
-  auto term = [&](int q) -> double {
-#pragma clang fp contract(off)
-    return (1.0 * lbasis<D, DEG>(i, fp.Y[q])) * fp.dx[q];
-  };
-  a.out[(int64_t)i * a.nf + f] = 0.0 + np_pairwise<0, Q>(term);
+        return (1.0 * phi[i][q]) * fp.dx[q];
+      };
+      a.out[(int64_t)i * nf + f] = 0.0 + np_pairwise<0, Q>(term);
+    }
+  }
 }
 
 // outward unit normals seen from element tn[f] through its local facet
@@ -179,32 +195,34 @@ __global__ void __launch_bounds__(kBlock) facet_load_kernel(FacetArgs a) {
 // (einsum 'ijkl,ik->jkl', i in order), |n|^2 summed over j in order, and
 // n_j * (1 / |n|); the same value at every quadrature point.
 template <int D>
-__global__ void __launch_bounds__(kBlock)
-facet_normals_kernel(const int32_t* __restrict__ tn, const int32_t* __restrict__ floc,
-                     const double* __restrict__ G, int64_t E, int64_t nf, int Q,
-                     double* __restrict__ out) {
+__global__ void __launch_bounds__(kFacetBlock)
+facet_normals_kernel(const double* __restrict__ p, const int32_t* __restrict__ t,
+                     const int32_t* __restrict__ tn, const int32_t* __restrict__ floc,
+                     int64_t N, int64_t E, int64_t nf, int Q, double* __restrict__ out) {
 #pragma clang fp contract(off)
-  const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t f = (int64_t)blockIdx.x * kFacetBlock + threadIdx.x;
   if (f >= nf) return;
   const int64_t e = tn[f];
   const int k = floc[f];
-  double N[D];
+  double inv[D][D];
+  exact::affine_inverse<D>(p, t, N, E, e, inv);
+  double Nr[D];
   if constexpr (D == 2) {
     // facets (0,1), (1,2), (0,2)
-    N[0] = k == 0 ? 0. : (k == 1 ? 1. : -1.);
-    N[1] = k == 0 ? -1. : (k == 1 ? 1. : 0.);
+    Nr[0] = k == 0 ? 0. : (k == 1 ? 1. : -1.);
+    Nr[1] = k == 0 ? -1. : (k == 1 ? 1. : 0.);
   } else {
     // facets (0,1,2), (0,1,3), (0,2,3), (1,2,3)
-    N[0] = k == 2 ? -1. : (k == 3 ? 1. : 0.);
-    N[1] = k == 1 ? -1. : (k == 3 ? 1. : 0.);
-    N[2] = k == 0 ? -1. : (k == 3 ? 1. : 0.);
+    Nr[0] = k == 2 ? -1. : (k == 3 ? 1. : 0.);
+    Nr[1] = k == 1 ? -1. : (k == 3 ? 1. : 0.);
+    Nr[2] = k == 0 ? -1. : (k == 3 ? 1. : 0.);
   }
   double n[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) {
     double acc = 0.0;
 #pragma unroll
-    for (int i = 0; i < D; ++i) acc = acc + G[(int64_t)(i * D + j) * E + e] * N[i];
+    for (int i = 0; i < D; ++i) acc = acc + inv[i][j] * Nr[i];
     n[j] = acc;
   }
   double l2 = n[0] * n[0];
@@ -220,12 +238,11 @@ facet_normals_kernel(const int32_t* __restrict__ tn, const int32_t* __restrict__
 
 template <int D, int DEG, int Q>
 int launch(const FacetArgs& a, int form, hipStream_t s) {
-  constexpr int NB = D == 2 ? (DEG == 1 ? 3 : 6) : (DEG == 1 ? 4 : 10);
-  const unsigned gx = (unsigned)ceil_div(a.nf, kBlock);
+  const unsigned grid = (unsigned)ceil_div(a.nf, kFacetBlock);
   if (form == FEMHIP_MASS)
-    facet_mass_kernel<D, DEG, Q><<<dim3(gx, NB * NB), kBlock, 0, s>>>(a);
+    facet_local_kernel<D, DEG, Q, true><<<grid, kFacetBlock, 0, s>>>(a);
   else
-    facet_load_kernel<D, DEG, Q><<<dim3(gx, NB), kBlock, 0, s>>>(a);
+    facet_local_kernel<D, DEG, Q, false><<<grid, kFacetBlock, 0, s>>>(a);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }
@@ -241,9 +258,23 @@ int dispatch_q(const FacetArgs& a, int form, int Q, hipStream_t s) {
     case 6: return launch<D, DEG, 6>(a, form, s);
     case 7: return launch<D, DEG, 7>(a, form, s);
     default:
-      set_error("facet forms: unsupported number of facet quadrature points");
+      set_error("facet forms: 1 to 7 facet quadrature points");
       return FEMHIP_EUNSUPPORTED;
   }
+}
+
+int check_facet_geom(const femhip_facet_geom_t* g) {
+  FEMHIP_REQUIRE(g != nullptr, FEMHIP_EINVAL, "facet forms: NULL geometry");
+  FEMHIP_REQUIRE((g->kind == FEMHIP_TRI && g->dim == 2) || (g->kind == FEMHIP_TET && g->dim == 3),
+                 FEMHIP_EUNSUPPORTED, "facet forms: triangles or tetrahedra only");
+  FEMHIP_REQUIRE(g->degree == 1 || g->degree == 2, FEMHIP_EUNSUPPORTED,
+                 "facet forms: P1 or P2 only");
+  FEMHIP_REQUIRE(g->nfacets >= 0 && g->nnodes >= 0 && g->nelems >= 0 && g->nq >= 1,
+                 FEMHIP_EINVAL, "facet forms: negative size or no quadrature points");
+  if (g->nfacets > 0)
+    FEMHIP_REQUIRE(g->p && g->facets && g->t && g->tind && g->X && g->W, FEMHIP_EINVAL,
+                   "facet forms: NULL array");
+  return FEMHIP_OK;
 }
 
 }  // namespace
@@ -251,45 +282,41 @@ int dispatch_q(const FacetArgs& a, int form, int Q, hipStream_t s) {
 
 using namespace femhip;
 
-extern "C" int femhip_facet_local(const femhip_facet_geom_t* g, int32_t form, const double* G,
-                                  double* out, femhip_stream_t s) {
-  FEMHIP_REQUIRE(g != nullptr && G != nullptr && out != nullptr, FEMHIP_EINVAL,
-                 "facet forms: NULL argument");
+extern "C" int femhip_facet_local(const femhip_facet_geom_t* g, int32_t form, double* out,
+                                  femhip_stream_t s) {
+  int rc = check_facet_geom(g);
+  if (rc) return rc;
   FEMHIP_REQUIRE(form == FEMHIP_MASS || form == FEMHIP_UNIT_LOAD, FEMHIP_EUNSUPPORTED,
                  "facet forms: mass and unit_load only");
-  FEMHIP_REQUIRE((g->kind == FEMHIP_TRI && g->dim == 2) || (g->kind == FEMHIP_TET && g->dim == 3),
-                 FEMHIP_EUNSUPPORTED, "facet forms: triangles or tetrahedra only");
-  FEMHIP_REQUIRE(g->degree == 1 || g->degree == 2, FEMHIP_EUNSUPPORTED,
-                 "facet forms: P1 or P2 only");
-  FEMHIP_REQUIRE(g->p && g->facets && g->t && g->tind && g->X && g->W, FEMHIP_EINVAL,
-                 "facet forms: NULL array");
   if (g->nfacets == 0) return FEMHIP_OK;
-  FacetArgs a{g->p, g->facets, g->t, g->tind, G, g->X, g->W, out, g->nnodes, g->nelems,
+  FEMHIP_REQUIRE(out != nullptr, FEMHIP_EINVAL, "facet forms: NULL output");
+  FacetArgs a{g->p, g->facets, g->t, g->tind, g->X, g->W, out, g->nnodes, g->nelems,
               g->nfacets};
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
   if (g->dim == 2)
-    return g->degree == 1 ? dispatch_q<2, 1>(a, form, g->nq, st) : dispatch_q<2, 2>(a, form, g->nq, st);
-  return g->degree == 1 ? dispatch_q<3, 1>(a, form, g->nq, st) : dispatch_q<3, 2>(a, form, g->nq, st);
+    return g->degree == 1 ? dispatch_q<2, 1>(a, form, g->nq, st)
+                          : dispatch_q<2, 2>(a, form, g->nq, st);
+  return g->degree == 1 ? dispatch_q<3, 1>(a, form, g->nq, st)
+                        : dispatch_q<3, 2>(a, form, g->nq, st);
 }
 
 extern "C" int femhip_facet_normals(const femhip_facet_geom_t* g, const int32_t* tind_normals,
-                                    const int32_t* floc, const double* G, double* out,
-                                    femhip_stream_t s) {
-  FEMHIP_REQUIRE(g != nullptr && tind_normals != nullptr && floc != nullptr && G != nullptr &&
-                     out != nullptr,
-                 FEMHIP_EINVAL, "facet normals: NULL argument");
-  FEMHIP_REQUIRE((g->kind == FEMHIP_TRI && g->dim == 2) || (g->kind == FEMHIP_TET && g->dim == 3),
-                 FEMHIP_EUNSUPPORTED, "facet normals: triangles or tetrahedra only");
-  FEMHIP_REQUIRE(g->nq >= 1, FEMHIP_EINVAL, "facet normals: no quadrature points");
+                                    const int32_t* floc, double* out, femhip_stream_t s) {
+  int rc = check_facet_geom(g);
+  if (rc) return rc;
   if (g->nfacets == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(tind_normals != nullptr && floc != nullptr && out != nullptr, FEMHIP_EINVAL,
+                 "facet normals: NULL argument");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
-  const unsigned grid = (unsigned)ceil_div(g->nfacets, kBlock);
+  const unsigned grid = (unsigned)ceil_div(g->nfacets, kFacetBlock);
   if (g->dim == 2)
-    facet_normals_kernel<2><<<grid, kBlock, 0, st>>>(tind_normals, floc, G, g->nelems, g->nfacets,
-                                                     g->nq, out);
+    facet_normals_kernel<2><<<grid, kFacetBlock, 0, st>>>(g->p, g->t, tind_normals, floc,
+                                                          g->nnodes, g->nelems, g->nfacets,
+                                                          g->nq, out);
   else
-    facet_normals_kernel<3><<<grid, kBlock, 0, st>>>(tind_normals, floc, G, g->nelems, g->nfacets,
-                                                     g->nq, out);
+    facet_normals_kernel<3><<<grid, kFacetBlock, 0, st>>>(g->p, g->t, tind_normals, floc,
+                                                          g->nnodes, g->nelems, g->nfacets,
+                                                          g->nq, out);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }

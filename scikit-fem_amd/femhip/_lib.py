@@ -130,8 +130,8 @@ SIGNATURES = {
                                              _p, _p, _p, _i32, _p]),
     'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
-    'femhip_facet_local': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _i32, _p, _p, _p]),
-    'femhip_facet_normals': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p, _p]),
+    'femhip_facet_local': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _i32, _p, _p]),
+    'femhip_facet_normals': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p]),
     'femhip_interpolate': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space), _p, _p,
                                           _p, _p, _p]),
     'femhip_segment_sum_multi': (ctypes.c_int, [_p, _i32, _i64, _p, _p, _i64, _p, _i64, _p]),

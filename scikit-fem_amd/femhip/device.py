@@ -326,11 +326,6 @@ class FacetDeviceBasis(DeviceBasis):
         self.tind = to_dev(basis.tind, np.int32)
         self.X = to_dev(basis.X, np.float64)
         self.W = to_dev(basis.W, np.float64)
-        # the element mapping of the whole cell mesh (affine: one entry per
-        # element, independent of the quadrature)
-        self.geom = _lib.Geom(_KIND[self.kind], self.dim, mesh.nverts, 1,
-                              mesh.nelements, mesh.p.shape[1], self.p.data_ptr(),
-                              self.t.data_ptr(), self.W.data_ptr(), 0)
         self.fgeom = _lib.FacetGeom(_KIND[self.kind], self.dim, elem.maxdeg, self.Q,
                                     self.E, mesh.p.shape[1], mesh.nelements,
                                     self.p.data_ptr(), self.facets.data_ptr(),
@@ -345,9 +340,14 @@ class FacetDeviceBasis(DeviceBasis):
         self.row_plan = None
 
     def set_coordinates(self, p):
-        super().set_coordinates(p)
+        assert p.shape == self.p.shape and p.dtype == torch.float64
+        self.p = p.contiguous()
         self.fgeom.p = self.p.data_ptr()
         self._normals = None
+        self.p_version += 1
+
+    def geometry(self):
+        raise NotImplementedError("a facet basis maps its facets inside the facet kernels")
 
     def local_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, out=None):
         if name != 'mass':
@@ -357,9 +357,8 @@ class FacetDeviceBasis(DeviceBasis):
                               device=self.p.device)
         if self.E == 0:
             return out
-        G = self.geometry()
         _lib.check(self.lib.femhip_facet_local(ctypes.byref(self.fgeom), _FORM[name],
-                                               _lib.ptr(G), _lib.ptr(out), _lib.stream_ptr()))
+                                               _lib.ptr(out), _lib.stream_ptr()))
         return out
 
     def local_linear(self, name='unit_load'):
@@ -368,9 +367,8 @@ class FacetDeviceBasis(DeviceBasis):
         out = torch.empty(self.nbfun * self.E, dtype=torch.float64, device=self.p.device)
         if self.E == 0:
             return out
-        G = self.geometry()
         _lib.check(self.lib.femhip_facet_local(ctypes.byref(self.fgeom), _FORM[name],
-                                               _lib.ptr(G), _lib.ptr(out), _lib.stream_ptr()))
+                                               _lib.ptr(out), _lib.stream_ptr()))
         return out
 
     def normals(self):
@@ -379,14 +377,13 @@ class FacetDeviceBasis(DeviceBasis):
             self._normals = torch.zeros((self.dim, 0, self.Q), dtype=torch.float64,
                                         device=self.p.device)
         if self._normals is None:
-            G = self.geometry()
             tn = to_dev(self.basis.tind_normals, np.int32)
             floc = to_dev(self.basis.facet_local_index(), np.int32)
             out = torch.empty((self.dim, self.E, self.Q), dtype=torch.float64,
                               device=self.p.device)
             _lib.check(self.lib.femhip_facet_normals(
-                ctypes.byref(self.fgeom), _lib.ptr(tn), _lib.ptr(floc), _lib.ptr(G),
-                _lib.ptr(out), _lib.stream_ptr()))
+                ctypes.byref(self.fgeom), _lib.ptr(tn), _lib.ptr(floc), _lib.ptr(out),
+                _lib.stream_ptr()))
             torch.cuda.current_stream().synchronize()
             self._normals = out
         return self._normals
