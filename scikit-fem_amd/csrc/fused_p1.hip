@@ -53,6 +53,28 @@ __device__ __forceinline__ uint32_t bits(const uint32_t (&w)[RW]) {
   }
 }
 
+// A field inside one word as one v_bfe_u32.  Written as inline assembly so
+// that a following shift by 3 (a slot index to a byte offset) folds into
+// the address add (v_lshl_add_u32) instead of being merged into a
+// shift-and-mask pair; fields that straddle two words use bits().
+template <int POS, int NB, int RW>
+__device__ __forceinline__ uint32_t field(const uint32_t (&w)[RW]) {
+  constexpr int wi = POS >> 5, sh = POS & 31;
+  if constexpr (sh + NB <= 32 && sh > 0) {
+    uint32_t r;
+    asm("v_bfe_u32 %0, %1, %2, %3" : "=v"(r) : "v"(w[wi]), "I"(sh), "I"(NB));
+    return r;
+  } else {
+    return bits<POS, NB, RW>(w);
+  }
+}
+
+template <int BASE, int OFB, int RW, int... F>
+__device__ __forceinline__ void decode_fields(const uint32_t (&w)[RW], uint32_t* o,
+                                              std::integer_sequence<int, F...>) {
+  ((o[F] = field<BASE + F * OFB, OFB, RW>(w)), ...);
+}
+
 template <int BASE, int OFB, int RW, int... F>
 __device__ __forceinline__ void decode_offsets(const uint32_t (&w)[RW], uint32_t* o,
                                                std::integer_sequence<int, F...>) {
@@ -115,8 +137,6 @@ __device__ __forceinline__ double element_matrix(const double* X, const int (&lv
     n[3][0] = c1[1] * c2[2] - c1[2] * c2[1];
     n[3][1] = c1[2] * c2[0] - c1[0] * c2[2];
     n[3][2] = c1[0] * c2[1] - c1[1] * c2[0];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) n[0][d] = -(n[1][d] + n[2][d] + n[3][d]);
     const double det = c1[0] * n[1][0] + c1[1] * n[1][1] + c1[2] * n[1][2];
     if constexpr (FORM == FEMHIP_MASS) {
       const double m = fabs(det) * (1.0 / 120.0);
@@ -125,24 +145,27 @@ __device__ __forceinline__ double element_matrix(const double* X, const int (&lv
 #pragma unroll
         for (int j = 0; j < 4; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
     } else {
+      // vertices 1..3 directly; row and column 0 from the zero row sums
+      // (n_0 = -(n_1 + n_2 + n_3))
       const double s = coef * recip(6.0 * fabs(det));
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 1; i < 4; ++i)
 #pragma unroll
         for (int j = i; j < 4; ++j) {
           const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1] + n[i][2] * n[j][2]);
           K[i][j] = v;
           K[j][i] = v;
         }
+#pragma unroll
+      for (int j = 1; j < 4; ++j) K[0][j] = K[j][0] = -(K[j][1] + K[j][2] + K[j][3]);
+      K[0][0] = -(K[0][1] + K[0][2] + K[0][3]);
     }
     return fabs(det);
   } else {
     const double c1x = x[1][0] - x[0][0], c1y = x[1][1] - x[0][1];
     const double c2x = x[2][0] - x[0][0], c2y = x[2][1] - x[0][1];
     const double det = c1x * c2y - c1y * c2x;
-    double n[3][2] = {{0, 0}, {c2y, -c2x}, {-c1y, c1x}};
-    n[0][0] = -(n[1][0] + n[2][0]);
-    n[0][1] = -(n[1][1] + n[2][1]);
+    const double n[3][2] = {{0, 0}, {c2y, -c2x}, {-c1y, c1x}};
     if constexpr (FORM == FEMHIP_MASS) {
       const double m = fabs(det) * (1.0 / 24.0);
 #pragma unroll
@@ -150,15 +173,19 @@ __device__ __forceinline__ double element_matrix(const double* X, const int (&lv
 #pragma unroll
         for (int j = 0; j < 3; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
     } else {
+      // vertices 1, 2 directly; row and column 0 from the zero row sums
       const double s = coef * recip(2.0 * fabs(det));
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+      for (int i = 1; i < 3; ++i)
 #pragma unroll
         for (int j = i; j < 3; ++j) {
           const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1]);
           K[i][j] = v;
           K[j][i] = v;
         }
+#pragma unroll
+      for (int j = 1; j < 3; ++j) K[0][j] = K[j][0] = -(K[j][1] + K[j][2]);
+      K[0][0] = -(K[0][1] + K[0][2]);
     }
     return fabs(det);
   }
@@ -323,6 +350,14 @@ __device__ __forceinline__ void lds_add(unsigned long long* p, unsigned long lon
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// max(m, |v|) in one instruction (the |v| as a source modifier); the
+// operands are finite, so the NaN canonicalisation of fmax is not needed
+__device__ __forceinline__ double max_abs(double m, double v) {
+  double r;
+  asm("v_max_f64 %0, %1, |%2|" : "=v"(r) : "v"(m), "v"(v));
+  return r;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -344,9 +379,11 @@ static_assert(kPrefetch >= 1 && kPrefetch <= 3, "record prefetch depth 1 to 3");
 
 template <int RW>
 struct RecRing {
-  uint32_t w[kPrefetch][RW];
-  int32_t rec[kPrefetch];
-  int count[kPrefetch];
+  // depth 1 alternates between two buffers (no register copies)
+  static constexpr int kSlots = kPrefetch > 1 ? kPrefetch : 2;
+  uint32_t w[kSlots][RW];
+  int32_t rec[kSlots];
+  int count[kSlots];
   // records of chunk c of the partition's chunk list into slot S (lane tid:
   // record tid of the chunk)
   template <int S>
@@ -392,11 +429,11 @@ __device__ __forceinline__ void fx_pass(const FusedArgs& a, int chunk_start, int
     lv[1] = bits<IDB, IDB, RW>(w);
     lv[2] = bits<2 * IDB, IDB, RW>(w);
     if constexpr (NV == 4) lv[3] = bits<3 * IDB, IDB, RW>(w);
-    decode_offsets<NV * IDB, OFB, RW>(w, offs, std::make_integer_sequence<int, NV * (NV - 1)>{});
+    decode_fields<NV * IDB, OFB, RW>(w, offs, std::make_integer_sequence<int, NV * (NV - 1)>{});
     const double vol = element_matrix<D, FORM>(
         X, lv, FORM == FEMHIP_KAPPA_LAPLACE ? a.kappa[a.rec_elem[rec0 + tid]] : 1.0, K);
 #pragma unroll
-    for (int i = 0; i < NV; ++i) mk = fmax(mk, fabs(K[i][i]));
+    for (int i = 0; i < NV; ++i) mk = max_abs(mk, K[i][i]);
     if constexpr (LOAD) mv = fmax(mv, vol * LOADC);
     if constexpr (SCATTER) {
       unsigned long long F[NV][NV];
@@ -436,10 +473,22 @@ __device__ __forceinline__ void fx_pass(const FusedArgs& a, int chunk_start, int
     if (c + kPrefetch < nchunks) ring.template load<S>(a, chunk_start, c + kPrefetch, tid);
     if (active) element(w, rec0);
   };
-  for (int c = 0; c < nchunks; c += kPrefetch) {
-    step(std::integral_constant<int, 0>{}, c);
-    if constexpr (kPrefetch > 1) step(std::integral_constant<int, (kPrefetch > 1 ? 1 : 0)>{}, c + 1);
-    if constexpr (kPrefetch > 2) step(std::integral_constant<int, kPrefetch - 1>{}, c + 2);
+  if constexpr (kPrefetch == 1) {
+    // chunk c + 1 loads into one buffer while chunk c is processed from the
+    // other, which then receives chunk c + 2
+    for (int c = 0; c < nchunks; c += 2) {
+      if (c + 1 < nchunks) ring.template load<1>(a, chunk_start, c + 1, tid);
+      if (tid < ring.count[0]) element(ring.w[0], ring.rec[0]);
+      if (c + 1 >= nchunks) break;
+      if (c + 2 < nchunks) ring.template load<0>(a, chunk_start, c + 2, tid);
+      if (tid < ring.count[1]) element(ring.w[1], ring.rec[1]);
+    }
+  } else {
+    for (int c = 0; c < nchunks; c += kPrefetch) {
+      step(std::integral_constant<int, 0>{}, c);
+      step(std::integral_constant<int, (kPrefetch > 1 ? 1 : 0)>{}, c + 1);
+      if constexpr (kPrefetch > 2) step(std::integral_constant<int, kPrefetch - 1>{}, c + 2);
+    }
   }
 }
 
