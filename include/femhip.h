@@ -43,7 +43,9 @@ enum {
   FEMHIP_MASS = 1,          /* u * v                                    */
   FEMHIP_KAPPA_LAPLACE = 2, /* dot(w.kappa * grad u, grad v)            */
   FEMHIP_ELASTICITY = 3,    /* ddot(C(sym_grad u), sym_grad v)          */
-  FEMHIP_UNIT_LOAD = 4      /* v  (linear form)                         */
+  FEMHIP_UNIT_LOAD = 4,     /* v  (linear form)                         */
+  FEMHIP_COEF_MASS = 5,     /* w.c * u * v  (Robin / weighted mass)     */
+  FEMHIP_COEF_LOAD = 6      /* w.c * v  (Neumann / source data)         */
 };
 
 /* Mesh geometry and quadrature (CellBasis.mapping + quadrature). */
@@ -78,7 +80,8 @@ typedef struct femhip_space {
 typedef struct femhip_form {
   int32_t kind;         /* FEMHIP_LAPLACE ...                              */
   int32_t kappa_mode;   /* 0: none, 1: kappa[e], 2: kappa[e*nq + q]        */
-  const double* kappa;
+  const double* kappa;  /* the coefficient of KAPPA_LAPLACE, COEF_MASS,    */
+                        /* COEF_LOAD                                       */
   double lam;           /* linear_elasticity Lambda                        */
   double two_mu;        /* 2. * Mu, as the reference evaluates it          */
 } femhip_form_t;
@@ -144,11 +147,25 @@ typedef struct femhip_facet_geom {
  * arithmetic (FacetBasis.__init__, facet_basis.py:93-110: MappingAffine.G
  * mapping_affine.py:234-245, invF :195-203, detDG :247-253; the owning
  * element's invA from its vertices, :55-131):
- * form FEMHIP_MASS -> out[(j*nb + i)*nfacets + f] (BilinearForm._assemble
- * layout, bilinear_form.py:58-128), FEMHIP_UNIT_LOAD -> out[i*nfacets + f]
- * (linear_form.py:18-49).  nq is 1..7. */
-int femhip_facet_local(const femhip_facet_geom_t* g, int32_t form, double* out,
-                       femhip_stream_t s);
+ * form FEMHIP_MASS / FEMHIP_COEF_MASS -> out[(j*nb + i)*nfacets + f]
+ * (BilinearForm._assemble layout, bilinear_form.py:58-128),
+ * FEMHIP_UNIT_LOAD / FEMHIP_COEF_LOAD -> out[i*nfacets + f]
+ * (linear_form.py:18-49).  The COEF forms read coef[f] (coef_mode 1) or
+ * coef[f*nq + q] (coef_mode 2).  nq is 1..7. */
+int femhip_facet_local(const femhip_facet_geom_t* g, int32_t form, const double* coef,
+                       int32_t coef_mode, double* out, femhip_stream_t s);
+
+/* FacetBasis.interpolate (abstract_basis.py:271-322 on a FacetBasis) of a
+ * DOF vector w: value[f*nq + q] and grad[(j*nfacets + f)*nq + q] (grad may be
+ * NULL); dofs[i*nfacets + f] = element_dofs[i, tind[f]]. */
+int femhip_facet_interpolate(const femhip_facet_geom_t* g, const int32_t* dofs, const double* w,
+                             double* value, double* grad, femhip_stream_t s);
+
+/* FacetBasis.global_coordinates / mesh_parameters (facet_basis.py:126-139):
+ * x[(i*nfacets + f)*nq + q] = G(X) (mapping_affine.py:234-245) and
+ * h[f*nq + q] = |detB| ** (1/(dim-1)) (h may be NULL). */
+int femhip_facet_global_coordinates(const femhip_facet_geom_t* g, double* x, double* h,
+                                    femhip_stream_t s);
 
 /* Outward unit normals of the facets seen from tind_normals[f] (= f2t[0]),
  * whose local facet (RefTri/RefTet order) is floc[f]
@@ -156,6 +173,16 @@ int femhip_facet_local(const femhip_facet_geom_t* g, int32_t form, double* out,
  * facet_basis.py:96-104): out[(j*nfacets + f)*nq + q]. */
 int femhip_facet_normals(const femhip_facet_geom_t* g, const int32_t* tind_normals,
                          const int32_t* floc, double* out, femhip_stream_t s);
+
+/* CellBasis.global_coordinates / mesh_parameters (cell_basis.py:129-141):
+ * x[(i*nelems + e)*nq + q] = F(X_q) (MappingAffine.F, mapping_affine.py:183-192,
+ * from X[dim][nq]; MappingIsoparametric.F, mapping_isoparametric.py:50-64,
+ * from phi1[8][nq], the trilinear basis at the points) and
+ * h[e*nq + q] = |detDF| ** (1/dim) from the mapping G of femhip_geometry. */
+int femhip_global_coordinates(const femhip_geom_t* g, const double* X, const double* phi1,
+                              double* x, femhip_stream_t s);
+int femhip_mesh_parameters(const femhip_geom_t* g, const double* G, double* h,
+                           femhip_stream_t s);
 
 /* Basis.interpolate (abstract_basis.py:271-322) of a DOF vector w[ndofs]
  * for a scalar element: value[nelems][nq] and, when grad != NULL,

@@ -736,6 +736,15 @@ class CellBasis:
         self.dx = (np.abs(self.mapping.detDF(self.X))
                    * np.broadcast_to(self.W, (self.nelems, self.W.shape[-1])))
 
+    def global_coordinates(self):
+        """``CellBasis.global_coordinates`` (``cell_basis.py:129-134``):
+        the mapping ``F`` at the quadrature points, (dim, E, Q)."""
+        return self.mapping.F(self.X)
+
+    def mesh_parameters(self):
+        """``CellBasis.mesh_parameters`` (``cell_basis.py:136-141``)."""
+        return np.abs(self.mapping.detDF(self.X)) ** (1. / self.mesh.p.shape[0])
+
 
 def facet_mapping(mesh):
     """``MappingAffine._init_boundary_mapping`` (``mapping_affine.py:155-181``):
@@ -792,6 +801,9 @@ class FacetBasis:
         B, c, detB = facet_mapping(mesh)
         B, c = B[:, :, self.find], c[:, self.find]
         x = (np.einsum('ijk,jl', B, self.X).T + c.T).T
+        self.x = x                                   # global_coordinates, :126-131
+        dg = np.tile(detB[self.find], (self.X.shape[-1], 1)).T
+        self.h = np.abs(dg) ** (1. / (mesh.p.shape[0] - 1.))   # mesh_parameters, :133-139
         invA, b = mp.invA[:, :, self.tind], mp.b[:, self.tind]
         Y = np.einsum('ijk,jkl->ikl', invA, (x.T - b.T).T)
         invDF = np.einsum('ijk,l->ijkl', invA, np.ones(self.X.shape[-1]))
@@ -873,6 +885,16 @@ def form_elasticity(Lambda, Mu):
 
 def form_unit_load(v, w):
     return v[0]                                  # models/poisson.py:22-24
+
+
+def form_coef_mass(u, v, w):
+    """``w['c'] * u * v`` (a Robin term or a weighted mass)."""
+    return w['c'] * u[0] * v[0]
+
+
+def form_coef_load(v, w):
+    """``w['c'] * v`` (Neumann data or a source)."""
+    return w['c'] * v[0]
 
 
 # ---------------------------------------------------------------------------

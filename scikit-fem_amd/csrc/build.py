@@ -29,6 +29,7 @@ PER_FILE = {
     'exact_hex.hip': ['-ffp-contract=off'],
     'interpolate.hip': ['-ffp-contract=off'],
     'facet.hip': ['-ffp-contract=off'],
+    'coords.hip': ['-ffp-contract=off'],
 }
 
 

@@ -166,6 +166,10 @@ __global__ void __launch_bounds__(kBlock) local_bilinear_kernel(LocalArgs a) {
     double val;
     if constexpr (FK == FEMHIP_MASS) {
       val = a.phi[fu * Q + q] * a.phi[fv * Q + q];
+    } else if constexpr (FK == FEMHIP_COEF_MASS) {
+      // w.c * u * v, evaluated left to right
+      const double k = (a.kappa_mode == 1) ? a.kappa[e] : a.kappa[e * Q + q];
+      val = (k * a.phi[fu * Q + q]) * a.phi[fv * Q + q];
     } else {
       double gu[D], gv[D];
       global_grad<D>(m.inv, a.dphi, fu, q, Q, gu);
@@ -191,7 +195,7 @@ __global__ void __launch_bounds__(kBlock) local_bilinear_kernel(LocalArgs a) {
   a.out[(int64_t)entry * E + e] = r;
 }
 
-template <int D, bool AFF, int Q>
+template <int D, bool AFF, int Q, bool COEF>
 __global__ void __launch_bounds__(kBlock) local_load_kernel(LocalArgs a) {
 #pragma clang fp contract(off)
   const int64_t E = a.E;
@@ -204,6 +208,10 @@ __global__ void __launch_bounds__(kBlock) local_load_kernel(LocalArgs a) {
 #pragma clang fp contract(off)
     if constexpr (!AFF) m.load(a.G, E, e, q);
     const double dx = m.adet * a.W[q];
+    if constexpr (COEF) {  // w.c * v
+      const double k = (a.kappa_mode == 1) ? a.kappa[e] : a.kappa[e * Q + q];
+      return (k * a.phi[i * Q + q]) * dx;
+    }
     return a.phi[i * Q + q] * dx;
   };
   a.out[(int64_t)i * E + e] = 0.0 + np_pairwise<0, Q>(term);
@@ -230,8 +238,11 @@ int launch_bilinear_q(const LocalArgs& a, int fk, bool vec, hipStream_t s) {
       case FEMHIP_KAPPA_LAPLACE:
         local_bilinear_kernel<D, AFF, Q, FEMHIP_KAPPA_LAPLACE, false><<<grid, kBlock, 0, s>>>(a);
         break;
+      case FEMHIP_COEF_MASS:
+        local_bilinear_kernel<D, AFF, Q, FEMHIP_COEF_MASS, false><<<grid, kBlock, 0, s>>>(a);
+        break;
       default:
-        set_error("scalar elements support laplace, mass and kappa_laplace");
+        set_error("scalar elements support laplace, mass, kappa_laplace and coef_mass");
         return FEMHIP_EUNSUPPORTED;
     }
   }
@@ -242,7 +253,10 @@ int launch_bilinear_q(const LocalArgs& a, int fk, bool vec, hipStream_t s) {
 template <int D, bool AFF, int Q>
 int launch_load_q(const LocalArgs& a, hipStream_t s) {
   dim3 grid((unsigned)ceil_div(a.E, kBlock), (unsigned)a.nb);
-  local_load_kernel<D, AFF, Q><<<grid, kBlock, 0, s>>>(a);
+  if (a.kappa != nullptr)
+    local_load_kernel<D, AFF, Q, true><<<grid, kBlock, 0, s>>>(a);
+  else
+    local_load_kernel<D, AFF, Q, false><<<grid, kBlock, 0, s>>>(a);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }

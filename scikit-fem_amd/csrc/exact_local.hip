@@ -178,9 +178,10 @@ extern "C" int femhip_local_bilinear(const femhip_geom_t* g, const femhip_space_
   FEMHIP_REQUIRE(f != nullptr, FEMHIP_EINVAL, "form is NULL");
   FEMHIP_REQUIRE(f->kind != FEMHIP_UNIT_LOAD, FEMHIP_EINVAL,
                  "unit_load is a linear form");
-  FEMHIP_REQUIRE(f->kind != FEMHIP_KAPPA_LAPLACE ||
+  FEMHIP_REQUIRE(f->kind != FEMHIP_COEF_LOAD, FEMHIP_EINVAL, "coef_load is a linear form");
+  FEMHIP_REQUIRE((f->kind != FEMHIP_KAPPA_LAPLACE && f->kind != FEMHIP_COEF_MASS) ||
                      (f->kappa != nullptr && (f->kappa_mode == 1 || f->kappa_mode == 2)),
-                 FEMHIP_EINVAL, "kappa_laplace needs kappa (mode 1 or 2)");
+                 FEMHIP_EINVAL, "kappa_laplace / coef_mass need the coefficient (mode 1 or 2)");
   if (g->nelems == 0) return FEMHIP_OK;
   FEMHIP_REQUIRE(G != nullptr && contrib != nullptr, FEMHIP_EINVAL, "NULL G or output");
   exact::LocalArgs a{G, g->W, sp->phi, sp->dphi, f->kappa, contrib, g->nelems, sp->nbfun,
@@ -195,12 +196,15 @@ extern "C" int femhip_local_linear(const femhip_geom_t* g, const femhip_space_t*
   int rc = check_geom(g);
   if (rc) return rc;
   if ((rc = check_space(g, sp))) return rc;
-  FEMHIP_REQUIRE(f != nullptr && f->kind == FEMHIP_UNIT_LOAD, FEMHIP_EUNSUPPORTED,
-                 "only unit_load is a built-in linear form");
+  FEMHIP_REQUIRE(f != nullptr && (f->kind == FEMHIP_UNIT_LOAD || f->kind == FEMHIP_COEF_LOAD),
+                 FEMHIP_EUNSUPPORTED, "the built-in linear forms are unit_load and coef_load");
   FEMHIP_REQUIRE(!sp->vector, FEMHIP_EUNSUPPORTED, "unit_load needs a scalar element");
+  const bool coef = f->kind == FEMHIP_COEF_LOAD;
+  FEMHIP_REQUIRE(!coef || (f->kappa != nullptr && (f->kappa_mode == 1 || f->kappa_mode == 2)),
+                 FEMHIP_EINVAL, "coef_load needs the coefficient (mode 1 or 2)");
   if (g->nelems == 0) return FEMHIP_OK;
   FEMHIP_REQUIRE(G != nullptr && contrib != nullptr, FEMHIP_EINVAL, "NULL G or output");
-  exact::LocalArgs a{G, g->W, sp->phi, sp->dphi, nullptr, contrib, g->nelems, sp->nbfun,
-              0, 0.0, 0.0};
+  exact::LocalArgs a{G, g->W, sp->phi, sp->dphi, coef ? f->kappa : nullptr, contrib, g->nelems,
+              sp->nbfun, coef ? f->kappa_mode : 0, 0.0, 0.0};
   return dispatch_load(g->kind, g->nq, a, reinterpret_cast<hipStream_t>(s));
 }

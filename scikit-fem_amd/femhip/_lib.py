@@ -23,7 +23,7 @@ EDEGENERATE = -5
 ESIZE = -6
 
 TRI, TET, HEX = 0, 1, 2
-LAPLACE, MASS, KAPPA_LAPLACE, ELASTICITY, UNIT_LOAD = 0, 1, 2, 3, 4
+LAPLACE, MASS, KAPPA_LAPLACE, ELASTICITY, UNIT_LOAD, COEF_MASS, COEF_LOAD = 0, 1, 2, 3, 4, 5, 6
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -130,7 +130,11 @@ SIGNATURES = {
                                              _p, _p, _p, _i32, _p]),
     'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
-    'femhip_facet_local': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _i32, _p, _p]),
+    'femhip_facet_local': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _i32, _p, _i32, _p, _p]),
+    'femhip_facet_interpolate': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p, _p]),
+    'femhip_facet_global_coordinates': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p]),
+    'femhip_global_coordinates': (ctypes.c_int, [ctypes.POINTER(Geom), _p, _p, _p, _p]),
+    'femhip_mesh_parameters': (ctypes.c_int, [ctypes.POINTER(Geom), _p, _p, _p]),
     'femhip_facet_normals': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p]),
     'femhip_interpolate': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space), _p, _p,
                                           _p, _p, _p]),

@@ -38,6 +38,11 @@ class DiscreteField:
             v = v.cpu().numpy()
         return np.asarray(v, dtype=dtype)
 
+    def __getitem__(self, key):
+        """Indexing reads the value, as the reference's ndarray subclass
+        does (``discrete_field.py:55-56``): ``w.x[0]`` is the x component."""
+        return self.value[key]
+
 
 class CellBasis:
 
@@ -120,6 +125,22 @@ class CellBasis:
 
     def with_element(self, elem):
         return type(self)(self.mesh, elem, quadrature=(self.X, self.W))
+
+    def global_coordinates(self):
+        """Coordinates of the quadrature points, ``DiscreteField`` with
+        ``value`` (dim, E, Q) (``cell_basis.py:129-134``; bit-identical to the
+        reference's mapping ``F``), computed on the device."""
+        return DiscreteField(self.device().global_coordinates().cpu().numpy())
+
+    def mesh_parameters(self):
+        """``|detDF| ** (1 / dim)`` at the quadrature points, (E, Q)
+        (``cell_basis.py:136-141``), computed on the device."""
+        return DiscreteField(self.device().mesh_parameters().cpu().numpy())
+
+    def default_parameters(self):
+        """``{'x': global_coordinates(), 'h': mesh_parameters()}``
+        (``cell_basis.py:124-127``)."""
+        return {'x': self.global_coordinates(), 'h': self.mesh_parameters()}
 
     def device(self):
         """Device copy of this basis (created on first use)."""
@@ -206,6 +227,40 @@ class FacetBasis:
         its ``tind_normals`` element."""
         t2f = self.mesh.t2f[:, self.tind_normals]
         return np.argmax(t2f == self.find[None, :], axis=0).astype(np.int32)
+
+    def interpolate_device(self, w, grad=True):
+        """``interpolate`` with the result left on the device (a
+        ``DiscreteField`` of torch tensors)."""
+        import torch
+        if w.shape[0] != self.N:
+            raise ValueError("Input array has wrong size.")
+        d = self.device()
+        wt = torch.as_tensor(w, dtype=torch.float64, device=d.p.device).contiguous()
+        value, gr = d.interpolate(wt, grad)
+        return DiscreteField(value, gr)
+
+    def interpolate(self, w):
+        """A DOF vector at the facet quadrature points
+        (``abstract_basis.py:271-322`` on a FacetBasis): ``DiscreteField``
+        with ``value`` (nfacets, Q) and ``grad`` (dim, nfacets, Q), in the
+        reference's operation order, computed on the device."""
+        f = self.interpolate_device(np.asarray(w, dtype=np.float64))
+        return DiscreteField(f.value.cpu().numpy(), f.grad.cpu().numpy())
+
+    def global_coordinates(self):
+        """Facet quadrature points in global coordinates, (dim, nfacets, Q)
+        (``facet_basis.py:126-131``)."""
+        return DiscreteField(self.device().global_coordinates()[0].cpu().numpy())
+
+    def mesh_parameters(self):
+        """``|detB| ** (1 / (dim - 1))`` at the facet quadrature points
+        (``facet_basis.py:133-139``)."""
+        return DiscreteField(self.device().global_coordinates()[1].cpu().numpy())
+
+    def default_parameters(self):
+        """``{'x', 'h', 'n'}`` (``facet_basis.py:118-124``)."""
+        return {'x': self.global_coordinates(), 'h': self.mesh_parameters(),
+                'n': self.normals}
 
     @property
     def normals(self):

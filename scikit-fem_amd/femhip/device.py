@@ -35,7 +35,8 @@ def to_dev(a, dtype):
 
 _KIND = {'tri': _lib.TRI, 'tet': _lib.TET, 'hex': _lib.HEX}
 _FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS,
-         'kappa_laplace': _lib.KAPPA_LAPLACE,
+         'kappa_laplace': _lib.KAPPA_LAPLACE, 'coef_mass': _lib.COEF_MASS,
+         'coef_load': _lib.COEF_LOAD,
          'linear_elasticity': _lib.ELASTICITY, 'unit_load': _lib.UNIT_LOAD}
 
 
@@ -131,7 +132,7 @@ class DeviceBasis:
     # -- forms ---------------------------------------------------------------
     def form_struct(self, name, kappa=None, lam=0.0, mu=0.0):
         mode, kp = 0, None
-        if name == 'kappa_laplace':
+        if name in ('kappa_laplace', 'coef_mass', 'coef_load'):
             kp = kappa
             mode = 1 if kp.dim() == 1 else 2
         fs = _lib.Form(_FORM[name], mode, 0 if kp is None else kp.data_ptr(),
@@ -149,9 +150,9 @@ class DeviceBasis:
             _lib.ptr(G), _lib.ptr(out), _lib.stream_ptr()))
         return out
 
-    def local_linear(self, name='unit_load'):
+    def local_linear(self, name='unit_load', kappa=None):
         G = self.geometry()
-        fs, _ = self.form_struct(name)
+        fs, _ = self.form_struct(name, kappa)
         out = torch.empty(self.nbfun * self.E, dtype=torch.float64, device=self.p.device)
         _lib.check(self.lib.femhip_local_linear(
             ctypes.byref(self.geom), ctypes.byref(self.space), ctypes.byref(fs),
@@ -295,10 +296,33 @@ class DeviceBasis:
             _lib.ptr(out), _lib.stream_ptr()))
         return out
 
-    def assemble_linear(self, name='unit_load'):
-        contrib = self.local_linear(name)
+    def assemble_linear(self, name='unit_load', kappa=None):
+        contrib = self.local_linear(name, kappa)
         perm, seg = self.rows_plan()
         return self.segment_sum(contrib, perm, seg, self.N)
+
+    # -- default form parameters -------------------------------------------
+    def global_coordinates(self):
+        """(dim, E, Q) coordinates of the quadrature points."""
+        X = to_dev(self.basis.X, np.float64)
+        phi1 = None
+        if self.kind == 'hex':
+            from .element import ElementHex1
+            phi1 = to_dev(ElementHex1().tables(self.basis.X)[0], np.float64)
+        out = torch.empty((self.dim, self.E, self.Q), dtype=torch.float64, device=self.p.device)
+        _lib.check(self.lib.femhip_global_coordinates(
+            ctypes.byref(self.geom), _lib.ptr(X), _lib.ptr(phi1), _lib.ptr(out),
+            _lib.stream_ptr()))
+        torch.cuda.current_stream().synchronize()
+        return out
+
+    def mesh_parameters(self):
+        """(E, Q) mesh parameter |detDF| ** (1 / dim)."""
+        G = self.geometry()
+        out = torch.empty((self.E, self.Q), dtype=torch.float64, device=self.p.device)
+        _lib.check(self.lib.femhip_mesh_parameters(ctypes.byref(self.geom), _lib.ptr(G),
+                                                   _lib.ptr(out), _lib.stream_ptr()))
+        return out
 
 
 class FacetDeviceBasis(DeviceBasis):
@@ -349,27 +373,50 @@ class FacetDeviceBasis(DeviceBasis):
     def geometry(self):
         raise NotImplementedError("a facet basis maps its facets inside the facet kernels")
 
+    def _local(self, name, kappa, out):
+        if self.E == 0:
+            return out
+        mode = 0 if kappa is None else (1 if kappa.dim() == 1 else 2)
+        _lib.check(self.lib.femhip_facet_local(ctypes.byref(self.fgeom), _FORM[name],
+                                               _lib.ptr(kappa), mode, _lib.ptr(out),
+                                               _lib.stream_ptr()))
+        return out
+
     def local_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, out=None):
-        if name != 'mass':
-            raise NotImplementedError("facet bilinear forms: mass only")
+        if name not in ('mass', 'coef_mass'):
+            raise NotImplementedError("facet bilinear forms: mass and coef_mass")
         if out is None:
             out = torch.empty(self.nbfun * self.nbfun * self.E, dtype=torch.float64,
                               device=self.p.device)
-        if self.E == 0:
-            return out
-        _lib.check(self.lib.femhip_facet_local(ctypes.byref(self.fgeom), _FORM[name],
-                                               _lib.ptr(out), _lib.stream_ptr()))
-        return out
+        return self._local(name, kappa if name == 'coef_mass' else None, out)
 
-    def local_linear(self, name='unit_load'):
-        if name != 'unit_load':
-            raise NotImplementedError("facet linear forms: unit_load only")
+    def local_linear(self, name='unit_load', kappa=None):
+        if name not in ('unit_load', 'coef_load'):
+            raise NotImplementedError("facet linear forms: unit_load and coef_load")
         out = torch.empty(self.nbfun * self.E, dtype=torch.float64, device=self.p.device)
-        if self.E == 0:
-            return out
-        _lib.check(self.lib.femhip_facet_local(ctypes.byref(self.fgeom), _FORM[name],
-                                               _lib.ptr(out), _lib.stream_ptr()))
-        return out
+        return self._local(name, kappa if name == 'coef_load' else None, out)
+
+    def interpolate(self, w, grad=True):
+        """(value (nf, Q), grad (dim, nf, Q) or None) of a DOF vector
+        (device tensor) at the facet quadrature points."""
+        value = torch.empty((self.E, self.Q), dtype=torch.float64, device=self.p.device)
+        gr = (torch.empty((self.dim, self.E, self.Q), dtype=torch.float64, device=self.p.device)
+              if grad else None)
+        if self.E:
+            _lib.check(self.lib.femhip_facet_interpolate(
+                ctypes.byref(self.fgeom), _lib.ptr(self.dofs), _lib.ptr(w), _lib.ptr(value),
+                _lib.ptr(gr), _lib.stream_ptr()))
+        return value, gr
+
+    def global_coordinates(self):
+        """(x (dim, nf, Q), h (nf, Q)): facet quadrature points and the mesh
+        parameter."""
+        x = torch.empty((self.dim, self.E, self.Q), dtype=torch.float64, device=self.p.device)
+        h = torch.empty((self.E, self.Q), dtype=torch.float64, device=self.p.device)
+        if self.E:
+            _lib.check(self.lib.femhip_facet_global_coordinates(
+                ctypes.byref(self.fgeom), _lib.ptr(x), _lib.ptr(h), _lib.stream_ptr()))
+        return x, h
 
     def normals(self):
         """(dim, nfacets, Q) outward unit normals seen from ``f2t[0]``."""
@@ -393,18 +440,18 @@ class FacetDeviceBasis(DeviceBasis):
         """Exact path: local entries -> pattern (cached by zero signature)
         -> segmented sums."""
         if self.E == 0:
-            self.local_bilinear(name)          # form check only
+            self.local_bilinear(name, kappa)   # form check only
             dev = self.p.device
             return DeviceCSR(torch.zeros(self.N + 1, dtype=torch.int32, device=dev),
                              torch.zeros(0, dtype=torch.int32, device=dev),
                              torch.zeros(0, dtype=torch.float64, device=dev), (self.N, self.N))
-        contrib = self.local_bilinear(name)
+        contrib = self.local_bilinear(name, kappa)
         plan = self.plan_for(contrib)
         data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
         return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N))
 
-    def assemble_linear(self, name='unit_load'):
+    def assemble_linear(self, name='unit_load', kappa=None):
         if self.E == 0:
-            self.local_linear(name)
+            self.local_linear(name, kappa)
             return torch.zeros(self.N, dtype=torch.float64, device=self.p.device)
-        return super().assemble_linear(name)
+        return super().assemble_linear(name, kappa)

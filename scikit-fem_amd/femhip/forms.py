@@ -6,8 +6,11 @@ signatures (``skfem/assembly/form/form.py:28-121``,
 A form is accelerated when it is one of the built-in models of
 ``femhip.models`` (``laplace``, ``mass``, ``unit_load``, ``kappa_laplace``,
 ``linear_elasticity(Lambda, Mu)``), or a user form that the tracer in
-``femhip.trace`` recognises as one of them times a constant.  Anything else
-raises ``NotImplementedError``: there is no CPU fallback.
+``femhip.trace`` recognises as one of them, or as a coefficient times
+``u * v`` (``coef_mass``) or times ``v`` (``coef_load``), times a constant.
+On a ``FacetBasis`` the accelerated forms are ``mass``, ``unit_load``,
+``coef_mass`` and ``coef_load``.  Anything else raises
+``NotImplementedError``: there is no CPU fallback.
 """
 import numbers
 
@@ -119,7 +122,7 @@ class BilinearForm(Form):
             raise NotImplementedError("different trial and test bases")
         name, scale, params = self.spec()
         dev = ubasis.device()
-        if name == 'kappa_laplace':
+        if name in ('kappa_laplace', 'coef_mass'):
             kappa, token = _kappa_tensor({**self.params, **kwargs}, ubasis,
                                          params.get('coef', 'kappa'))
             A = dev.assemble_bilinear(name, kappa=kappa, token=token, path=path)
@@ -140,7 +143,7 @@ class BilinearForm(Form):
         name, scale, params = self.spec()
         dev = ubasis.device()
         kappa = None
-        if name == 'kappa_laplace':
+        if name in ('kappa_laplace', 'coef_mass'):
             kappa, _ = _kappa_tensor({**self.params, **kwargs}, ubasis,
                                      params.get('coef', 'kappa'))
         c = dev.local_bilinear(name, kappa=kappa, lam=params.get('Lambda', 0.0),
@@ -160,10 +163,14 @@ class LinearForm(Form):
         self._check_dtype()
         if vbasis is not None:
             raise ValueError("LinearForm takes one basis")
-        name, scale, _ = self.spec()
-        if name != 'unit_load':
-            raise NotImplementedError("only unit_load-type linear forms are accelerated")
-        b = basis.device().assemble_linear('unit_load')
+        name, scale, params = self.spec()
+        if name == 'coef_load':
+            kappa, _ = _kappa_tensor({**self.params, **kwargs}, basis, params['coef'])
+            b = basis.device().assemble_linear('coef_load', kappa=kappa)
+        elif name == 'unit_load':
+            b = basis.device().assemble_linear('unit_load')
+        else:
+            raise NotImplementedError("linear forms: unit_load and coef_load")
         if scale != 1.0:
             b = b * scale
         return b

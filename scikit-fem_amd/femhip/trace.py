@@ -7,7 +7,9 @@ resulting expression is matched against the built-in kernels:
 * ``dot(grad(u), grad(v))``                 -> laplace
 * ``u * v``                                 -> mass
 * ``dot(w.k * grad(u), grad(v))``           -> kappa_laplace (coefficient k)
+* ``w.c * u * v`` (factors in any order)    -> coef_mass (coefficient c)
 * ``v``                                     -> unit_load
+* ``w.c * v``                               -> coef_load (coefficient c)
 
 each possibly times numeric constants (the result is scaled; constant
 factors do not change which entries are exactly zero).  Built-in models
@@ -142,13 +144,21 @@ def identify(form, nargs):
             % (getattr(form, '__name__', form), exc))
     expr = wrap(expr)
     c, fs = _split(expr)
+    fields = [f for f in fs if isinstance(f, Field)]
+    coefs = [f for f in fs if isinstance(f, Coef)]
     if nargs == 2:
         if len(fs) == 1 and isinstance(fs[0], Field) and fs[0].which == 'v':
             return ('unit_load', c, {})
+        if (len(fs) == 2 and len(fields) == 1 and fields[0].which == 'v'
+                and len(coefs) == 1):
+            return ('coef_load', c, {'coef': coefs[0].name})
     else:
         if (len(fs) == 2 and all(isinstance(f, Field) for f in fs)
                 and {f.which for f in fs} == {'u', 'v'}):
             return ('mass', c, {})
+        if (len(fs) == 3 and len(fields) == 2 and len(coefs) == 1
+                and sorted(f.which for f in fields) == ['u', 'v']):
+            return ('coef_mass', c, {'coef': coefs[0].name})
         if len(fs) == 1 and isinstance(fs[0], Dot):
             ca, fa = _split(fs[0].a)
             cb, fb = _split(fs[0].b)

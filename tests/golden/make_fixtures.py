@@ -19,7 +19,13 @@ What is written (all under tests/golden/):
 * ``facet_*.npz`` -- boundary forms on a ``FacetBasis``: the facets and
   their owning elements, the facet quadrature, dx, the outward normals, the
   CSR facet mass matrix and the facet load vector (default boundary facets,
-  and a set of interior facets seen from either side).
+  and a set of interior facets seen from either side); the facet points and
+  mesh parameter, a Robin matrix ``alpha * u * v`` and a Neumann vector
+  ``g * v`` with data computed at the facet points, a Neumann vector whose
+  data is a DOF vector (interpolated by the reference), and that
+  interpolation (value and gradient).
+* ``coef_*.npz`` -- the same coefficient forms on a ``CellBasis`` (the
+  quadrature points and mesh parameter, ``alpha * u * v``, ``g * v``).
 * ``grad_kappa65.npz`` (``--big``) -- the reference gradient of
   L = 1/2 u^T A(kappa, X) u - f^T u on the 65^3 tet mesh: dL/dkappa_e from
   single-element assemblies (unit coefficient) and dL/dX from central
@@ -51,6 +57,23 @@ from skfem.models.elasticity import linear_elasticity, lame_parameters  # noqa: 
 @BilinearForm
 def kappa_laplace(u, v, w):
     return dot(w['kappa'] * grad(u), grad(v))
+
+
+@BilinearForm
+def robin(u, v, w):
+    return w['alpha'] * u * v
+
+
+@LinearForm
+def neumann(v, w):
+    return w['g'] * v
+
+
+def coef_data(x):
+    """Robin and Neumann data at points x (dim, n, Q)."""
+    alpha = 1.0 + x[0] ** 2 + 0.5 * x[1]
+    g = np.sin(3.0 * x[0]) + x[1] * x[-1]
+    return alpha, g
 
 
 def jitter(mesh, a, h, seed=0):
@@ -183,16 +206,52 @@ def write_facet():
         fb = FacetBasis(m, ELEMS[elem](), facets=facets, side=side)
         A = mass.assemble(fb).tocsr()
         f = unit_load.assemble(fb)
+        x = np.asarray(fb.global_coordinates())
+        alpha, g = coef_data(x)
+        R = robin.assemble(fb, alpha=alpha).tocsr()
+        wdof = 1.0 + 0.5 * np.cos(np.arange(fb.N, dtype=np.float64))
+        ip = fb.interpolate(wdof)
         out = dict(p=m.p, t=m.t, find=fb.find, tind=fb.tind,
                    element_dofs=fb.element_dofs, N=np.array(fb.N), X=fb.X, W=fb.W,
                    dx=fb.dx, normals=fb.normals.value,
                    mass_indptr=A.indptr, mass_indices=A.indices, mass_data=A.data,
-                   unit_load_b=f)
+                   unit_load_b=f, x=x, h=np.asarray(fb.mesh_parameters()),
+                   alpha=alpha, g=g, robin_indptr=R.indptr, robin_indices=R.indices,
+                   robin_data=R.data, neumann_b=neumann.assemble(fb, g=g),
+                   wdof=wdof, neumann_dof_b=neumann.assemble(fb, g=wdof),
+                   interp_value=np.asarray(ip), interp_grad=ip.grad)
         meta = dict(kind=kind, n=n, jitter=a, element=elem,
                     facets='boundary' if sel is None else sel, side=side)
         np.savez_compressed(os.path.join(HERE, 'facet_%s.npz' % name),
                             meta=json.dumps(meta), **out)
         print('wrote facet', name, len(fb.find), 'facets, nnz', A.nnz)
+
+
+COEF = [
+    # name, mesh kind, n, jitter a, element
+    ('tri2_n9j', 'tri', 9, 0.2, 'tri2'),
+    ('tet1_n7j', 'tet', 7, 0.2, 'tet1'),
+    ('tet2_n5j', 'tet', 5, 0.2, 'tet2'),
+    ('hex1_n5j', 'hex', 5, 0.15, 'hex1'),
+]
+
+
+def write_coef():
+    for name, kind, n, a, elem in COEF:
+        m = mk_mesh(kind, n, a)
+        b = make_basis(m, elem)
+        x = np.asarray(b.global_coordinates())
+        alpha, g = coef_data(x)
+        R = robin.assemble(b, alpha=alpha).tocsr()
+        wdof = 1.0 + 0.5 * np.cos(np.arange(b.N, dtype=np.float64))
+        out = dict(p=m.p, t=m.t, N=np.array(b.N), x=x, h=np.asarray(b.mesh_parameters()),
+                   alpha=alpha, g=g, robin_indptr=R.indptr, robin_indices=R.indices,
+                   robin_data=R.data, neumann_b=neumann.assemble(b, g=g),
+                   wdof=wdof, neumann_dof_b=neumann.assemble(b, g=wdof))
+        meta = dict(kind=kind, n=n, jitter=a, element=elem)
+        np.savez_compressed(os.path.join(HERE, 'coef_%s.npz' % name),
+                            meta=json.dumps(meta), **out)
+        print('wrote coef', name, R.nnz)
 
 
 def write_config1():
@@ -304,6 +363,8 @@ if __name__ == '__main__':
         write_config1()
     if args.only in (None, 'facet'):
         write_facet()
+    if args.only in (None, 'coef'):
+        write_coef()
     if args.big:
         big = {
             'tet1_n101j': ('tet', 101, 0.2, 'tet1', False, None,

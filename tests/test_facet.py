@@ -158,3 +158,43 @@ def test_host_facet_selection():
         femhip.FacetBasis(m, femhip.ElementTriP1(), side=1)   # boundary: no side 1
     with pytest.raises(NotImplementedError):
         femhip.FacetBasis(femhip.MeshHex.init_tensor(x, x, x), femhip.ElementHex1())
+
+
+@pytest.mark.parametrize('fn', FIXTURES, ids=lambda f: os.path.basename(f)[6:-4])
+def test_oracle_coefficient_forms_match_reference(fn):
+    """Facet points, mesh parameter, Robin matrix alpha*u*v, Neumann vector
+    g*v, a DOF-vector coefficient (interpolated on the facets) and the
+    interpolation itself: bit-identical to the reference's."""
+    z, meta = _load(fn)
+    fb = _ofacet(z, meta)
+    assert np.array_equal(fb.x, z['x']) and np.array_equal(fb.h, z['h'])
+    R = O.bilinear_assemble(O.form_coef_mass, fb, c=z['alpha'])
+    for k in ('indptr', 'indices', 'data'):
+        assert np.array_equal(getattr(R, k), z['robin_' + k]), k
+    assert np.array_equal(O.linear_assemble(O.form_coef_load, fb, c=z['g']), z['neumann_b'])
+    v, g = O.interpolate(fb, z['wdof'])
+    assert np.array_equal(v, z['interp_value']) and np.array_equal(g, z['interp_grad'])
+    assert np.array_equal(O.linear_assemble(O.form_coef_load, fb, c=v), z['neumann_dof_b'])
+
+
+@pytest.mark.parametrize('kind,elem,n', [('tri', 'tri1', 9), ('tri', 'tri2', 5),
+                                         ('tet', 'tet1', 4), ('tet', 'tet2', 3)])
+def test_oracle_robin_closed_form(kind, elem, n):
+    """1^T R 1 = int alpha ds for alpha = 1 + x^2 + y/2 on the unit square /
+    cube boundary (exact: the facet rules integrate quadratics), and the
+    facet points lie on their facets' planes."""
+    x = np.linspace(0, 1, n)
+    m = (O.MeshTri.init_tensor(x, x) if kind == 'tri' else O.MeshTet.init_tensor(x, x, x))
+    m = O.jitter(m, 0.2, 1.0 / (n - 1))
+    fb = O.FacetBasis(m, OELEM[elem]())
+    alpha = 1.0 + fb.x[0] ** 2 + 0.5 * fb.x[1]
+    R = O.bilinear_assemble(O.form_coef_mass, fb, c=alpha)
+    one = np.ones(fb.N)
+    if kind == 'tri':
+        exact = 1.25 + 2.25 + 4.0 / 3.0 + 11.0 / 6.0
+    else:
+        # faces x=0, x=1 (alpha in y), y=0, y=1 (in x), z=0, z=1 (in x, y)
+        exact = 1.25 + 2.25 + 4.0 / 3.0 + 11.0 / 6.0 + 2 * (1.0 + 1.0 / 3.0 + 0.25)
+    assert abs(one @ R @ one - exact) < 1e-13
+    on_plane = np.min(np.abs(np.stack([fb.x, 1.0 - fb.x])), axis=(0, 1))
+    assert on_plane.max() < 1e-15
