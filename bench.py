@@ -310,6 +310,46 @@ def bench_facet(m):
             "load_ms": load_ms, "max_rel_err_vs_first_call": max(err, errb)}
 
 
+def bench_p2_tet(n=41):
+    """Laplace on ElementTetP2, MeshTet 41^3 jittered (384,000 tets,
+    531,441 DoF, 11 points at intorder 4): the exact path's steady state
+    (local entries in the reference's arithmetic + segmented sums over the
+    cached pattern).  Bytes: the local kernel writes 8 B x 100 entries per
+    element (and reads the mapping), the sums read 12 B per contribution
+    and write 8 B per slot."""
+    import torch
+    import femhip
+    from femhip.models import laplace
+    x = np.linspace(0, 1, n)
+    m = femhip.MeshTet.init_tensor(x, x, x).jittered(0.2, 1.0 / (n - 1))
+    b = femhip.Basis(m, femhip.ElementTetP2(), intorder=4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A = laplace.assemble_device(b)
+    torch.cuda.synchronize()
+    first = (time.perf_counter() - t0) * 1e3
+    d = b.device()
+    plan = next(iter(d.plans.values()))
+    E, nb = d.E, d.nbfun
+    contrib = torch.empty(nb * nb * E, dtype=torch.float64, device='cuda')
+    out = torch.empty(plan.nnz, dtype=torch.float64, device='cuda')
+    stream = torch.cuda.current_stream()
+    d.local_bilinear('laplace', out=contrib)
+    local_ms = time_stream(lambda: d.local_bilinear('laplace', out=contrib), 10, stream)
+    sum_ms = time_stream(lambda: d.segment_sum(contrib, plan.perm, plan.seg, plan.nnz, out=out),
+                         10, stream)
+    torch.cuda.synchronize()
+    err = float((out - A.data).abs().max() / A.data.abs().max())
+    C = nb * nb * E
+    return {"elements": E, "dofs": b.N, "nnz": plan.nnz, "points": d.Q,
+            "first_call_ms": first, "local_ms": local_ms, "segment_sum_ms": sum_ms,
+            "assembly_ms": local_ms + sum_ms,
+            "elements_per_s": E / ((local_ms + sum_ms) * 1e-3),
+            "local_GBps_written": 8 * C / (local_ms * 1e-3) / 1e9,
+            "sum_GBps": (12 * C + 8 * plan.nnz) / (sum_ms * 1e-3) / 1e9,
+            "max_rel_err_vs_first_call": err}
+
+
 def bench_elasticity(n=41):
     """linear_elasticity on ElementVector(ElementHex2), 27 points, MeshHex
     (n-1)^3 jittered 0.15h: MFMA local matrices + segmented sums."""
@@ -365,7 +405,7 @@ def main():
     ap.add_argument('--extra', default=None,
                     help='comma-separated subset of the extra measurements '
                          '(interpolate, facet, device_solve, batched_kappa, '
-                         'mesh_batch, adjoint, elasticity)')
+                         'mesh_batch, adjoint, elasticity, p2_tet)')
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -482,6 +522,7 @@ def main():
             ('mesh_batch', "mesh_batch_64_p1tri", bench_mesh_batch),
             ('adjoint', "adjoint_kappa_tet65", bench_adjoint),
             ('elasticity', "elasticity_vhex2_40", bench_elasticity),
+            ('p2_tet', "laplace_p2tet41", bench_p2_tet),
         )
         chosen = None if args.extra is None else set(args.extra.split(','))
         for short, name, fn in extras:

@@ -45,8 +45,6 @@ struct MapAt {
   }
 };
 
-// grad[j] = sum_i invDF[i, j] * dphi[i]   (einsum 'ijkl,il->jkl',
-// element_h1.py:17), accumulated from 0 in index order.
 // MappingAffine._init_Ab/_init_invA (mapping_affine.py:55-131) for element e:
 // A[:, j] = p[t[j+1, e]] - p[t[0, e]], the determinant by cofactor expansion
 // along the first row and every entry of invA as the reference writes it.
@@ -89,6 +87,8 @@ __device__ __forceinline__ double affine_inverse(const double* __restrict__ p,
   }
 }
 
+// grad[j] = sum_i invDF[i, j] * dphi[i]   (einsum 'ijkl,il->jkl',
+// element_h1.py:17), accumulated from 0 in index order.
 template <int D>
 __device__ __forceinline__ void global_grad(const double (&inv)[D][D],
                                             const double* __restrict__ dphi,
@@ -195,6 +195,66 @@ __global__ void __launch_bounds__(kBlock) local_bilinear_kernel(LocalArgs a) {
   a.out[(int64_t)entry * E + e] = r;
 }
 
+// Affine scalar forms, one thread per (element, test function i): the
+// element's mapping and the test function's values or gradients at every
+// point are formed once for the row, and each entry (j, i) is the same sum
+// as in local_bilinear_kernel.  laplace and mass integrands are symmetric in
+// (u, v) bit for bit, so only j >= i is computed and stored at both (j, i)
+// and (i, j); kappa_laplace ((k grad u) . grad v) and coef_mass ((c u) v)
+// are not, and compute the whole row.
+template <int D, int Q, int FK>
+__global__ void __launch_bounds__(kBlock) local_row_kernel(LocalArgs a) {
+#pragma clang fp contract(off)
+  constexpr bool GRAD = FK == FEMHIP_LAPLACE || FK == FEMHIP_KAPPA_LAPLACE;
+  constexpr bool SYM = FK == FEMHIP_LAPLACE || FK == FEMHIP_MASS;
+  constexpr bool COEF = FK == FEMHIP_KAPPA_LAPLACE || FK == FEMHIP_COEF_MASS;
+  const int64_t E = a.E;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= E) return;
+  const int i = blockIdx.y;
+  const int nb = a.nb;
+  MapAt<D, true, Q> m;
+  m.load(a.G, E, e, 0);
+  double vi[Q][GRAD ? D : 1];
+  double k[COEF ? Q : 1];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    if constexpr (GRAD)
+      global_grad<D>(m.inv, a.dphi, i, q, Q, vi[q]);
+    else
+      vi[q][0] = a.phi[i * Q + q];
+    if constexpr (COEF) k[q] = (a.kappa_mode == 1) ? a.kappa[e] : a.kappa[e * Q + q];
+  }
+  for (int j = SYM ? i : 0; j < nb; ++j) {
+    auto term = [&](int q) -> double {
+#pragma clang fp contract(off)
+      const double dx = m.adet * a.W[q];
+      double val;
+      if constexpr (GRAD) {
+        double gj[D];
+        global_grad<D>(m.inv, a.dphi, j, q, Q, gj);
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          if constexpr (FK == FEMHIP_KAPPA_LAPLACE)
+            s = s + (k[q] * gj[d]) * vi[q][d];
+          else
+            s = s + gj[d] * vi[q][d];
+        }
+        val = s;
+      } else if constexpr (FK == FEMHIP_COEF_MASS) {
+        val = (k[q] * a.phi[j * Q + q]) * vi[q][0];
+      } else {
+        val = a.phi[j * Q + q] * vi[q][0];
+      }
+      return val * dx;
+    };
+    const double r = 0.0 + np_pairwise<0, Q>(term);
+    a.out[(int64_t)(j * nb + i) * E + e] = r;
+    if (SYM && j != i) a.out[(int64_t)(i * nb + j) * E + e] = r;
+  }
+}
+
 template <int D, bool AFF, int Q, bool COEF>
 __global__ void __launch_bounds__(kBlock) local_load_kernel(LocalArgs a) {
 #pragma clang fp contract(off)
@@ -222,6 +282,31 @@ __global__ void __launch_bounds__(kBlock) local_load_kernel(LocalArgs a) {
 // ---------------------------------------------------------------------------
 template <int D, bool AFF, int Q>
 int launch_bilinear_q(const LocalArgs& a, int fk, bool vec, hipStream_t s) {
+  if constexpr (AFF) {
+    // scalar forms on simplices: one thread per (element, row)
+    if (!vec && fk != FEMHIP_ELASTICITY) {
+      const dim3 rgrid((unsigned)ceil_div(a.E, kBlock), (unsigned)a.nb);
+      switch (fk) {
+        case FEMHIP_LAPLACE:
+          local_row_kernel<D, Q, FEMHIP_LAPLACE><<<rgrid, kBlock, 0, s>>>(a);
+          break;
+        case FEMHIP_MASS:
+          local_row_kernel<D, Q, FEMHIP_MASS><<<rgrid, kBlock, 0, s>>>(a);
+          break;
+        case FEMHIP_KAPPA_LAPLACE:
+          local_row_kernel<D, Q, FEMHIP_KAPPA_LAPLACE><<<rgrid, kBlock, 0, s>>>(a);
+          break;
+        case FEMHIP_COEF_MASS:
+          local_row_kernel<D, Q, FEMHIP_COEF_MASS><<<rgrid, kBlock, 0, s>>>(a);
+          break;
+        default:
+          set_error("scalar elements support laplace, mass, kappa_laplace and coef_mass");
+          return FEMHIP_EUNSUPPORTED;
+      }
+      FEMHIP_CHECK_LAUNCH();
+      return FEMHIP_OK;
+    }
+  }
   dim3 grid((unsigned)ceil_div(a.E, kBlock), (unsigned)(a.nb * a.nb));
   if (vec) {
     FEMHIP_REQUIRE(fk == FEMHIP_ELASTICITY, FEMHIP_EUNSUPPORTED,
