@@ -184,6 +184,17 @@ int femhip_global_coordinates(const femhip_geom_t* g, const double* X, const dou
 int femhip_mesh_parameters(const femhip_geom_t* g, const double* G, double* h,
                            femhip_stream_t s);
 
+/* A scalar CSR matrix (n rows, nnz entries) expanded to the components of
+ * ElementVector (element_vector.py:36-48): row dim*r + c holds scalar row r
+ * with columns dim*s + c and the same values (data may be NULL for the
+ * pattern alone).  The assembly of vector_laplace (models/poisson.py:12-14)
+ * on ElementVector(e) is the expansion of laplace on e: the same values and
+ * the same exact zeros.  out: indptr[dim*n + 1], indices/data[dim*nnz]. */
+int femhip_csr_expand_components(const int32_t* indptr, const int32_t* indices,
+                                 const double* data, int64_t n, int64_t nnz, int32_t dim,
+                                 int32_t* out_indptr, int32_t* out_indices, double* out_data,
+                                 femhip_stream_t s);
+
 /* Basis.interpolate (abstract_basis.py:271-322) of a DOF vector w[ndofs]
  * for a scalar element: value[nelems][nq] and, when grad != NULL,
  * grad[dim][nelems][nq] (the global gradients of ElementH1.gbasis,

@@ -869,6 +869,10 @@ def form_mass(u, v, w):
     return u[0] * v[0]                           # models/poisson.py:17-19
 
 
+def form_vector_laplace(u, v, w):
+    return ddot(u[1], v[1])                      # models/poisson.py:12-14
+
+
 def form_kappa_laplace(u, v, w):
     return dot(w['kappa'] * u[1], v[1])
 
@@ -1032,6 +1036,8 @@ def assemble_named(name, basis, **kw):
         return bilinear_assemble(form_laplace, basis)
     if name == 'mass':
         return bilinear_assemble(form_mass, basis)
+    if name == 'vector_laplace':
+        return bilinear_assemble(form_vector_laplace, basis)
     if name == 'kappa_laplace':
         return bilinear_assemble(form_kappa_laplace, basis,
                                  kappa=kw['kappa'])

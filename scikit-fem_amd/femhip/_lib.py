@@ -134,6 +134,7 @@ SIGNATURES = {
     'femhip_facet_interpolate': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p, _p]),
     'femhip_facet_global_coordinates': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p]),
     'femhip_global_coordinates': (ctypes.c_int, [ctypes.POINTER(Geom), _p, _p, _p, _p]),
+    'femhip_csr_expand_components': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _i32, _p, _p, _p, _p]),
     'femhip_mesh_parameters': (ctypes.c_int, [ctypes.POINTER(Geom), _p, _p, _p]),
     'femhip_facet_normals': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p]),
     'femhip_interpolate': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space), _p, _p,

@@ -121,6 +121,11 @@ class BilinearForm(Form):
         if vbasis is not None and vbasis is not ubasis:
             raise NotImplementedError("different trial and test bases")
         name, scale, params = self.spec()
+        if name == 'vector_laplace':
+            A = _vector_laplace(ubasis, path)
+            if scale != 1.0:
+                A.data = A.data * scale
+            return A
         dev = ubasis.device()
         if name in ('kappa_laplace', 'coef_mass'):
             kappa, token = _kappa_tensor({**self.params, **kwargs}, ubasis,
@@ -152,6 +157,34 @@ class BilinearForm(Form):
         loc = c.reshape(nb, nb, ubasis.nelems).permute(2, 1, 0)  # (e, i, j)
         loc = loc.cpu().numpy() * scale
         return loc
+
+
+def _vector_laplace(basis, path):
+    """``vector_laplace`` (``models/poisson.py:12-14``) on ElementVector(e):
+    ``laplace`` on e (fused for P1 simplices) expanded to the components --
+    the reference's ``ddot(grad u, grad v)`` adds the scalar dot product's
+    terms in the same order for equal components and exact zeros otherwise,
+    so the values and the pruned pattern are the scalar ones, repeated."""
+    import ctypes
+    from . import _lib
+    from .device import DeviceCSR
+    if not basis.elem.vector or getattr(basis, 'find', None) is not None:
+        raise NotImplementedError("vector_laplace on a CellBasis of ElementVector")
+    sb = getattr(basis, '_scalar_companion', None)
+    if sb is None:
+        sb = basis.with_element(basis.elem.elem)
+        basis._scalar_companion = sb
+    S = sb.device().assemble_bilinear('laplace', path=path)
+    dim = basis.elem.dim
+    n, nnz = S.shape[0], S.nnz
+    dev = S.data.device
+    indptr = torch.empty(dim * n + 1, dtype=torch.int32, device=dev)
+    indices = torch.empty(max(dim * nnz, 1), dtype=torch.int32, device=dev)[:dim * nnz]
+    data = torch.empty(max(dim * nnz, 1), dtype=torch.float64, device=dev)[:dim * nnz]
+    _lib.check(_lib.load().femhip_csr_expand_components(
+        _lib.ptr(S.indptr), _lib.ptr(S.indices), _lib.ptr(S.data), n, nnz, dim,
+        _lib.ptr(indptr), _lib.ptr(indices), _lib.ptr(data), _lib.stream_ptr()))
+    return DeviceCSR(indptr, indices, data, (dim * n, dim * n))
 
 
 class LinearForm(Form):

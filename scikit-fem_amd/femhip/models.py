@@ -14,6 +14,7 @@ def _tagged(name, nargs, params=None, doc=""):
 
 laplace = BilinearForm(_tagged('laplace', 3, doc="dot(grad(u), grad(v))"))
 mass = BilinearForm(_tagged('mass', 3, doc="u * v"))
+vector_laplace = BilinearForm(_tagged('vector_laplace', 3, doc="ddot(grad(u), grad(v))"))
 unit_load = LinearForm(_tagged('unit_load', 2, doc="v"))
 kappa_laplace = BilinearForm(_tagged('kappa_laplace', 3, {'coef': 'kappa'},
                                      doc="dot(w['kappa'] * grad(u), grad(v))"))

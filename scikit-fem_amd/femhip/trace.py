@@ -7,6 +7,7 @@ resulting expression is matched against the built-in kernels:
 * ``dot(grad(u), grad(v))``                 -> laplace
 * ``u * v``                                 -> mass
 * ``dot(w.k * grad(u), grad(v))``           -> kappa_laplace (coefficient k)
+* ``ddot(grad(u), grad(v))``                -> vector_laplace
 * ``w.c * u * v`` (factors in any order)    -> coef_mass (coefficient c)
 * ``v``                                     -> unit_load
 * ``w.c * v``                               -> coef_load (coefficient c)
@@ -79,6 +80,11 @@ class Mul(Expr):
 
 
 class Dot(Expr):
+    def __init__(self, a, b):
+        self.a, self.b = a, b
+
+
+class DDot(Expr):
     def __init__(self, a, b):
         self.a, self.b = a, b
 
@@ -159,6 +165,9 @@ def identify(form, nargs):
         if (len(fs) == 3 and len(fields) == 2 and len(coefs) == 1
                 and sorted(f.which for f in fields) == ['u', 'v']):
             return ('coef_mass', c, {'coef': coefs[0].name})
+        if (len(fs) == 1 and isinstance(fs[0], DDot) and isinstance(fs[0].a, Grad)
+                and isinstance(fs[0].b, Grad) and {fs[0].a.f.which, fs[0].b.f.which} == {'u', 'v'}):
+            return ('vector_laplace', c, {})
         if len(fs) == 1 and isinstance(fs[0], Dot):
             ca, fa = _split(fs[0].a)
             cb, fb = _split(fs[0].b)

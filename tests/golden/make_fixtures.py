@@ -50,7 +50,7 @@ from skfem import (MeshTri, MeshTet, MeshHex, ElementTriP1, ElementTriP2,  # noq
                    ElementTetP1, ElementTetP2, ElementHex1, ElementHex2,
                    ElementVector, Basis, CellBasis, BilinearForm, LinearForm)
 from skfem.helpers import dot, grad  # noqa: E402
-from skfem.models.poisson import laplace, mass, unit_load  # noqa: E402
+from skfem.models.poisson import laplace, mass, unit_load, vector_laplace  # noqa: E402
 from skfem.models.elasticity import linear_elasticity, lame_parameters  # noqa: E402
 
 
@@ -125,6 +125,8 @@ def assemble(form, basis, kappa_e=None):
         return kappa_laplace.assemble(basis, kappa=kq)
     if form == 'linear_elasticity':
         return linear_elasticity(lam, mu).assemble(basis)
+    if form == 'vector_laplace':
+        return vector_laplace.assemble(basis)
     raise KeyError(form)
 
 
@@ -150,8 +152,8 @@ SMALL = [
     ('hex2_n3', 'hex', 3, 0.0, 'hex2', False, 5, ['laplace']),
     ('vhex2_n4j', 'hex', 4, 0.15, 'hex2', True, 5, ['linear_elasticity']),
     ('vhex2_n3', 'hex', 3, 0.0, 'hex2', True, 5, ['linear_elasticity']),
-    ('vtet1_n5j', 'tet', 5, 0.2, 'tet1', True, None, ['linear_elasticity']),
-    ('vtri2_n5j', 'tri', 5, 0.2, 'tri2', True, None, ['linear_elasticity']),
+    ('vtet1_n5j', 'tet', 5, 0.2, 'tet1', True, None, ['linear_elasticity', 'vector_laplace']),
+    ('vtri2_n5j', 'tri', 5, 0.2, 'tri2', True, None, ['linear_elasticity', 'vector_laplace']),
 ]
 
 

@@ -9,7 +9,7 @@ import pytest
 
 import femhip
 from femhip.models import (laplace, mass, unit_load, kappa_laplace,
-                           linear_elasticity, lame_parameters)
+                           linear_elasticity, lame_parameters, vector_laplace)
 from oracle import skfem_np as O
 
 GOLD = os.path.join(os.path.dirname(__file__), 'golden')
@@ -48,6 +48,8 @@ def _assemble(form, b, z):
     if form == 'linear_elasticity':
         lam, mu = lame_parameters(1.0, 0.3)
         return linear_elasticity(lam, mu).assemble(b)
+    if form == 'vector_laplace':
+        return vector_laplace.assemble(b)
     raise KeyError(form)
 
 
@@ -184,3 +186,25 @@ def test_enforce_on_device():
     torch.cuda.synchronize()
     assert np.array_equal(A.data.cpu().numpy(), Ah.data)
     assert np.array_equal(f.cpu().numpy(), fh)
+
+
+@pytest.mark.parametrize('kind,elem,n', [('tet', 'tet1', 13), ('tri', 'tri2', 17),
+                                         ('hex', 'hex1', 6)])
+def test_vector_laplace_vs_oracle(kind, elem, n):
+    """vector_laplace on ElementVector(e), larger jittered meshes, first and
+    second call (P1: the fused kernel for the scalar matrix), and a user form
+    ddot(grad(u), grad(v)) recognised by the tracer."""
+    from femhip.helpers import ddot, grad
+    print('case', kind, elem, n, flush=True)
+    x = np.linspace(0, 1, n)
+    M = MESH[kind]
+    m = (M.init_tensor(x, x) if kind == 'tri' else M.init_tensor(x, x, x)).jittered(
+        0.15, 1.0 / (n - 1))
+    b = femhip.Basis(m, femhip.ElementVector(ELEM[elem]()))
+    om = {'tri': O.MeshTri, 'tet': O.MeshTet, 'hex': O.MeshHex}[kind](m.p, m.t)
+    oe = {'tet1': O.ElementTetP1, 'tri2': O.ElementTriP2, 'hex1': O.ElementHex1}[elem]()
+    R = O.assemble_named('vector_laplace', O.CellBasis(om, O.ElementVector(oe)))
+    user = femhip.BilinearForm(lambda u, v, w: ddot(grad(u), grad(v)))
+    for A in (vector_laplace.assemble(b), vector_laplace.assemble(b), user.assemble(b)):
+        assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
+        assert np.abs(A.data - R.data).max() <= TOL * np.abs(R.data).max()

@@ -423,3 +423,31 @@ def test_threaded_assembly_identical():
     A2 = O.bilinear_assemble(O.form_laplace, b, nthreads=3)
     assert np.array_equal(A1.indptr, A2.indptr) and np.array_equal(A1.indices, A2.indices)
     assert np.array_equal(A1.data, A2.data)
+
+
+@pytest.mark.parametrize('name', ['small_vtet1_n5j', 'small_vtri2_n5j'])
+def test_vector_laplace_is_expanded_scalar_laplace(name):
+    """The reference's vector_laplace on ElementVector(e) equals laplace on e
+    expanded to the components (row dim*r + c = scalar row r, columns
+    dim*s + c), bit for bit: the relation the device path relies on."""
+    z = np.load(os.path.join(GOLD, name + '.npz'))
+    meta = json.loads(str(z['meta']))
+    M = {'tri': O.MeshTri, 'tet': O.MeshTet}[meta['kind']]
+    E = {'tet1': O.ElementTetP1, 'tri2': O.ElementTriP2}[meta['element']]
+    m = M(z['p'], z['t'])
+    S = O.assemble_named('laplace', O.CellBasis(m, E()))
+    dim = m.p.shape[0]
+    n = S.shape[0]
+    rows, cols, vals = [], [], []
+    for c in range(dim):
+        coo = S.tocoo()
+        rows.append(dim * coo.row + c)
+        cols.append(dim * coo.col + c)
+        vals.append(coo.data)
+    from scipy.sparse import csr_matrix
+    V = csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                   shape=(dim * n, dim * n))
+    V.sort_indices()
+    assert np.array_equal(V.indptr, z['vector_laplace_indptr'])
+    assert np.array_equal(V.indices, z['vector_laplace_indices'])
+    assert np.array_equal(V.data, z['vector_laplace_data'])
