@@ -41,7 +41,7 @@ def compile_one(src):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + '.o')
     cmd = [HIPCC] + COMMON + PER_FILE.get(src, []) + ['-c', os.path.join(HERE, src), '-o', obj]
     if src.endswith('.cpp'):
-        cmd = ['g++', '-O3', '-std=c++17', '-fPIC', '-Wall', '-c',
+        cmd = ['g++', '-O3', '-std=c++17', '-fPIC', '-Wall', '-pthread', '-c',
                os.path.join(HERE, src), '-o', obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -55,7 +55,7 @@ def build(jobs=8):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, srcs))
     tmp = OUT + '.tmp'
-    cmd = [HIPCC, '-shared', '-fPIC', '--offload-arch=gfx950', '-o', tmp] + objs
+    cmd = [HIPCC, '-shared', '-fPIC', '--offload-arch=gfx950', '-pthread', '-o', tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError('link failed: %s\n%s' % (r.stdout, r.stderr))

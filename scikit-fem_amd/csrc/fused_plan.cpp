@@ -24,7 +24,14 @@
 // row that lost entries to eliminate_zeros has one spare slot after its CSR
 // slots; its pruned entries (a pruned diagonal too) carry the offset
 // len = the spare slot, which is never written out.
+//
+// The blocks are built independently by a pool of host threads
+// (FEMHIP_PLAN_THREADS, default: the CPUs available) and concatenated in
+// block order: the plan is the same for any number of threads.
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <thread>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -48,12 +55,23 @@ struct Plan {
   std::vector<int32_t> rec_elem; // global element id per record
 };
 
-// Writes `nbits` of `v` at bit position `pos` of a little-endian word array.
+// Writes `nbits` (<= 32) of `v` at bit position `pos` of a little-endian
+// word array (the field may straddle two words).
 void put_bits(uint32_t* w, int pos, int nbits, uint32_t v) {
-  for (int i = 0; i < nbits; ++i) {
-    const int bit = pos + i;
-    if ((v >> i) & 1u) w[bit >> 5] |= (1u << (bit & 31));
-  }
+  const uint64_t x = ((uint64_t)v & ((1ull << nbits) - 1ull)) << (pos & 31);
+  w[pos >> 5] |= (uint32_t)x;
+  if ((pos & 31) + nbits > 32) w[(pos >> 5) + 1] |= (uint32_t)(x >> 32);
+}
+
+// Threads for the per-block phase: FEMHIP_PLAN_THREADS, else the CPUs this
+// process may run on (at most 32), at most one per block.
+int plan_threads(int64_t nblocks) {
+  int n = 0;
+  if (const char* e = std::getenv("FEMHIP_PLAN_THREADS")) n = std::atoi(e);
+  if (n <= 0) n = (int)std::thread::hardware_concurrency();
+  if (n <= 0) n = 1;
+  n = std::min(n, 32);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(n, nblocks));
 }
 
 }  // namespace
@@ -182,19 +200,31 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
     }
   }
 
-  Plan* P = new Plan();
-  std::memset(&P->info, 0, sizeof(P->info));
-  std::vector<int32_t> nstamp(Nv, -1), lid(Nv, -1), estamp(E, -1);
-  std::vector<int32_t> owned, halo, elems;
-  std::vector<uint64_t> used;  // per owned local row: colours used
-  std::vector<int32_t> color;
-  int32_t pid = 0;
-  int max_slots = 0, max_nodes = 0, max_rows = 0, max_colors = 0;
-  int64_t visits = 0;
-
-  for (size_t leaf = 0; leaf < leaves.size(); ++leaf) {
+  // 2. every leaf block on its own (per-thread scratch), in parallel; the
+  // blocks are then concatenated in leaf order, so the plan does not depend
+  // on the number of threads
+  struct LeafOut {
+    std::vector<int32_t> nodemap, rows, chunks, rec_elem;  // chunks: leaf-relative rec0
+    std::vector<uint32_t> recs;
+    int L = 0, R = 0, lds = 0, ncolors = 0, nelems = 0, err = 0;
+    std::string msg;
+  };
+  struct Scratch {
+    std::vector<int32_t> nstamp, lid, estamp, owned, elems, color;
+    std::vector<uint64_t> used;
+    Scratch(int64_t nv, int64_t ne) : nstamp(nv, -1), lid(nv, -1), estamp(ne, -1) {}
+  };
+  std::vector<LeafOut> outs(leaves.size());
+  auto build_leaf = [&](size_t leaf, Scratch& S, LeafOut& O) {
+    auto& owned = S.owned;
+    auto& elems = S.elems;
+    auto& nstamp = S.nstamp;
+    auto& lid = S.lid;
+    auto& estamp = S.estamp;
+    auto& used = S.used;
+    auto& color = S.color;
+    const int32_t pid = (int32_t)leaf;
     owned.clear();
-    halo.clear();
     elems.clear();
     int nloc = 0;
     for (int64_t k = leaves[leaf].first; k < leaves[leaf].second; ++k) {
@@ -220,9 +250,9 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
         }
       }
     if (nloc > node_cap || (int)owned.size() > 65535) {
-      delete P;
-      set_error("fused plan: a block exceeds the local node budget");
-      return FEMHIP_EUNSUPPORTED;
+      O.err = FEMHIP_EUNSUPPORTED;
+      O.msg = "fused plan: a block exceeds the local node budget";
+      return;
     }
     // local numbering: owned rows first, then halo nodes, each in global
     // order -- consecutive owned rows are then mostly consecutive CSR rows
@@ -255,9 +285,10 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
         if (l < R) forbid |= used[l];
       }
       if (~forbid == 0) {
-        delete P;
-        set_error("fused plan: more than 64 colours needed");
-        return FEMHIP_EUNSUPPORTED;
+        for (int32_t v : locals) lid[v] = -1;
+        O.err = FEMHIP_EUNSUPPORTED;
+        O.msg = "fused plan: more than 64 colours needed";
+        return;
       }
       const int c = __builtin_ctzll(~forbid);
       color[i] = c;
@@ -291,7 +322,6 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
         for (int bb = 0; bb < NV; ++bb)
           if (!in_row(ga, t[(int64_t)bb * E + e])) spare[la] = 1;
       }
-    const int32_t row_start = (int32_t)(P->rows.size() / 2);
     int lds = 0;
     std::vector<int> row_len(R, 0);
     for (int r = 0; r < R; ++r) {
@@ -301,27 +331,27 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
       int diag = row_len[r];  // pruned diagonal: the spare slot
       for (int k = b; k < en; ++k)
         if (indices[k] == g) diag = k - b;
-      P->rows.push_back(b);
-      P->rows.push_back((int32_t)((uint32_t)lds | ((uint32_t)diag << 16) |
-                                  ((uint32_t)row_len[r] << 24)));
+      O.rows.push_back(b);
+      O.rows.push_back((int32_t)((uint32_t)lds | ((uint32_t)diag << 16) |
+                                 ((uint32_t)row_len[r] << 24)));
       lds += row_len[r] + spare[r];
     }
     if (lds > 65535) {
-      delete P;
-      set_error("fused plan: a block exceeds 65535 LDS slots");
-      return FEMHIP_EUNSUPPORTED;
+      for (int32_t v : locals) lid[v] = -1;
+      O.err = FEMHIP_EUNSUPPORTED;
+      O.msg = "fused plan: a block exceeds 65535 LDS slots";
+      return;
     }
     // chunks and records
-    const int32_t chunk_start = (int32_t)(P->chunks.size() / 2);
     size_t i = 0;
     while (i < order.size()) {
       const int c0 = color[order[i]];
       size_t j = i;
       while (j < order.size() && (int)(j - i) < block && color[order[j]] - c0 < 16) ++j;
       const int nrounds = color[order[j - 1]] - c0 + 1;
-      const int32_t rec0 = (int32_t)P->rec_elem.size();
-      P->chunks.push_back(rec0);
-      P->chunks.push_back((int32_t)((j - i) | ((size_t)nrounds << 16)));
+      const int32_t rec0 = (int32_t)O.rec_elem.size();  // leaf-relative
+      O.chunks.push_back(rec0);
+      O.chunks.push_back((int32_t)((j - i) | ((size_t)nrounds << 16)));
       for (size_t q = i; q < j; ++q) {
         const int32_t e = elems[order[q]];
         uint32_t w[4] = {0, 0, 0, 0};
@@ -349,25 +379,71 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
             bitpos += offbits;
           }
         put_bits(w, bitpos, 4, (uint32_t)(color[order[q]] - c0));
-        for (int k = 0; k < rec_words; ++k) P->recs.push_back(w[k]);
-        P->rec_elem.push_back(e);
+        for (int k = 0; k < rec_words; ++k) O.recs.push_back(w[k]);
+        O.rec_elem.push_back(e);
       }
       i = j;
     }
-    const int32_t nchunks = (int32_t)(P->chunks.size() / 2) - chunk_start;
-    const int32_t node_start = (int32_t)P->nodemap.size();
-    for (int32_t v : locals) P->nodemap.push_back(v);
-    P->parts.insert(P->parts.end(), {node_start, L, row_start, R, lds, chunk_start, nchunks,
-                                     (int32_t)elems.size()});
+    O.nodemap.assign(locals.begin(), locals.end());
     for (int32_t v : locals) lid[v] = -1;
-    max_slots = std::max(max_slots, lds);
-    max_nodes = std::max(max_nodes, L);
-    max_rows = std::max(max_rows, R);
-    max_colors = std::max(max_colors, ncolors);
-    visits += (int64_t)elems.size();
-    ++pid;
+    O.L = L;
+    O.R = R;
+    O.lds = lds;
+    O.ncolors = ncolors;
+    O.nelems = (int32_t)elems.size();
+  };
+  {
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+      Scratch S(Nv, E);
+      for (;;) {
+        const size_t k = next.fetch_add(1);
+        if (k >= leaves.size()) break;
+        build_leaf(k, S, outs[k]);
+      }
+    };
+    const int nth = plan_threads((int64_t)leaves.size());
+    std::vector<std::thread> pool;
+    for (int k = 1; k < nth; ++k) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
   }
-  (void)halo;
+  for (const LeafOut& O : outs)
+    if (O.err) {
+      set_error(O.msg);
+      return O.err;
+    }
+
+  Plan* P = new Plan();
+  std::memset(&P->info, 0, sizeof(P->info));
+  int max_slots = 0, max_nodes = 0, max_rows = 0, max_colors = 0;
+  int64_t visits = 0;
+  for (size_t leaf = 0; leaf < outs.size(); ++leaf) {
+    LeafOut& O = outs[leaf];
+    const int32_t node_start = (int32_t)P->nodemap.size();
+    const int32_t row_start = (int32_t)(P->rows.size() / 2);
+    const int32_t chunk_start = (int32_t)(P->chunks.size() / 2);
+    const int32_t rec_base = (int32_t)P->rec_elem.size();
+    const int32_t nchunks = (int32_t)(O.chunks.size() / 2);
+    P->nodemap.insert(P->nodemap.end(), O.nodemap.begin(), O.nodemap.end());
+    P->rows.insert(P->rows.end(), O.rows.begin(), O.rows.end());
+    for (int32_t c = 0; c < nchunks; ++c) {
+      P->chunks.push_back(O.chunks[2 * c] + rec_base);
+      P->chunks.push_back(O.chunks[2 * c + 1]);
+    }
+    P->recs.insert(P->recs.end(), O.recs.begin(), O.recs.end());
+    P->rec_elem.insert(P->rec_elem.end(), O.rec_elem.begin(), O.rec_elem.end());
+    P->parts.insert(P->parts.end(), {node_start, O.L, row_start, O.R, O.lds, chunk_start,
+                                     nchunks, O.nelems});
+    max_slots = std::max(max_slots, O.lds);
+    max_nodes = std::max(max_nodes, O.L);
+    max_rows = std::max(max_rows, O.R);
+    max_colors = std::max(max_colors, O.ncolors);
+    visits += O.nelems;
+    LeafOut().nodemap.swap(O.nodemap);  // release as we go
+    std::vector<uint32_t>().swap(O.recs);
+  }
+  const int32_t pid = (int32_t)outs.size();
   femhip_fused_plan_info_t& I = P->info;
   I.nparts = pid;
   I.nrecs = (int64_t)P->rec_elem.size();
