@@ -165,3 +165,20 @@ def test_fused_plan_rejects_long_rows():
                                      indices.ctypes.data_as(ctypes.c_void_p),
                                      6144, 1024, 256, ctypes.byref(h), ctypes.byref(info))
     assert rc == _lib.EUNSUPPORTED
+
+
+def test_plan_independent_of_thread_count(monkeypatch):
+    """The blocks are built by a pool of host threads and concatenated in
+    block order: one thread and five threads give the same plan."""
+    x = np.linspace(0, 1, 13)
+    m = femhip.MeshTet.init_tensor(x, x, x).jittered(0.2, 1 / 12)
+    om = O.MeshTet(m.p, m.t)
+    A = O.bilinear_assemble(O.form_laplace, O.CellBasis(om, O.ElementTetP1()))
+    plans = []
+    for nth in ('1', '5'):
+        monkeypatch.setenv('FEMHIP_PLAN_THREADS', nth)
+        plans.append(_build(m, A.indptr, A.indices, slot_cap=700, node_cap=200, block=128))
+    (i1, *a1), (i2, *a2) = plans
+    assert i1.nparts == i2.nparts > 8
+    for x1, x2 in zip(a1, a2):
+        assert np.array_equal(x1, x2)
