@@ -40,23 +40,52 @@ _FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS,
          'linear_elasticity': _lib.ELASTICITY, 'unit_load': _lib.UNIT_LOAD}
 
 
+_PINNED = {}
+
+
+def to_host(t):
+    """A fresh NumPy copy of a device tensor, through a pinned staging
+    buffer kept per dtype (a pageable copy runs at a fraction of the link's
+    rate)."""
+    n = t.numel()
+    buf = _PINNED.get(t.dtype)
+    if buf is None or buf.numel() < n:
+        buf = torch.empty(max(n, 1), dtype=t.dtype, pin_memory=True)
+        _PINNED[t.dtype] = buf
+    view = buf[:n]
+    view.copy_(t.reshape(-1))           # ordered after the producing kernels
+    out = np.empty(tuple(t.shape), dtype=view.numpy().dtype)
+    np.copyto(out.reshape(-1), view.numpy())
+    return out
+
+
 class DeviceCSR:
     """A CSR matrix whose arrays live in HBM (int32 indptr/indices, float64
-    data).  ``to_scipy`` copies it to the host as ``csr_matrix``."""
+    data).  ``to_scipy`` copies it to the host as ``csr_matrix``.  Matrices
+    assembled on one pattern share ``plan``, which keeps a host copy of the
+    pattern after the first transfer (each matrix gets its own copy of it)."""
 
-    def __init__(self, indptr, indices, data, shape):
+    def __init__(self, indptr, indices, data, shape, plan=None):
         self.indptr, self.indices, self.data, self.shape = indptr, indices, data, shape
+        self.plan = plan
 
     @property
     def nnz(self):
         return int(self.indices.shape[0])
 
+    def host_pattern(self):
+        """(indptr, indices) as NumPy arrays owned by the caller."""
+        cache = getattr(self.plan, 'host_pattern', None) if self.plan is not None else None
+        if cache is None:
+            cache = (to_host(self.indptr), to_host(self.indices))
+            if self.plan is not None:
+                self.plan.host_pattern = cache
+        return cache[0].copy(), cache[1].copy()
+
     def to_scipy(self):
         from scipy.sparse import csr_matrix
-        torch.cuda.current_stream().synchronize()
-        A = csr_matrix((self.data.cpu().numpy(), self.indices.cpu().numpy(),
-                        self.indptr.cpu().numpy()), shape=self.shape)
-        return A
+        indptr, indices = self.host_pattern()
+        return csr_matrix((to_host(self.data), indices, indptr), shape=self.shape)
 
 
 class Plan:
@@ -68,6 +97,7 @@ class Plan:
         self.indptr, self.indices, self.perm, self.seg = indptr, indices, perm, seg
         self.nnz, self.signature = nnz, signature
         self.fused = None   # fused-kernel plan, built on demand
+        self.host_pattern = None   # (indptr, indices) on the host, after the first transfer
 
 
 class DeviceBasis:
@@ -270,7 +300,7 @@ class DeviceBasis:
                 self.fused_plans[plan.signature].assemble(name, self.p, out, kappa)
             else:
                 self.elasticity_mfma(lam, mu, plan, out)
-            return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N))
+            return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N), plan)
         contrib = self.local_bilinear(name, kappa, lam, mu)
         plan = self.plan_for(contrib)
         data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
@@ -279,7 +309,7 @@ class DeviceBasis:
             self.validated[vkey] = plan
         elif mfma:
             self.validated[vkey] = plan
-        return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N))
+        return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N), plan)
 
     def elasticity_mfma(self, lam, mu, plan, out, loc=None):
         """Numeric vector-Q2-hex elasticity: MFMA local matrices (element
@@ -448,7 +478,7 @@ class FacetDeviceBasis(DeviceBasis):
         contrib = self.local_bilinear(name, kappa)
         plan = self.plan_for(contrib)
         data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
-        return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N))
+        return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N), plan)
 
     def assemble_linear(self, name='unit_load', kappa=None):
         if self.E == 0:

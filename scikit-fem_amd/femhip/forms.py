@@ -209,9 +209,8 @@ class LinearForm(Form):
         return b
 
     def assemble(self, basis, vbasis=None, **kwargs):
-        b = self.assemble_device(basis, vbasis, **kwargs)
-        torch.cuda.current_stream().synchronize()
-        return b.cpu().numpy()
+        from .device import to_host
+        return to_host(self.assemble_device(basis, vbasis, **kwargs))
 
 
 def asm(form, *bases, **kwargs):
