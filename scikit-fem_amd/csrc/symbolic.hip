@@ -112,8 +112,28 @@ segment_sum_kernel(const double* __restrict__ contrib, const int32_t* __restrict
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= nseg) return;
   const int32_t b = seg[k], en = seg[k + 1];
+  // the gathers of up to four contributions are issued together; the sum
+  // keeps its order
   double acc = 0.0;
-  for (int32_t j = b; j < en; ++j) acc += contrib[perm[j]];
+  int32_t j = b;
+  for (; j + 4 <= en; j += 4) {
+    const int32_t p0 = perm[j], p1 = perm[j + 1], p2 = perm[j + 2], p3 = perm[j + 3];
+    const double v0 = contrib[p0], v1 = contrib[p1], v2 = contrib[p2], v3 = contrib[p3];
+    acc += v0;
+    acc += v1;
+    acc += v2;
+    acc += v3;
+  }
+  const int r = en - j;
+  if (r > 0) {
+    const int32_t p0 = perm[j];
+    const int32_t p1 = r > 1 ? perm[j + 1] : p0;
+    const int32_t p2 = r > 2 ? perm[j + 2] : p0;
+    const double v0 = contrib[p0], v1 = contrib[p1], v2 = contrib[p2];
+    acc += v0;
+    if (r > 1) acc += v1;
+    if (r > 2) acc += v2;
+  }
   out[k] = acc;
 }
 
