@@ -133,6 +133,7 @@ class DeviceBasis:
         self.plans = {}          # zero signature -> Plan
         self.validated = {}      # (form, params, coefficient token, p version) -> Plan
         self.fused_plans = {}    # zero signature -> FusedPlan
+        self.unfusable = {}      # zero signature -> why the fused plan was refused
         self.row_plan = None
 
     # -- coordinates ---------------------------------------------------------
@@ -304,9 +305,15 @@ class DeviceBasis:
         contrib = self.local_bilinear(name, kappa, lam, mu)
         plan = self.plan_for(contrib)
         data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
-        if fusable:
-            self.fused_plan(plan, name)
-            self.validated[vkey] = plan
+        if fusable and plan.signature not in self.unfusable:
+            try:
+                self.fused_plan(plan, name)
+                self.validated[vkey] = plan
+            except (NotImplementedError, ValueError) as exc:
+                # a mesh the fused plan cannot encode (e.g. a CSR row longer
+                # than 31 entries, more than 64 colours): this pattern stays
+                # on the exact path, which is always available
+                self.unfusable[plan.signature] = str(exc)
         elif mfma:
             self.validated[vkey] = plan
         return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N), plan)
@@ -391,6 +398,7 @@ class FacetDeviceBasis(DeviceBasis):
         self.plans = {}
         self.validated = {}
         self.fused_plans = {}
+        self.unfusable = {}
         self.row_plan = None
 
     def set_coordinates(self, p):

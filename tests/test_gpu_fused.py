@@ -207,3 +207,22 @@ def test_fixed_point_accumulation(kind, n, a, caps, block):
         if key in results:           # same input after other inputs: same bits
             assert np.array_equal(results[key], got)
         results[key] = got
+
+
+def test_unfusable_mesh_falls_back_to_exact_path():
+    """A node with 40 neighbours (a CSR row of 41 entries, more than the
+    fused records can encode): the fused plan is refused and every call
+    stays on the exact path, matching the oracle."""
+    K = 40
+    ang = 2 * np.pi * np.arange(K) / K
+    p = np.vstack([np.r_[0.0, np.cos(ang)], np.r_[0.0, np.sin(ang)]])
+    t = np.array([[0] * K, 1 + np.arange(K), 1 + (np.arange(K) + 1) % K])
+    m = femhip.MeshTri(p, t)
+    b = femhip.Basis(m, femhip.ElementTriP1())
+    R = O.bilinear_assemble(O.form_laplace, O.CellBasis(O.MeshTri(m.p, m.t), O.ElementTriP1()))
+    for _ in range(3):
+        A = laplace.assemble(b)
+        assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
+        assert np.abs(A.data - R.data).max() <= TOL * np.abs(R.data).max()
+    d = b.device()
+    assert d.unfusable and not d.validated
