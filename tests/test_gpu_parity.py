@@ -208,3 +208,27 @@ def test_vector_laplace_vs_oracle(kind, elem, n):
     for A in (vector_laplace.assemble(b), vector_laplace.assemble(b), user.assemble(b)):
         assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
         assert np.abs(A.data - R.data).max() <= TOL * np.abs(R.data).max()
+
+
+@pytest.mark.parametrize('kind,elem', [('tri', 'tri1'), ('tri', 'tri2'), ('tet', 'tet1'),
+                                       ('tet', 'tet2')])
+def test_single_element(kind, elem):
+    """One element: laplace, mass and unit_load, twice (P1: the second call
+    runs the fused kernel on a one-partition plan), against the oracle."""
+    if kind == 'tri':
+        p, t = np.array([[0.1, 1.3, 0.2], [0.0, 0.2, 0.9]]), np.array([[0], [1], [2]])
+    else:
+        p = np.array([[0.0, 1.1, 0.1, 0.2], [0.0, 0.1, 0.9, 0.1], [0.0, 0.2, 0.1, 1.2]])
+        t = np.array([[0], [1], [2], [3]])
+    m = MESH[kind](p, t)
+    b = femhip.Basis(m, ELEM[elem]())
+    om = {'tri': O.MeshTri, 'tet': O.MeshTet}[kind](m.p, m.t)
+    ob = O.CellBasis(om, getattr(O, ELEM[elem].__name__)())
+    for name, form in (('laplace', laplace), ('mass', mass)):
+        R = O.assemble_named(name, ob)
+        for _ in range(2):
+            A = form.assemble(b)
+            assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
+            assert np.abs(A.data - R.data).max() <= TOL * np.abs(R.data).max()
+    r = O.assemble_named('unit_load', ob)
+    assert np.abs(unit_load.assemble(b) - r).max() <= TOL * np.abs(r).max()
