@@ -208,9 +208,18 @@ def bench_adjoint(n=65):
     ms_f = time_stream(fwd, 10, stream)
     ms_b = time_stream(bwd, 10, stream)
     ms_fb = time_stream(fwdbwd, 10, stream)
-    return {"elements": m.nelements, "dofs": b.N, "nnz": int(data.numel()),
+    # algorithmic bytes: connectivity, coordinates and kappa read, values and
+    # load written (forward); + dL/dA, dL/db read, dkappa and dX written
+    # (adjoint; dL/dA on the CSR pattern)
+    E, N, nnz = m.nelements, b.N, int(data.numel())
+    fwd_bytes = 16 * E + 24 * N + 8 * E + 8 * nnz + 8 * N
+    bwd_bytes = 16 * E + 24 * N + 8 * E + 8 * nnz + 8 * N + 8 * E + 24 * N
+    return {"elements": E, "dofs": N, "nnz": nnz,
             "forward_ms": ms_f, "adjoint_ms": ms_b, "adjoint_over_forward": ms_b / ms_f,
-            "forward_loss_backward_ms": ms_fb}
+            "forward_loss_backward_ms": ms_fb,
+            "forward_frac_of_8TBps": fwd_bytes / (ms_f * 1e-3) / HBM_PEAK,
+            "adjoint_alg_bytes": bwd_bytes,
+            "adjoint_frac_of_8TBps": bwd_bytes / (ms_b * 1e-3) / HBM_PEAK}
 
 
 def bench_device_solve(basis, A):

@@ -153,7 +153,26 @@ segment_sum_multi_kernel(const double* __restrict__ contrib, int32_t ncomp, int6
   const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (s >= nseg) return;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int32_t k = seg[s]; k < seg[s + 1]; ++k) {
+  const int32_t b = seg[s], en = seg[s + 1];
+  int32_t k = b;
+  // four contributions' gathers are issued together; every component is
+  // summed in the same order as one at a time
+  for (; k + 4 <= en; k += 4) {
+    int64_t c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = perm[k + u];
+    double v[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) v[u][i] = i < ncomp ? contrib[i * in_stride + c[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (i < ncomp) acc[i] += v[u][i];
+  }
+  for (; k < en; ++k) {
     const int64_t c = perm[k];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
