@@ -395,6 +395,7 @@ def bench_elasticity(n=41):
     exec_flops = E * 6 * 2 * 32 * 32 * 28
     return {"elements": E, "dofs": b.N, "nnz": plan.nnz, "first_call_ms": first,
             "mfma_local_ms": ms_local, "assembly_ms": ms_all,
+            "segment_sum_ms": ms_all - ms_local,
             "mfma_TFLOPs_executed": exec_flops / (ms_local * 1e-3) / 1e12,
             "mfma_util_of_78.6TF": exec_flops / (ms_local * 1e-3) / 78.6e12,
             "survey_BtDB_TFLOPs_equiv": E * 2283228 / (ms_local * 1e-3) / 1e12,

@@ -442,17 +442,18 @@ int femhip_pcg(const int32_t* indptr, const int32_t* indices, const double* data
  * Vector Q2 hexahedron linear elasticity on FP64 matrix cores.  Numeric
  * counterpart of BilinearForm._assemble for linear_elasticity(lam, mu) on
  * ElementVector(ElementHex2) with 27 Gauss points
- * (bilinear_form.py:58-128, models/elasticity.py:35-53): out[e][j*81 + i]
- * (element-major; j trial, i test) from the mapping G of femhip_geometry.
- * groups: persistent workgroups (0 = automatic).
- * femhip_segment_sum_em: the segmented sums of femhip_segment_sum for
- * element-major local matrices (nb2 = nbfun^2 entries per element).
+ * (bilinear_form.py:58-128, models/elasticity.py:35-53): out[e][i*81 + j]
+ * (element-major, row-major; i test, j trial) from the mapping G of
+ * femhip_geometry.  groups: persistent workgroups (0 = automatic).
+ * femhip_segment_sum_em: the segmented sums of femhip_segment_sum for such
+ * element-major, row-major local matrices of nb x nb entries; contribution
+ * c = (j*nb + i)*nelems + e of the plan names entry (i, j) of element e.
  * --------------------------------------------------------------------- */
 int femhip_hex2_elasticity_mfma(const femhip_geom_t* g, const femhip_space_t* sp, double lam,
                                 double mu, const double* G, double* out, int32_t groups,
                                 femhip_stream_t s);
 int femhip_segment_sum_em(const double* loc, const int32_t* perm, const int32_t* seg,
-                          int64_t nseg, int64_t nelems, int64_t nb2, double* out,
+                          int64_t nseg, int64_t nelems, int64_t nb, double* out,
                           femhip_stream_t s);
 
 #ifdef __cplusplus

@@ -89,10 +89,11 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
       for (int r = 0; r < 4; ++r) s_S[kl][ti * 16 + 4 * r + k4][tj * 16 + r16] = acc[r];
     }
     __syncthreads();
-    // 81 x 81 entries, element-major, index j*81 + i (j trial, i test)
+    // 81 x 81 entries, element-major and row-major: index i*81 + j (i test,
+    // j trial), so that what one element adds to one global row is adjacent
     double* o = out + e * (int64_t)(81 * 81);
     for (int idx = tid; idx < 81 * 81; idx += kThreads) {
-      const int j = idx / 81, i = idx - j * 81;
+      const int i = idx / 81, j = idx - i * 81;
       const int a = i / 3, c = i - 3 * a, b = j / 3, d = j - 3 * b;
       auto S = [&](int p, int r, int x, int y) -> double {
         // S_pr[x][y] with S_rp = S_pr^T
@@ -110,19 +111,42 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
   }
 }
 
-// out[k] = sum of loc[(c % E) * nb2 + c / E] over the plan's contributions
-// c = perm[j], j in [seg[k], seg[k+1]) (element-major local matrices).
+// out[k] = sum of the plan's contributions c = perm[j], j in [seg[k], seg[k+1]),
+// in that order; contribution c = (tj*nb + ti)*E + e is entry (ti test, tj
+// trial) of element e, read from the row-major local matrices
+// loc[e*nb*nb + ti*nb + tj].  The slots of one global row then gather from the
+// same few cache lines of each element; up to four gathers are in flight.
 __global__ void __launch_bounds__(256)
 segment_sum_em_kernel(const double* __restrict__ loc, const int32_t* __restrict__ perm,
-                      const int32_t* __restrict__ seg, int64_t nseg, int64_t E, int64_t nb2,
+                      const int32_t* __restrict__ seg, int64_t nseg, uint32_t E, uint32_t nb,
                       double* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (k >= nseg) return;
+  auto at = [&](int32_t c) -> double {
+    const uint32_t cu = (uint32_t)c, ent = cu / E, e = cu - ent * E;
+    const uint32_t tj = ent / nb, ti = ent - tj * nb;
+    return loc[(int64_t)e * (nb * nb) + ti * nb + tj];
+  };
+  const int32_t b = seg[k], en = seg[k + 1];
   double acc = 0.0;
-  for (int32_t j = seg[k]; j < seg[k + 1]; ++j) {
-    const int64_t c = perm[j];
-    const int64_t e = c % E, entry = c / E;
-    acc += loc[e * nb2 + entry];
+  int32_t j = b;
+  for (; j + 4 <= en; j += 4) {
+    const int32_t p0 = perm[j], p1 = perm[j + 1], p2 = perm[j + 2], p3 = perm[j + 3];
+    const double v0 = at(p0), v1 = at(p1), v2 = at(p2), v3 = at(p3);
+    acc += v0;
+    acc += v1;
+    acc += v2;
+    acc += v3;
+  }
+  const int r = en - j;
+  if (r > 0) {
+    const int32_t p0 = perm[j];
+    const int32_t p1 = r > 1 ? perm[j + 1] : p0;
+    const int32_t p2 = r > 2 ? perm[j + 2] : p0;
+    const double v0 = at(p0), v1 = at(p1), v2 = at(p2);
+    acc += v0;
+    if (r > 1) acc += v1;
+    if (r > 2) acc += v2;
   }
   out[k] = acc;
 }
@@ -149,14 +173,16 @@ extern "C" int femhip_hex2_elasticity_mfma(const femhip_geom_t* g, const femhip_
 }
 
 extern "C" int femhip_segment_sum_em(const double* loc, const int32_t* perm, const int32_t* seg,
-                                     int64_t nseg, int64_t nelems, int64_t nb2, double* out,
+                                     int64_t nseg, int64_t nelems, int64_t nb, double* out,
                                      femhip_stream_t s) {
-  FEMHIP_REQUIRE(nseg >= 0 && nelems > 0 && nb2 > 0, FEMHIP_EINVAL, "bad sizes");
+  FEMHIP_REQUIRE(nseg >= 0 && nelems > 0 && nb > 0, FEMHIP_EINVAL, "bad sizes");
+  FEMHIP_REQUIRE(nelems * nb * nb <= INT32_MAX, FEMHIP_EOVERFLOW,
+                 "segment sum: contributions exceed int32 indices");
   if (nseg == 0) return FEMHIP_OK;
   FEMHIP_REQUIRE(loc && perm && seg && out, FEMHIP_EINVAL, "NULL argument");
   segment_sum_em_kernel<<<(unsigned)ceil_div(nseg, 256), 256, 0,
-                          reinterpret_cast<hipStream_t>(s)>>>(loc, perm, seg, nseg, nelems,
-                                                              nb2, out);
+                          reinterpret_cast<hipStream_t>(s)>>>(loc, perm, seg, nseg, (uint32_t)nelems,
+                                                              (uint32_t)nb, out);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }

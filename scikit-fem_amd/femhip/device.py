@@ -320,7 +320,7 @@ class DeviceBasis:
 
     def elasticity_mfma(self, lam, mu, plan, out, loc=None):
         """Numeric vector-Q2-hex elasticity: MFMA local matrices (element
-        major) + segmented sums over the plan."""
+        major, row-major) + segmented sums over the plan."""
         G = self.geometry()
         nb2 = self.nbfun * self.nbfun
         if loc is None:
@@ -329,7 +329,7 @@ class DeviceBasis:
             ctypes.byref(self.geom), ctypes.byref(self.space), float(lam), float(mu),
             _lib.ptr(G), _lib.ptr(loc), 0, _lib.stream_ptr()))
         _lib.check(self.lib.femhip_segment_sum_em(
-            _lib.ptr(loc), _lib.ptr(plan.perm), _lib.ptr(plan.seg), plan.nnz, self.E, nb2,
+            _lib.ptr(loc), _lib.ptr(plan.perm), _lib.ptr(plan.seg), plan.nnz, self.E, self.nbfun,
             _lib.ptr(out), _lib.stream_ptr()))
         return out
 
