@@ -13,9 +13,12 @@
 // |detDF| per point, from femhip_geometry) and the reference gradients.
 //
 // One workgroup (4 waves) per element at a time (grid-stride over
-// elements); each wave owns 6 of the 24 output tiles.  The local matrix is
-// written element-major, out[e][j*81 + i] (j trial, i test), contiguous per
-// element.
+// elements); each wave owns 6 of the 24 output tiles.  The six 27 x 27
+// products share LDS with the operand tables they are computed from, so a
+// workgroup holds 62.7 KB and two fit on a CU: one assembles its local matrix
+// while the other runs its products.  The local matrix is written
+// element-major and row-major, out[e][i*81 + j] (i test, j trial),
+// contiguous per element.
 #include "common.h"
 
 namespace femhip {
@@ -35,19 +38,19 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
                             double mu, double* __restrict__ out) {
   __shared__ double s_dphi[kNB * 3 * kQ];           // reference gradients
   __shared__ double s_map[kQ * 10];                 // invDF (9) + omega per point
-  __shared__ double s_ga[3][kBP][kQP];              // G_k^T: [k][function][point]
-  __shared__ double s_gb[3][kQP][kBP];              // diag(omega) G_l: [l][point][fn]
-  __shared__ double s_S[6][kBP][kBP];               // S_kl, k <= l
+  // the operand tables, then (same memory) the products
+  constexpr int kTab = 3 * kBP * kQP;
+  constexpr int kBuf = 2 * kTab > 6 * kNB * kNB ? 2 * kTab : 6 * kNB * kNB;
+  __shared__ double s_buf[kBuf];
+  double(*s_ga)[kBP][kQP] = reinterpret_cast<double(*)[kBP][kQP]>(s_buf);  // G_k^T
+  double(*s_gb)[kQP][kBP] =
+      reinterpret_cast<double(*)[kQP][kBP]>(s_buf + kTab);  // diag(omega) G_l
+  double(*s_S)[kNB][kNB] = reinterpret_cast<double(*)[kNB][kNB]>(s_buf);  // S_kl, k <= l
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int64_t EQ = E * kQ;
 
   for (int i = tid; i < kNB * 3 * kQ; i += kThreads) s_dphi[i] = dphi[i];
-  // zero padding of the operand tables once (never overwritten)
-  for (int i = tid; i < 3 * kBP * kQP; i += kThreads) {
-    (&s_ga[0][0][0])[i] = 0.0;
-    (&s_gb[0][0][0])[i] = 0.0;
-  }
 
   for (int64_t e = blockIdx.x; e < E; e += gridDim.x) {
     __syncthreads();
@@ -58,35 +61,54 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
       s_map[i] = v;
     }
     __syncthreads();
-    // global gradients g^a_k(q) = sum_i invDF[i][k] dphi[a][i][q]
-    for (int i = tid; i < 3 * kNB * kQ; i += kThreads) {
-      const int k = i / (kNB * kQ);
-      const int a = (i / kQ) % kNB;
-      const int q = i % kQ;
-      const double* m = s_map + q * 10;
-      const double g = m[0 * 3 + k] * s_dphi[(a * 3 + 0) * kQ + q] +
-                       m[1 * 3 + k] * s_dphi[(a * 3 + 1) * kQ + q] +
-                       m[2 * 3 + k] * s_dphi[(a * 3 + 2) * kQ + q];
+    // global gradients g^a_k(q) = sum_i invDF[i][k] dphi[a][i][q], and the
+    // zero padding (a >= 27, q >= 27), which the products overwrote
+    for (int i = tid; i < kTab; i += kThreads) {
+      const int k = i / (kBP * kQP);
+      const int a = (i / kQP) % kBP;
+      const int q = i % kQP;
+      double g = 0.0, gw = 0.0;
+      if (a < kNB && q < kQ) {
+        const double* m = s_map + q * 10;
+        g = m[0 * 3 + k] * s_dphi[(a * 3 + 0) * kQ + q] +
+            m[1 * 3 + k] * s_dphi[(a * 3 + 1) * kQ + q] +
+            m[2 * 3 + k] * s_dphi[(a * 3 + 2) * kQ + q];
+        gw = g * m[9];
+      }
       s_ga[k][a][q] = g;
-      s_gb[k][q][a] = g * m[9];
+      s_gb[k][q][a] = gw;
     }
     __syncthreads();
-    // 6 products x 4 tiles = 24 tiles, 6 per wave
-    for (int tile = wave; tile < 24; tile += 4) {
+    // 6 products x 4 tiles = 24 tiles, 6 per wave, kept in registers until
+    // every wave has read the operand tables
+    double4_t acc[6];
+    const int r16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      const int tile = wave + 4 * t;
       const int kl = tile >> 2, ti = (tile >> 1) & 1, tj = tile & 1;
       const int k = kl < 3 ? kl : (kl == 3 ? 0 : (kl == 4 ? 0 : 1));
       const int l = kl < 3 ? kl : (kl == 3 ? 1 : 2);
-      double4_t acc = {0.0, 0.0, 0.0, 0.0};
-      const int r16 = lane & 15, k4 = lane >> 4;
+      acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int q0 = 0; q0 < kQP; q0 += 4) {
         const double a = s_ga[k][ti * 16 + r16][q0 + k4];   // A[i][kk]
         const double b = s_gb[l][q0 + k4][tj * 16 + r16];   // B[kk][j]
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
       }
-      // v_mfma_f64_16x16x4f64 output: register r of lane l is D[4r + l/16][l%16]
+    }
+    __syncthreads();
+    // v_mfma_f64_16x16x4f64 output: register r of lane l is D[4r + l/16][l%16]
 #pragma unroll
-      for (int r = 0; r < 4; ++r) s_S[kl][ti * 16 + 4 * r + k4][tj * 16 + r16] = acc[r];
+    for (int t = 0; t < 6; ++t) {
+      const int tile = wave + 4 * t;
+      const int kl = tile >> 2, ti = (tile >> 1) & 1, tj = tile & 1;
+      const int col = tj * 16 + r16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = ti * 16 + 4 * r + k4;
+        if (row < kNB && col < kNB) s_S[kl][row][col] = acc[t][r];
+      }
     }
     __syncthreads();
     // 81 x 81 entries, element-major and row-major: index i*81 + j (i test,
