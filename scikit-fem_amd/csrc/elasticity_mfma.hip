@@ -14,9 +14,10 @@
 //
 // One workgroup (4 waves) per element at a time (grid-stride over
 // elements); each wave owns 6 of the 24 output tiles.  The six 27 x 27
-// products share LDS with the operand tables they are computed from, so a
-// workgroup holds 62.7 KB and two fit on a CU: one assembles its local matrix
-// while the other runs its products.  The local matrix is written
+// products share LDS with the operand tables they are computed from, and the
+// reference gradients are read through the cache, so a workgroup holds
+// 38.5 KB and four fit on a CU: while some assemble their local matrices,
+// others run their products.  The local matrix is written
 // element-major and row-major, out[e][i*81 + j] (i test, j trial),
 // contiguous per element.
 #include "common.h"
@@ -36,21 +37,26 @@ __global__ void __launch_bounds__(kThreads)
 hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restrict__ W,
                             const double* __restrict__ dphi, int64_t E, double lam,
                             double mu, double* __restrict__ out) {
-  __shared__ double s_dphi[kNB * 3 * kQ];           // reference gradients
   __shared__ double s_map[kQ * 10];                 // invDF (9) + omega per point
-  // the operand tables, then (same memory) the products
-  constexpr int kTab = 3 * kBP * kQP;
-  constexpr int kBuf = 2 * kTab > 6 * kNB * kNB ? 2 * kTab : 6 * kNB * kNB;
+  // The operand tables, then (same memory) the products.  The tables hold
+  // the 27 functions only: a tile's rows 27..31 of A read the next rows of
+  // the buffer and its columns 27..31 of B the next row's first entries (or
+  // the zeroed tail), which reach only the discarded rows / columns of the
+  // product; the padded point q = 27 is an explicit zero in both tables.
+  constexpr int kTab = 3 * kNB * kQP;
+  constexpr int kTail = 8;
+  constexpr int kBuf = 2 * kTab + kTail;
+  static_assert(6 * kNB * kNB <= kBuf, "products fit the table memory");
   __shared__ double s_buf[kBuf];
-  double(*s_ga)[kBP][kQP] = reinterpret_cast<double(*)[kBP][kQP]>(s_buf);  // G_k^T
-  double(*s_gb)[kQP][kBP] =
-      reinterpret_cast<double(*)[kQP][kBP]>(s_buf + kTab);  // diag(omega) G_l
+  double(*s_ga)[kNB][kQP] = reinterpret_cast<double(*)[kNB][kQP]>(s_buf);  // G_k^T
+  double(*s_gb)[kQP][kNB] =
+      reinterpret_cast<double(*)[kQP][kNB]>(s_buf + kTab);  // diag(omega) G_l
   double(*s_S)[kNB][kNB] = reinterpret_cast<double(*)[kNB][kNB]>(s_buf);  // S_kl, k <= l
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int64_t EQ = E * kQ;
+  if (tid < kTail) s_buf[2 * kTab + tid] = 0.0;
 
-  for (int i = tid; i < kNB * 3 * kQ; i += kThreads) s_dphi[i] = dphi[i];
 
   for (int64_t e = blockIdx.x; e < E; e += gridDim.x) {
     __syncthreads();
@@ -62,17 +68,17 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
     }
     __syncthreads();
     // global gradients g^a_k(q) = sum_i invDF[i][k] dphi[a][i][q], and the
-    // zero padding (a >= 27, q >= 27), which the products overwrote
+    // zeros of the padded point q = 27
     for (int i = tid; i < kTab; i += kThreads) {
-      const int k = i / (kBP * kQP);
-      const int a = (i / kQP) % kBP;
+      const int k = i / (kNB * kQP);
+      const int a = (i / kQP) % kNB;
       const int q = i % kQP;
       double g = 0.0, gw = 0.0;
-      if (a < kNB && q < kQ) {
+      if (q < kQ) {
         const double* m = s_map + q * 10;
-        g = m[0 * 3 + k] * s_dphi[(a * 3 + 0) * kQ + q] +
-            m[1 * 3 + k] * s_dphi[(a * 3 + 1) * kQ + q] +
-            m[2 * 3 + k] * s_dphi[(a * 3 + 2) * kQ + q];
+        g = m[0 * 3 + k] * dphi[(a * 3 + 0) * kQ + q] +
+            m[1 * 3 + k] * dphi[(a * 3 + 1) * kQ + q] +
+            m[2 * 3 + k] * dphi[(a * 3 + 2) * kQ + q];
         gw = g * m[9];
       }
       s_ga[k][a][q] = g;
@@ -92,8 +98,8 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
       acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int q0 = 0; q0 < kQP; q0 += 4) {
-        const double a = s_ga[k][ti * 16 + r16][q0 + k4];   // A[i][kk]
-        const double b = s_gb[l][q0 + k4][tj * 16 + r16];   // B[kk][j]
+        const double a = s_buf[(k * kNB + ti * 16 + r16) * kQP + q0 + k4];           // A[i][kk]
+        const double b = s_buf[kTab + (l * kQP + q0 + k4) * kNB + tj * 16 + r16];   // B[kk][j]
         acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
       }
     }
