@@ -118,23 +118,35 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
     }
     __syncthreads();
     // 81 x 81 entries, element-major and row-major: index i*81 + j (i test,
-    // j trial), so that what one element adds to one global row is adjacent
-    double* o = out + e * (int64_t)(81 * 81);
-    for (int idx = tid; idx < 81 * 81; idx += kThreads) {
-      const int i = idx / 81, j = idx - i * 81;
-      const int a = i / 3, c = i - 3 * a, b = j / 3, d = j - 3 * b;
-      auto S = [&](int p, int r, int x, int y) -> double {
-        // S_pr[x][y] with S_rp = S_pr^T
+    // j trial), so that what one element adds to one global row is adjacent.
+    // Thread t < 243 writes entry t of rows 3a .. 3a+2 for a = 0..26: its
+    // components c = t / 81 (test) and d = (t % 81) % 3 (trial) and trial
+    // function b = (t % 81) / 3 stay fixed, the test function a advances.
+    if (tid < 3 * 81) {
+      const int c = tid / 81, j = tid - 81 * c, b = j / 3, d = j - 3 * b;
+      // S_pr[a][b] for p <= r, S_rp[b][a] = S_pr[a][b] otherwise: the address
+      // of S_pr[a][b] is base + a * step
+      auto locate = [&](int p, int r, const double*& base, int& step) {
+        const int lo = p < r ? p : r, hi = p < r ? r : p;
+        const int kl = lo == hi ? lo : (lo == 0 ? (hi == 1 ? 3 : 4) : 5);
         if (p <= r) {
-          const int kl = p == r ? p : (p == 0 ? (r == 1 ? 3 : 4) : 5);
-          return s_S[kl][x][y];
+          base = &s_S[kl][0][b];
+          step = kNB;
+        } else {
+          base = &s_S[kl][b][0];
+          step = 1;
         }
-        const int kl = (r == 0 ? (p == 1 ? 3 : 4) : 5);
-        return s_S[kl][y][x];
       };
-      double v = lam * S(c, d, a, b) + mu * S(d, c, a, b);
-      if (c == d) v += mu * (s_S[0][a][b] + s_S[1][a][b] + s_S[2][a][b]);
-      __builtin_nontemporal_store(v, o + idx);
+      const double *b1, *b2;
+      int s1, s2;
+      locate(c, d, b1, s1);
+      locate(d, c, b2, s2);
+      double* o = out + e * (int64_t)(81 * 81) + tid;
+      for (int a = 0; a < kNB; ++a) {
+        double v = lam * b1[a * s1] + mu * b2[a * s2];
+        if (c == d) v += mu * (s_S[0][a][b] + s_S[1][a][b] + s_S[2][a][b]);
+        __builtin_nontemporal_store(v, o + a * (3 * 81));
+      }
     }
   }
 }
