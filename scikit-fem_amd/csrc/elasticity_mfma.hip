@@ -142,6 +142,7 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
       locate(c, d, b1, s1);
       locate(d, c, b2, s2);
       double* o = out + e * (int64_t)(81 * 81) + tid;
+#pragma unroll 3
       for (int a = 0; a < kNB; ++a) {
         double v = lam * b1[a * s1] + mu * b2[a * s2];
         if (c == d) v += mu * (s_S[0][a][b] + s_S[1][a][b] + s_S[2][a][b]);
