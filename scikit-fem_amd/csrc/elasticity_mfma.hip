@@ -152,6 +152,24 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
   }
 }
 
+// Division of n < 2^31 by a divisor d fixed for a launch, as a multiply and a
+// shift (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", 1994): with 2^s >= d and m = ceil(2^(31+s) / d),
+// floor(n m / 2^(31+s)) = floor(n / d), the error n (m d - 2^(31+s)) /
+// (d 2^(31+s)) being below 2^-s <= 1/d; n m < 2^63.
+struct Div31 {
+  uint64_t m;
+  uint32_t shift;
+  static Div31 make(uint32_t d) {
+    uint32_t s = 0;
+    while ((1ull << s) < d) ++s;
+    return Div31{((1ull << (31 + s)) + d - 1) / d, 31 + s};
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    return (uint32_t)(((uint64_t)n * m) >> shift);
+  }
+};
+
 // out[k] = sum of the plan's contributions c = perm[j], j in [seg[k], seg[k+1]),
 // in that order; contribution c = (tj*nb + ti)*E + e is entry (ti test, tj
 // trial) of element e, read from the row-major local matrices
@@ -160,12 +178,12 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
 __global__ void __launch_bounds__(256)
 segment_sum_em_kernel(const double* __restrict__ loc, const int32_t* __restrict__ perm,
                       const int32_t* __restrict__ seg, int64_t nseg, uint32_t E, uint32_t nb,
-                      double* __restrict__ out) {
+                      Div31 byE, Div31 byNB, double* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (k >= nseg) return;
   auto at = [&](int32_t c) -> double {
-    const uint32_t cu = (uint32_t)c, ent = cu / E, e = cu - ent * E;
-    const uint32_t tj = ent / nb, ti = ent - tj * nb;
+    const uint32_t cu = (uint32_t)c, ent = byE.div(cu), e = cu - ent * E;
+    const uint32_t tj = byNB.div(ent), ti = ent - tj * nb;
     return loc[(int64_t)e * (nb * nb) + ti * nb + tj];
   };
   const int32_t b = seg[k], en = seg[k + 1];
@@ -223,7 +241,9 @@ extern "C" int femhip_segment_sum_em(const double* loc, const int32_t* perm, con
   FEMHIP_REQUIRE(loc && perm && seg && out, FEMHIP_EINVAL, "NULL argument");
   segment_sum_em_kernel<<<(unsigned)ceil_div(nseg, 256), 256, 0,
                           reinterpret_cast<hipStream_t>(s)>>>(loc, perm, seg, nseg, (uint32_t)nelems,
-                                                              (uint32_t)nb, out);
+                                                              (uint32_t)nb,
+                                                              Div31::make((uint32_t)nelems),
+                                                              Div31::make((uint32_t)nb), out);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }
