@@ -144,13 +144,17 @@ __device__ __forceinline__ double elasticity_value(const double (&gu)[D], int cu
   return s;
 }
 
+// The nb*nb entry blocks of one block of kBlock elements are adjacent in the
+// grid, so the elements' mapping (27 points x 10 values per hexahedron) is
+// fetched once into L2 and read there by every entry, not once per entry.
 template <int D, bool AFF, int Q, int FK, bool VEC>
 __global__ void __launch_bounds__(kBlock) local_bilinear_kernel(LocalArgs a) {
 #pragma clang fp contract(off)
   const int64_t E = a.E;
-  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const unsigned nb2 = (unsigned)(a.nb * a.nb);
+  const int64_t e = (int64_t)(blockIdx.x / nb2) * kBlock + threadIdx.x;
   if (e >= E) return;
-  const int entry = blockIdx.y;  // = j * nb + i
+  const int entry = (int)(blockIdx.x % nb2);  // = j * nb + i
   const int j = entry / a.nb;
   const int i = entry - j * a.nb;
   const int fu = VEC ? j / D : j, cu = VEC ? j - (j / D) * D : 0;
@@ -307,7 +311,9 @@ int launch_bilinear_q(const LocalArgs& a, int fk, bool vec, hipStream_t s) {
       return FEMHIP_OK;
     }
   }
-  dim3 grid((unsigned)ceil_div(a.E, kBlock), (unsigned)(a.nb * a.nb));
+  FEMHIP_REQUIRE(ceil_div(a.E, kBlock) * a.nb * a.nb <= INT32_MAX, FEMHIP_EOVERFLOW,
+                 "local entries: grid exceeds 2^31 blocks");
+  const unsigned grid = (unsigned)(ceil_div(a.E, kBlock) * a.nb * a.nb);
   if (vec) {
     FEMHIP_REQUIRE(fk == FEMHIP_ELASTICITY, FEMHIP_EUNSUPPORTED,
                    "vector elements support only linear_elasticity");
