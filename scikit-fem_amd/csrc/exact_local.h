@@ -262,10 +262,13 @@ __global__ void __launch_bounds__(kBlock) local_row_kernel(LocalArgs a) {
 template <int D, bool AFF, int Q, bool COEF>
 __global__ void __launch_bounds__(kBlock) local_load_kernel(LocalArgs a) {
 #pragma clang fp contract(off)
+  // isoparametric: the nb row blocks of one element block are adjacent in
+  // the grid (the per-point mapping is read from L2, as in the entry kernel)
   const int64_t E = a.E;
-  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const unsigned eb = AFF ? blockIdx.x : blockIdx.x / (unsigned)a.nb;
+  const int i = AFF ? (int)blockIdx.y : (int)(blockIdx.x % (unsigned)a.nb);
+  const int64_t e = (int64_t)eb * kBlock + threadIdx.x;
   if (e >= E) return;
-  const int i = blockIdx.y;
   MapAt<D, AFF, Q> m;
   if constexpr (AFF) m.load(a.G, E, e, 0);
   auto term = [&](int q) -> double {
@@ -343,7 +346,8 @@ int launch_bilinear_q(const LocalArgs& a, int fk, bool vec, hipStream_t s) {
 
 template <int D, bool AFF, int Q>
 int launch_load_q(const LocalArgs& a, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(a.E, kBlock), (unsigned)a.nb);
+  const dim3 grid = AFF ? dim3((unsigned)ceil_div(a.E, kBlock), (unsigned)a.nb)
+                        : dim3((unsigned)(ceil_div(a.E, kBlock) * a.nb));
   if (a.kappa != nullptr)
     local_load_kernel<D, AFF, Q, true><<<grid, kBlock, 0, s>>>(a);
   else
