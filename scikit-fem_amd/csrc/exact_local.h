@@ -144,17 +144,30 @@ __device__ __forceinline__ double elasticity_value(const double (&gu)[D], int cu
   return s;
 }
 
-// The nb*nb entry blocks of one block of kBlock elements are adjacent in the
-// grid, so the elements' mapping (27 points x 10 values per hexahedron) is
-// fetched once into L2 and read there by every entry, not once per entry.
+// Simplices: the nb*nb entry blocks of one block of kBlock elements are
+// adjacent in the grid, so the elements' mapping is fetched once into L2 and
+// read there by every entry.  Hexahedra (Q points x 10 values of mapping per
+// element, up to 27 KB): blocks of kIsoBlock elements, and workgroup b, which
+// the dispatcher places on XCD b % 8, takes an element block that only that
+// XCD works on, so one XCD's L2 holds the mappings its entries read.
+constexpr int kIsoBlock = 64;
+constexpr int kXcds = 8;
 template <int D, bool AFF, int Q, int FK, bool VEC>
 __global__ void __launch_bounds__(kBlock) local_bilinear_kernel(LocalArgs a) {
 #pragma clang fp contract(off)
   const int64_t E = a.E;
   const unsigned nb2 = (unsigned)(a.nb * a.nb);
-  const int64_t e = (int64_t)(blockIdx.x / nb2) * kBlock + threadIdx.x;
+  int64_t e;
+  int entry;  // = j * nb + i
+  if constexpr (AFF) {
+    e = (int64_t)(blockIdx.x / nb2) * kBlock + threadIdx.x;
+    entry = (int)(blockIdx.x % nb2);
+  } else {
+    const unsigned xcd = blockIdx.x % kXcds, r = blockIdx.x / kXcds;
+    entry = (int)(r % nb2);
+    e = ((int64_t)(r / nb2) * kXcds + xcd) * kIsoBlock + threadIdx.x;
+  }
   if (e >= E) return;
-  const int entry = (int)(blockIdx.x % nb2);  // = j * nb + i
   const int j = entry / a.nb;
   const int i = entry - j * a.nb;
   const int fu = VEC ? j / D : j, cu = VEC ? j - (j / D) * D : 0;
@@ -314,26 +327,29 @@ int launch_bilinear_q(const LocalArgs& a, int fk, bool vec, hipStream_t s) {
       return FEMHIP_OK;
     }
   }
-  FEMHIP_REQUIRE(ceil_div(a.E, kBlock) * a.nb * a.nb <= INT32_MAX, FEMHIP_EOVERFLOW,
+  const int64_t nblk = AFF ? ceil_div(a.E, kBlock)
+                           : ceil_div(ceil_div(a.E, kIsoBlock), kXcds) * kXcds;
+  FEMHIP_REQUIRE(nblk * a.nb * a.nb <= INT32_MAX, FEMHIP_EOVERFLOW,
                  "local entries: grid exceeds 2^31 blocks");
-  const unsigned grid = (unsigned)(ceil_div(a.E, kBlock) * a.nb * a.nb);
+  const unsigned grid = (unsigned)(nblk * a.nb * a.nb);
+  const int threads = AFF ? kBlock : kIsoBlock;
   if (vec) {
     FEMHIP_REQUIRE(fk == FEMHIP_ELASTICITY, FEMHIP_EUNSUPPORTED,
                    "vector elements support only linear_elasticity");
-    local_bilinear_kernel<D, AFF, Q, FEMHIP_ELASTICITY, true><<<grid, kBlock, 0, s>>>(a);
+    local_bilinear_kernel<D, AFF, Q, FEMHIP_ELASTICITY, true><<<grid, threads, 0, s>>>(a);
   } else {
     switch (fk) {
       case FEMHIP_LAPLACE:
-        local_bilinear_kernel<D, AFF, Q, FEMHIP_LAPLACE, false><<<grid, kBlock, 0, s>>>(a);
+        local_bilinear_kernel<D, AFF, Q, FEMHIP_LAPLACE, false><<<grid, threads, 0, s>>>(a);
         break;
       case FEMHIP_MASS:
-        local_bilinear_kernel<D, AFF, Q, FEMHIP_MASS, false><<<grid, kBlock, 0, s>>>(a);
+        local_bilinear_kernel<D, AFF, Q, FEMHIP_MASS, false><<<grid, threads, 0, s>>>(a);
         break;
       case FEMHIP_KAPPA_LAPLACE:
-        local_bilinear_kernel<D, AFF, Q, FEMHIP_KAPPA_LAPLACE, false><<<grid, kBlock, 0, s>>>(a);
+        local_bilinear_kernel<D, AFF, Q, FEMHIP_KAPPA_LAPLACE, false><<<grid, threads, 0, s>>>(a);
         break;
       case FEMHIP_COEF_MASS:
-        local_bilinear_kernel<D, AFF, Q, FEMHIP_COEF_MASS, false><<<grid, kBlock, 0, s>>>(a);
+        local_bilinear_kernel<D, AFF, Q, FEMHIP_COEF_MASS, false><<<grid, threads, 0, s>>>(a);
         break;
       default:
         set_error("scalar elements support laplace, mass, kappa_laplace and coef_mass");

@@ -287,20 +287,19 @@ class DeviceBasis:
         calls with the same inputs go through the fused kernel when the
         element/form has one (P1 triangles/tetrahedra), the MFMA kernel for
         vector Q2-hex elasticity, else the exact path on the validated
-        pattern (no exact-zero signature pass for scalar simplex forms).
+        pattern (no exact-zero signature pass for the scalar forms).
         """
         from . import fused
         fusable = (path != 'generic' and fused.eligible(self.basis, name)
                    and (kappa is None or kappa.dim() == 1))
         mfma = (path != 'generic' and name == 'linear_elasticity' and self.kind == 'hex'
                 and self.basis.elem.vector and self.basis.elem.nbs == 27 and self.Q == 27)
-        # other scalar forms on simplices: the exact path without the
-        # exact-zero signature (same inputs, same zeros)
-        simplex = (path != 'generic' and not fusable and self.kind in ('tri', 'tet')
-                   and not self.basis.elem.vector
-                   and name in ('laplace', 'mass', 'kappa_laplace', 'coef_mass'))
+        # the other scalar forms: the exact path without the exact-zero
+        # signature (same inputs, same zeros)
+        scalar = (path != 'generic' and not fusable and not self.basis.elem.vector
+                  and name in ('laplace', 'mass', 'kappa_laplace', 'coef_mass'))
         vkey = (name, lam, mu, token, self.p_version)
-        if (fusable or mfma or simplex) and vkey in self.validated:
+        if (fusable or mfma or scalar) and vkey in self.validated:
             plan = self.validated[vkey]
             out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
                               device=self.p.device)[:plan.nnz]
@@ -324,7 +323,7 @@ class DeviceBasis:
                 # than 31 entries, more than 64 colours): this pattern stays
                 # on the exact path, which is always available
                 self.unfusable[plan.signature] = str(exc)
-        elif mfma or simplex:
+        elif mfma or scalar:
             self.validated[vkey] = plan
         return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N), plan)
 

@@ -1,5 +1,5 @@
-"""Later calls of scalar forms on triangles and tetrahedra that have no fused
-kernel (P2, and P1 with a per-point coefficient or ``w.c * u * v``): the
+"""Later calls of scalar forms that have no fused kernel (P2, P1 with a
+per-point coefficient or ``w.c * u * v``, Q1/Q2 hexahedra): the
 validated pattern is reused without the exact-zero signature pass.  The
 values and the summation order are the first call's, so the CSR must be that
 call's bit for bit."""
@@ -13,7 +13,8 @@ from femhip.models import laplace, mass, kappa_laplace
 pytestmark = pytest.mark.gpu
 
 ELEM = {'tri1': femhip.ElementTriP1, 'tri2': femhip.ElementTriP2,
-        'tet1': femhip.ElementTetP1, 'tet2': femhip.ElementTetP2}
+        'tet1': femhip.ElementTetP1, 'tet2': femhip.ElementTetP2,
+        'hex1': femhip.ElementHex1, 'hex2': femhip.ElementHex2}
 
 
 @BilinearForm
@@ -23,14 +24,15 @@ def robin(u, v, w):
 
 def _basis(kind, elem, n):
     x = np.linspace(0, 1, n)
-    M = {'tri': femhip.MeshTri, 'tet': femhip.MeshTet}[kind]
+    M = {'tri': femhip.MeshTri, 'tet': femhip.MeshTet, 'hex': femhip.MeshHex}[kind]
     m = (M.init_tensor(x, x) if kind == 'tri' else M.init_tensor(x, x, x)).jittered(
         0.2, 1.0 / (n - 1))
     return femhip.Basis(m, ELEM[elem]())
 
 
 @pytest.mark.parametrize('kind,elem,n', [('tri', 'tri2', 33), ('tet', 'tet2', 9),
-                                         ('tri', 'tri1', 17), ('tet', 'tet1', 7)])
+                                         ('tri', 'tri1', 17), ('tet', 'tet1', 7),
+                                         ('hex', 'hex1', 6), ('hex', 'hex2', 4)])
 def test_second_call_bit_identical(kind, elem, n):
     print('case', kind, elem, n, flush=True)
     b = _basis(kind, elem, n)
@@ -40,7 +42,7 @@ def test_second_call_bit_identical(kind, elem, n):
     kap = rng.lognormal(0, 1, (E, 1) if elem.endswith('2') else (E, Q))
     alpha = 1.0 + x[0] ** 2 + 0.5 * x[1]
     calls = [(kappa_laplace, {'kappa': kap}), (robin, {'alpha': alpha})]
-    if elem.endswith('2'):     # P1 laplace / mass have the fused kernel
+    if not elem.startswith(('tri1', 'tet1')):   # P1 laplace / mass: fused kernel
         calls += [(laplace, {}), (mass, {})]
     for form, kw in calls:
         A1 = form.assemble(b, **kw)
