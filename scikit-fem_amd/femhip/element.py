@@ -19,6 +19,8 @@ that agree with the reference's generated table to rounding.
 """
 import numpy as np
 
+from .hex2_forms import HEX2_FORMS
+
 
 class Element:
     nodal_dofs = 0
@@ -197,18 +199,9 @@ HEX2_NODES = np.array([
     [.5, .5, .5]])
 
 
-def _quad1d(s, node):
-    """Quadratic Lagrange polynomial on {0, 1/2, 1} equal to 1 at ``node``,
-    and its derivative."""
-    if node == 0.0:
-        return (1. - s) * (1. - 2. * s), 4. * s - 3.
-    if node == 0.5:
-        return 4. * s * (1. - s), 4. - 8. * s
-    return s * (2. * s - 1.), 4. * s - 1.
-
-
 class ElementHex2(Element):
-    """Triquadratic hexahedron (27 nodes)."""
+    """Triquadratic hexahedron (27 nodes), evaluated in the reference's
+    operation order (``element_hex2.py:11-1260``; see ``hex2_forms``)."""
     nodal_dofs = edge_dofs = facet_dofs = interior_dofs = 1
     maxdeg, kind, dim, nbs = 6, 'hex', 3, 27
     doflocs = HEX2_NODES
@@ -217,11 +210,8 @@ class ElementHex2(Element):
         if not 0 <= i < 27:
             raise ValueError("Index %d out of range" % i)
         x, y, z = X
-        a, b, c = HEX2_NODES[i]
-        lx, dx = _quad1d(x, a)
-        ly, dy = _quad1d(y, b)
-        lz, dz = _quad1d(z, c)
-        return lx * ly * lz, np.array([dx * ly * lz, lx * dy * lz, lx * ly * dz])
+        v, g = HEX2_FORMS[i](x, y, z)
+        return v, np.array(g)
 
 
 class ElementVector(Element):

@@ -378,6 +378,8 @@ if __name__ == '__main__':
                          ['laplace', 'kappa_laplace']),
             'vhex2_n11j': ('hex', 11, 0.15, 'hex2', True, 5,
                            ['linear_elasticity']),
+            'vhex2_n21j': ('hex', 21, 0.15, 'hex2', True, 5,
+                           ['linear_elasticity']),
         }
         for k, v in big.items():
             if args.only in (None, k):

@@ -53,11 +53,6 @@ def _assemble(form, b, z):
     raise KeyError(form)
 
 
-# Patterns that depend on the last bit of the reference's generated Q2
-# polynomial table (element_hex2.py:11-1210), which this build does not copy.
-Q2_TABLE_DEPENDENT = {('vhex2_n3', 'linear_elasticity')}
-
-
 @pytest.mark.parametrize('path', _fixtures(), ids=lambda p: os.path.basename(p)[6:-4])
 def test_small_fixture(path):
     z = np.load(path)
@@ -72,8 +67,6 @@ def test_small_fixture(path):
             ref = z['unit_load_b']
             assert r.shape == ref.shape
             assert np.abs(r - ref).max() <= TOL * np.abs(ref).max()
-            continue
-        if (name, form) in Q2_TABLE_DEPENDENT:
             continue
         assert r.indptr.dtype == np.int32 and r.indices.dtype == np.int32
         assert r.has_sorted_indices
