@@ -71,32 +71,146 @@ def measured_traffic(kernel_name, n):
     return int(m.group(1)), 'profiles/' + os.path.basename(path)
 
 
-def cpu_baseline(n_sample=81):
-    """The oracle (NumPy restatement of the reference algorithm) on a
-    bounded sample: Basis + laplace.assemble on the 81^3 jittered mesh, with
-    the reference's thread knob set to the host's cores (nthreads, the
-    (j, i) entries split over Python threads, bilinear_form.py:100-119) and
-    with its default (nthreads=0, one thread)."""
+def _median_s(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline(n_sample=61, reps=3):
+    """The oracle (NumPy/SciPy restatement of the reference algorithm) on a
+    bounded sample of the headline workload: MeshTet 61^3 jittered
+    (1,296,000 tets) -- Basis construction, laplace, mass and unit_load,
+    each the median of ``reps`` runs -- with the reference's thread knob at
+    the host's cores (nthreads, the (j, i) entries split over Python
+    threads, bilinear_form.py:100-119) and with its default (nthreads=0)."""
     from oracle import skfem_np as O
     m = build_mesh(n_sample, 0.2)
     om = O.MeshTet(m.p, m.t)
     E = m.nelements
     cores = len(os.sched_getaffinity(0))
-    t0 = time.perf_counter()
-    ob = O.CellBasis(om, O.ElementTetP1())
-    t1 = time.perf_counter()
-    O.bilinear_assemble(O.form_laplace, ob, nthreads=cores)
-    t2 = time.perf_counter()
-    O.bilinear_assemble(O.form_laplace, ob)
-    t3 = time.perf_counter()
-    return {"value": E / (t2 - t0), "unit": "elements/s", "cores": cores,
+    holder = {}
+
+    def basis():
+        holder['b'] = O.CellBasis(om, O.ElementTetP1())
+    t_basis = _median_s(basis, reps)
+    ob = holder['b']
+    out = {}
+    for nth in (cores, 0):
+        tl = _median_s(lambda: O.bilinear_assemble(O.form_laplace, ob, nthreads=nth), reps)
+        tm = _median_s(lambda: O.bilinear_assemble(O.form_mass, ob, nthreads=nth), reps)
+        tu = _median_s(lambda: O.linear_assemble(O.form_unit_load, ob), reps)
+        out[nth] = (tl, tm, tu)
+    tl, tm, tu = out[cores]
+    tl0, tm0, tu0 = out[0]
+    return {"value": E / (t_basis + tl + tm + tu), "unit": "elements/s", "cores": cores,
             "kind": "port",
-            "sample": "oracle Basis+laplace on MeshTet 81^3 (%d tets), jittered; "
-                      "basis %.2fs, assembly %.2fs with nthreads=%d (NumPy/SciPy)"
-                      % (E, t1 - t0, t2 - t1, cores),
-            "single_thread_value": E / ((t1 - t0) + (t3 - t2)),
-            "single_thread_assembly_s": t3 - t2,
+            "sample": "oracle on MeshTet %d^3 jittered (%d tets): Basis %.2fs, laplace %.2fs, "
+                      "mass %.2fs, unit_load %.2fs with nthreads=%d (median of %d; NumPy/SciPy)"
+                      % (n_sample, E, t_basis, tl, tm, tu, cores, reps),
+            "single_thread_value": E / (t_basis + tl0 + tm0 + tu0),
+            "single_thread_s": {"basis": t_basis, "laplace": tl0, "mass": tm0, "unit_load": tu0},
             "host_cpus": os.cpu_count(), "host_affinity": cores}
+
+
+def cpu_configs():
+    """One-time timings of the oracle on every benchmark configuration (the
+    reference's thread knob at the host's cores), progress on stderr:
+    P1 triangles 129^2 (laplace + unit_load), P1 tetrahedra 101^3 jittered
+    (laplace, mass, unit_load; median of 3), P2 tetrahedra 41^3 jittered,
+    vector Q2 hexahedral elasticity 20^3 jittered (the 40^3 case is 8x the
+    elements), kappa-Laplace on P1 tetrahedra 65^3, one kappa sample on P2
+    triangles 257^2 and the 64 P1 triangle meshes (per-mesh p50/p99)."""
+    from oracle import skfem_np as O
+    cores = len(os.sched_getaffinity(0))
+    res = {"cores": cores}
+
+    def tick(name):
+        def f(j, i):
+            if i == 0:
+                log('  %s: test functions of trial %d done' % (name, j))
+        return f
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        return time.perf_counter() - t0, r
+    # config 1
+    x = np.linspace(0, 1, 129)
+    m = O.MeshTri.init_tensor(x, x)
+    tb, b = timed(lambda: O.CellBasis(m, O.ElementTriP1()))
+    tl, _ = timed(lambda: O.bilinear_assemble(O.form_laplace, b, nthreads=cores))
+    tu, _ = timed(lambda: O.linear_assemble(O.form_unit_load, b))
+    res["tri1_n129"] = {"elements": b.nelems, "basis_s": tb, "laplace_s": tl, "unit_load_s": tu}
+    log('cpu config tri1_n129 done')
+    # config 2a
+    mm = build_mesh(101, 0.2)
+    m = O.MeshTet(mm.p, mm.t)
+    tb, b = timed(lambda: O.CellBasis(m, O.ElementTetP1()))
+    r = {"elements": b.nelems, "basis_s": tb}
+    for name, f in (('laplace', O.form_laplace), ('mass', O.form_mass)):
+        r[name + '_s'] = _median_s(lambda: O.bilinear_assemble(f, b, nthreads=cores), 3)
+        log('cpu config tet1_n101j', name, 'done')
+    r['unit_load_s'] = _median_s(lambda: O.linear_assemble(O.form_unit_load, b), 3)
+    res["tet1_n101j"] = r
+    del b
+    # config 2b
+    x = np.linspace(0, 1, 41)
+    m = O.jitter(O.MeshTet.init_tensor(x, x, x), 0.2, 1 / 40)
+    tb, b = timed(lambda: O.CellBasis(m, O.ElementTetP2(), intorder=4))
+    tl, _ = timed(lambda: O.bilinear_assemble(O.form_laplace, b, nthreads=cores,
+                                              progress=tick('tet2_n41j')))
+    res["tet2_n41j"] = {"elements": b.nelems, "basis_s": tb, "laplace_s": tl}
+    log('cpu config tet2_n41j done')
+    del b
+    # config 3 at 20^3
+    x = np.linspace(0, 1, 21)
+    m = O.jitter(O.MeshHex.init_tensor(x, x, x), 0.15, 1 / 20)
+    lam, mu = O.lame_parameters(1.0, 0.3)
+    tb, b = timed(lambda: O.CellBasis(m, O.ElementVector(O.ElementHex2()), intorder=5))
+    tl, _ = timed(lambda: O.bilinear_assemble(O.form_elasticity(lam, mu), b, nthreads=cores,
+                                              progress=tick('vhex2_n21j')))
+    res["vhex2_n21j"] = {"elements": b.nelems, "basis_s": tb, "elasticity_s": tl}
+    log('cpu config vhex2_n21j done')
+    del b
+    # config 4
+    x = np.linspace(0, 1, 65)
+    m = O.MeshTet.init_tensor(x, x, x)
+    kap = np.random.default_rng(0).lognormal(0, 1, m.t.shape[1])
+    tb, b = timed(lambda: O.CellBasis(m, O.ElementTetP1()))
+    kq = np.repeat(kap[:, None], b.dx.shape[1], axis=1)
+    tl, _ = timed(lambda: O.bilinear_assemble(O.form_kappa_laplace, b, nthreads=cores, kappa=kq))
+    res["tet1_n65_kappa"] = {"elements": b.nelems, "basis_s": tb, "kappa_laplace_s": tl}
+    log('cpu config tet1_n65 done')
+    del b
+    # config 5a: one sample
+    x = np.linspace(0, 1, 257)
+    m = O.MeshTri.init_tensor(x, x)
+    tb, b = timed(lambda: O.CellBasis(m, O.ElementTriP2()))
+    kap = np.random.default_rng(0).lognormal(0, 0.5, m.t.shape[1])
+    kq = np.repeat(kap[:, None], b.dx.shape[1], axis=1)
+    tl, _ = timed(lambda: O.bilinear_assemble(O.form_kappa_laplace, b, nthreads=cores, kappa=kq))
+    res["tri2_n257_sample"] = {"elements": b.nelems, "basis_s": tb, "kappa_laplace_s": tl,
+                               "x256_s": 256 * tl}
+    log('cpu config tri2_n257 done')
+    del b
+    # config 5b
+    per = []
+    for n in np.random.default_rng(0).integers(17, 258, 64):
+        x = np.linspace(0, 1, int(n))
+        m = O.MeshTri.init_tensor(x, x)
+        t0 = time.perf_counter()
+        b = O.CellBasis(m, O.ElementTriP1())
+        O.bilinear_assemble(O.form_laplace, b)
+        O.linear_assemble(O.form_unit_load, b)
+        per.append(time.perf_counter() - t0)
+    res["mesh_set_64"] = {"total_s": float(np.sum(per)), "p50_s": _pct(per, 50),
+                          "p99_s": _pct(per, 99)}
+    log('cpu config mesh_set_64 done')
+    return res
 
 
 def _pct(a, q):
@@ -220,6 +334,54 @@ def bench_adjoint(n=65):
             "forward_frac_of_8TBps": fwd_bytes / (ms_f * 1e-3) / HBM_PEAK,
             "adjoint_alg_bytes": bwd_bytes,
             "adjoint_frac_of_8TBps": bwd_bytes / (ms_b * 1e-3) / HBM_PEAK}
+
+
+def bench_changed_inputs(basis, calls=10):
+    """The drop-in ``assemble_device`` on the 101^3 basis with a new input on
+    every call, each call validated in its own launch sequence (fused pass,
+    recheck of the entries within the rounding bound of zero, verdict read
+    back): a new per-element kappa ~ lognormal(0, 1) (kappa-Laplace), and
+    new interior-node coordinates (jitter U(-0.2h, 0.2h), a new seed per
+    call; Laplace).  Wall time per call including the verdict's
+    synchronisation; the fraction of calls that stayed on the fused path."""
+    import torch
+    from femhip.models import laplace, kappa_laplace
+    d = basis.device()
+    m = basis.mesh
+    E = m.nelements
+    rng = np.random.default_rng(21)
+    kappas = [torch.tensor(rng.lognormal(0, 1, E), device='cuda')[:, None]
+              for _ in range(calls + 1)]
+    x = np.linspace(0, 1, int(round(m.p.shape[1] ** (1 / 3))))
+    base = type(m).init_tensor(x, x, x)
+    I = base.interior_nodes()
+    h = 1.0 / (len(x) - 1)
+    coords = []
+    for _ in range(calls + 1):
+        p = base.p.copy()
+        p[:, I] += rng.uniform(-0.2 * h, 0.2 * h, size=(3, len(I)))
+        coords.append(torch.tensor(p, device='cuda'))
+    p0 = d.p
+    res = {}
+    for name, run in (('new_kappa', lambda k: kappa_laplace.assemble_device(basis, kappa=kappas[k])),
+                      ('new_coordinates', None)):
+        if name == 'new_coordinates':
+            def run(k):
+                d.set_coordinates(coords[k])
+                return laplace.assemble_device(basis)
+        run(calls)                       # first call of the form: exact path
+        torch.cuda.synchronize()
+        fused = 0
+        t0 = time.perf_counter()
+        for k in range(calls):
+            run(k)
+            fused += int(d.last_verdict is not None and d.last_verdict[0] == 1)
+        torch.cuda.synchronize()
+        res[name + '_ms'] = (time.perf_counter() - t0) * 1e3 / calls
+        res[name + '_fused_fraction'] = fused / calls
+    d.set_coordinates(p0)
+    laplace.assemble_device(basis)
+    return res
 
 
 def bench_device_solve(basis, A):
@@ -414,17 +576,24 @@ def main():
     ap.add_argument('--n', type=int, default=101)
     ap.add_argument('--path', default='auto', choices=['auto', 'generic', 'fused'])
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--cpu-configs', action='store_true',
+                    help='time the oracle on every benchmark configuration (minutes; '
+                         'prints its own JSON line, no GPU work)')
     ap.add_argument('--no-extra', action='store_true',
                     help='skip the batched / adjoint / elasticity measurements')
     ap.add_argument('--extra', default=None,
                     help='comma-separated subset of the extra measurements '
-                         '(interpolate, facet, device_solve, batched_kappa, '
+                         '(changed, interpolate, facet, device_solve, batched_kappa, '
                          'mesh_batch, adjoint, elasticity, p2_tet)')
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.cpu_configs:
+        print(json.dumps({"cpu_configs": cpu_configs()}))
+        return
 
     import torch
     import femhip
@@ -459,9 +628,7 @@ def main():
 
         def step():
             fp.assemble('laplace', dvb.p, out)
-        kernel_name = ('p1_sweep_kernel' if getattr(fp, 'kind', '') == 'sweep'
-                       else 'p1_fused_fx_kernel' if getattr(fp, 'accum', '') == 'fixed'
-                       else 'p1_fused_kernel')
+        kernel_name = 'p1_fused_kernel'
     else:
         plan = next(iter(dvb.plans.values()))
         contrib = torch.empty(16 * E, dtype=torch.float64, device=dev)
@@ -528,6 +695,7 @@ def main():
     del src, dst
     if not args.no_extra:
         extras = (
+            ('changed', "changed_inputs_tet101", lambda: bench_changed_inputs(basis)),
             ('interpolate', "interpolate_tet101", lambda: bench_interpolate(basis)),
             ('facet', "facet_mass_tet101_boundary", lambda: bench_facet(m)),
             ('device_solve', "device_solve_tet101", lambda: bench_device_solve(basis, A)),
@@ -548,6 +716,10 @@ def main():
                 res[name] = {"error": "%s: %s" % (type(exc).__name__, exc)}
             torch.cuda.synchronize()
             torch.cuda.empty_cache()
+    ch = res.get("changed_inputs_tet101")
+    if isinstance(ch, dict) and "new_kappa_ms" in ch:
+        ch["new_kappa_over_headline"] = ch["new_kappa_ms"] / ms
+        ch["new_coordinates_over_headline"] = ch["new_coordinates_ms"] / ms
     if not args.no_cpu:
         log('cpu baseline ...')
         res["cpu_baseline"] = cpu_baseline()

@@ -231,12 +231,17 @@ int femhip_rows_plan_build(const int32_t* row_dofs, int32_t nb_row,
                            int32_t* seg, void* workspace, size_t ws_bytes,
                            femhip_stream_t s);
 
-/* Signature of the exact-zero positions of contrib[n]: out[0] = number of
- * entries equal to 0.0, out[1] = wrapping sum of a 64-bit mix of their
- * indices (device int64[2]).  A plan built from one contrib array is valid
- * for another one iff the signatures agree (up to hash collisions). */
-int femhip_zero_signature(const double* contrib, int64_t n, int64_t* out,
-                          femhip_stream_t s);
+/* Exact-zero mask of contrib[n] (the contributions eliminate_zeros drops,
+ * coo_data.py:34): bit (c & 63) of mask[c >> 6] is set iff contrib[c] ==
+ * 0.0; mask holds ceil(n / 64) words (device).  A plan built from one
+ * contrib array serves another one exactly when their masks are equal. */
+int femhip_zero_mask(const double* contrib, int64_t n, uint64_t* mask,
+                     femhip_stream_t s);
+/* *count (device int64) <- number of contributions of contrib[n] whose
+ * zero bit differs from mask (0: the same CSR pattern). */
+int femhip_zero_mask_mismatch(const double* contrib, int64_t n,
+                              const uint64_t* mask, int64_t* count,
+                              femhip_stream_t s);
 
 /* Numeric phase of the generic path: out[k] = sum of contrib[perm[j]] for
  * j in [seg[k], seg[k+1]), summed in perm order (deterministic). */
@@ -246,27 +251,28 @@ int femhip_segment_sum(const double* contrib, const int32_t* perm,
 
 /* ---------------------------------------------------------------------
  * Fused P1 path (triangles / tetrahedra; laplace, mass, kappa_laplace).
- * One kernel replaces BilinearForm._assemble + COOData._assemble_scipy_csr
+ * One pass replaces BilinearForm._assemble + COOData._assemble_scipy_csr
  * (bilinear_form.py:58-148, coo_data.py:28-36) once the CSR pattern is
- * known: every workgroup owns a set of rows (a Morton-ordered node
- * partition), recomputes the local matrices of the elements touching them
- * in registers, sums the contributions in LDS and writes each CSR row
- * once.  Two deterministic summation modes: colour rounds (floating point,
- * a fixed order) or fixed-point integers (LDS integer atomics, exact
- * associativity; error <= n * 3.6e-15 * max|A| for a slot with n
- * contributions) -- chosen by femhip_fused_dev_t.scale.
+ * known: every workgroup owns a set of rows (a spatial node partition),
+ * recomputes the local matrices of the elements touching them in
+ * registers, sums the contributions in LDS as fixed-point integers (one
+ * exponent per launch, 2^46 <= max|contribution| 2^s < 2^50; error <=
+ * n * 7.2e-15 * max|contribution| for a slot with n contributions;
+ * deterministic) and writes each CSR row once.
  *
  * femhip_fused_plan_build runs on the host (host pointers: t, p, indptr,
  * indices of the pattern) and keeps the plan in library-owned host memory
  * until femhip_fused_plan_free; femhip_fused_plan_export copies its arrays
  * into caller buffers sized from the info struct:
- *   parts[8*nparts] nodemap[nlocal] rows[2*nrows] chunks[2*nchunks]
- *   recs[rec_words*nrecs] rec_elem[nrecs].
+ *   parts[8*nparts] nodemap[nlocal] rows[2*nrows] recs[rec_words*nrecs]
+ *   rec_elem[nrecs].
+ * order: 0 = records of a partition in greedy colour classes, 1 = in
+ * element order.
  * --------------------------------------------------------------------- */
 typedef struct femhip_fused_plan_info {
-  int32_t nparts, nchunks, nrows, max_slots;
+  int32_t nparts, max_recs, nrows, max_slots;
   int32_t max_nodes, max_rows, rec_words, idbits;
-  int32_t offbits, max_colors, block, reserved;
+  int32_t offbits, block, reserved0, reserved1;
   int64_t nrecs, nlocal;
   double dup;           /* element visits / elements (halo recomputation)  */
 } femhip_fused_plan_info_t;
@@ -274,9 +280,10 @@ typedef struct femhip_fused_plan_info {
 int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t nelems, int64_t nnodes,
                             const int32_t* t, const double* p, const int32_t* indptr,
                             const int32_t* indices, int32_t slot_cap, int32_t node_cap,
-                            int32_t block, void** handle, femhip_fused_plan_info_t* info);
+                            int32_t block, int32_t order, void** handle,
+                            femhip_fused_plan_info_t* info);
 int femhip_fused_plan_export(void* handle, int32_t* parts, int32_t* nodemap, int32_t* rows,
-                             int32_t* chunks, uint32_t* recs, int32_t* rec_elem);
+                             uint32_t* recs, int32_t* rec_elem);
 void femhip_fused_plan_free(void* handle);
 
 /* The exported plan in device memory. */
@@ -284,30 +291,66 @@ typedef struct femhip_fused_dev {
   const int32_t* parts;
   const int32_t* nodemap;
   const int32_t* rows;
-  const int32_t* chunks;
   const uint32_t* recs;
   const int32_t* rec_elem;
-  /* Fixed-point accumulation state, owned by the kernel: 6*nparts int32,
-   * an exponent pair per form and partition, every entry set to INT32_MIN
-   * before first use.  NULL selects the colour-round kernel. */
-  int32_t* scale;
   femhip_fused_plan_info_t info;
 } femhip_fused_dev_t;
 
+/* Accumulation and validation state of one form on one plan, in device
+ * memory (zeroed before first use, then owned by the kernels; calls that
+ * share a state must not overlap). */
+enum { FEMHIP_FUSED_BAD = 1, FEMHIP_FUSED_OVERFLOW = 2 };
+typedef struct femhip_fused_state {
+  int32_t s, sv;         /* fixed-point exponents of the last pass (values, load) */
+  int32_t valid;         /* s, sv have been measured                          */
+  int32_t redo;          /* the last first pass ran with other exponents      */
+  uint64_t maxk, maxv;   /* maxima of the running pass (bit patterns of doubles >= 0) */
+  int32_t flags;         /* FEMHIP_FUSED_BAD | FEMHIP_FUSED_OVERFLOW           */
+  int32_t reserved0;
+  uint32_t reserved1[8];
+  int64_t nsusp;         /* suspects queued by the last first pass            */
+  int64_t mismatch, zeros; /* recheck counters                                */
+  int32_t reserved[4];
+} femhip_fused_state_t;
+
+/* Validation of a pass against the pattern the plan was built for (the
+ * changed-input check): entries within the rounding bound of zero are
+ * queued and recomputed in the reference's arithmetic; the pass is valid
+ * iff exactly the plan's zero contributions (its femhip_zero_mask, nzeros
+ * set bits) are zero.  verdict (device int32[4]) <- {valid, mismatches,
+ * zeros found, flags}.  queue: device uint64[capacity] scratch. */
+typedef struct femhip_fused_check {
+  const int32_t* t;        /* connectivity [nverts][nelems] (reference layout) */
+  const double* dphi;      /* reference gradients [nverts][dim][Q]             */
+  const double* W;         /* quadrature weights [Q]                           */
+  int64_t nelems;
+  int32_t Q, reserved;
+  const uint64_t* zero_mask;
+  int64_t nzeros;
+  uint64_t* queue;
+  int64_t capacity;
+  int32_t* verdict;
+} femhip_fused_check_t;
+
 /* LDS bytes per workgroup that femhip_fused_assemble asks for. */
 size_t femhip_fused_lds_bytes(const femhip_fused_plan_info_t* info, int32_t dim,
-                              int32_t with_load, int32_t fixed_point);
+                              int32_t with_load);
 
 /* data[nnz] <- assembled CSR values of `form` (FEMHIP_LAPLACE, FEMHIP_MASS,
  * FEMHIP_KAPPA_LAPLACE with kappa[nelems] per element) on the pattern the
  * plan was built for; if load != NULL also load[nnodes] <- the unit-load
  * vector (LinearForm unit_load, linear_form.py:18-49) in the same pass.
- * p is [dim][nnodes] on the device.  groups = number of persistent
- * workgroups (0: one per partition).  A plan may concatenate several meshes
+ * p is [dim][nnodes] on the device.  Two launches (the second repeats the
+ * pass only when the first measured another fixed-point exponent), then,
+ * with check != NULL, the recheck of the suspects (verdict on the device).
+ * groups = persistent workgroups taking partitions from work counters (0:
+ * one workgroup per partition).  A plan may concatenate several meshes
  * (femhip.batch): then p, data and load are the concatenated arrays. */
-int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t form, int32_t dim,
-                          const double* p, int64_t nnodes, const double* kappa,
-                          double* data, double* load, int32_t groups, femhip_stream_t s);
+int femhip_fused_assemble(const femhip_fused_dev_t* plan, femhip_fused_state_t* state,
+                          int32_t form, int32_t dim, const double* p, int64_t nnodes,
+                          const double* kappa, double* data, double* load,
+                          const femhip_fused_check_t* check, int32_t groups,
+                          femhip_stream_t s);
 
 /* out[S][nseg] <- kappa-weighted segmented sums  sum_k kappa[s][c_k % E] *
  * K1[c_k]  over the plan (perm, seg): many per-element coefficient samples
@@ -317,46 +360,6 @@ int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t form, int32_t 
 int femhip_batched_kappa_values(const double* K1, const int32_t* perm, const int32_t* seg,
                                 int64_t nseg, int64_t nelems, const double* kappa,
                                 int64_t nsamples, double* out, femhip_stream_t s);
-
-/* ---------------------------------------------------------------------
- * Fused P1 path, layer-sweep variant: the same computation as
- * femhip_fused_assemble with blocks of several thousand rows assembled
- * level by level (breadth-first levels of the node graph), two levels of
- * rows and coordinates resident in LDS at a time.  Plan arrays:
- *   blocks[4*nblocks] levels[6*nlevels] nodemap[nlocal] rows[2*nrows]
- *   chunks[2*nchunks] recs[rec_words*nrecs] rec_elem[nrecs].
- * --------------------------------------------------------------------- */
-typedef struct femhip_sweep_plan_info {
-  int32_t nblocks, nlevels, nchunks, nrows;
-  int32_t max_level_nodes, max_level_rows, max_level_slots, max_colors;
-  int32_t rec_words, offbits, chunk, reserved;
-  int64_t nrecs, nlocal;
-  double dup;
-} femhip_sweep_plan_info_t;
-
-typedef struct femhip_sweep_dev {
-  const int32_t* blocks;
-  const int32_t* levels;
-  const int32_t* nodemap;
-  const int32_t* rows;
-  const int32_t* chunks;
-  const uint32_t* recs;
-  const int32_t* rec_elem;
-  femhip_sweep_plan_info_t info;
-} femhip_sweep_dev_t;
-
-int femhip_sweep_plan_build(int32_t nverts, int32_t dim, int64_t nelems, int64_t nnodes,
-                            const int32_t* t, const double* p, const int32_t* indptr,
-                            const int32_t* indices, int32_t block_rows, int32_t chunk,
-                            void** handle, femhip_sweep_plan_info_t* info);
-int femhip_sweep_plan_export(void* handle, int32_t* blocks, int32_t* levels, int32_t* nodemap,
-                             int32_t* rows, int32_t* chunks, uint32_t* recs, int32_t* rec_elem);
-void femhip_sweep_plan_free(void* handle);
-size_t femhip_sweep_lds_bytes(const femhip_sweep_plan_info_t* info, int32_t dim,
-                              int32_t with_load);
-int femhip_sweep_assemble(const femhip_sweep_dev_t* plan, int32_t form, int32_t dim,
-                          const double* p, int64_t nnodes, const double* kappa, double* data,
-                          double* load, int32_t groups, femhip_stream_t s);
 
 /* Dirichlet rows on a device CSR (utils.enforce, skfem/utils.py:327-400):
  * rows D (distinct) are zeroed, their diagonal set to `diag`, b[D] = x[D]

@@ -907,12 +907,13 @@ def form_coef_load(v, w):
 # ---------------------------------------------------------------------------
 # assembly (skfem/assembly/form/*.py)
 # ---------------------------------------------------------------------------
-def bilinear_coo(form, basis, nthreads=0, **w):
+def bilinear_coo(form, basis, nthreads=0, progress=None, **w):
     """``BilinearForm._assemble``, ``bilinear_form.py:58-128``: the data is
     laid out ``(j, i, e)`` C-order, rows from the test (i), cols from the
     trial (j) basis.  ``nthreads > 0``: the (j, i) entries are split over
     Python threads as the reference does (``bilinear_form.py:100-119``); the
-    values do not change."""
+    values do not change.  ``progress(j, i)``, when given, is called after
+    each entry (long timing runs report their progress through it)."""
     nt, nb = basis.nelems, basis.Nbfun
     data = np.zeros((nb, nb, nt))
     rows = np.zeros(nb * nb * nt, dtype=np.int32)
@@ -921,6 +922,8 @@ def bilinear_coo(form, basis, nthreads=0, **w):
     def entry(j, i):
         data[j, i, :] = np.sum(form(basis.basis[j], basis.basis[i], w)
                                * basis.dx, axis=1)
+        if progress is not None:
+            progress(j, i)
 
     for j in range(nb):
         for i in range(nb):
@@ -948,9 +951,9 @@ def coo_to_csr(indices, data, shape):
     return K.tocsr()
 
 
-def bilinear_assemble(form, basis, nthreads=0, **w):
+def bilinear_assemble(form, basis, nthreads=0, progress=None, **w):
     """``BilinearForm.assemble``, ``bilinear_form.py:130-148``."""
-    return coo_to_csr(*bilinear_coo(form, basis, nthreads=nthreads, **w))
+    return coo_to_csr(*bilinear_coo(form, basis, nthreads=nthreads, progress=progress, **w))
 
 
 def linear_assemble(form, basis, **w):

@@ -208,3 +208,181 @@ extern "C" int femhip_local_linear(const femhip_geom_t* g, const femhip_space_t*
               sp->nbfun, coef ? f->kappa_mode : 0, 0.0, 0.0};
   return dispatch_load(g->kind, g->nq, a, reinterpret_cast<hipStream_t>(s));
 }
+
+// ---------------------------------------------------------------------------
+// recheck of the fused path's suspects
+// ---------------------------------------------------------------------------
+#include "fused_check.h"
+
+namespace femhip {
+namespace {
+
+struct RecheckArgs {
+  const femhip_fused_check_t* c;   // unused on the device
+  const int32_t* t;
+  const double* p;
+  const double* dphi;
+  const double* W;
+  const double* kappa;
+  const uint64_t* queue;
+  const uint64_t* mask;
+  int64_t cap;
+  int64_t E;
+  int64_t N;
+  int64_t nzeros;
+  femhip_fused_state_t* st;
+  int32_t* verdict;
+};
+
+constexpr int kCheckBlocks = 128;
+
+// Entry (e, i, j) of the P1 Laplace / kappa-Laplace matrix in the
+// reference's arithmetic (the same operations as local_row_kernel: the
+// affine inverse of mapping_affine.py:55-131, einsum-ordered gradients,
+// (kappa grad u) . grad v summed from 0, times dx, NumPy's pairwise sum over
+// the points) compared with 0.0; counts the zero entries and those whose
+// zero-ness differs from the plan's mask bit of contribution
+// (j * nb + i) * E + e (fused_recheck_verdict_kernel then forms the
+// verdict).
+template <int D, int Q, int FK>
+__global__ void __launch_bounds__(kBlock) fused_recheck_kernel(RecheckArgs a) {
+#pragma clang fp contract(off)
+  constexpr int NB = D + 1;
+  __shared__ unsigned long long sm[kBlock / 64], sz[kBlock / 64];
+  femhip_fused_state_t* st = a.st;
+  const int64_t nq = st->nsusp;
+  const int64_t n = nq < a.cap ? nq : a.cap;
+  unsigned long long mism = 0, zeros = 0;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * kBlock) {
+    const uint64_t q = a.queue[k];
+    const int64_t e = (int64_t)(uint32_t)q;
+    const int i = (int)((q >> 32) & 0xffu), j = (int)((q >> 40) & 0xffu);
+    double inv[D][D];
+    const double adet = fabs(exact::affine_inverse<D>(a.p, a.t, a.N, a.E, e, inv));
+    const double kap = FK == FEMHIP_KAPPA_LAPLACE ? a.kappa[e] : 1.0;
+    auto term = [&](int qq) -> double {
+#pragma clang fp contract(off)
+      const double dx = adet * a.W[qq];
+      double gi[D], gj[D];
+      exact::global_grad<D>(inv, a.dphi, i, qq, Q, gi);
+      exact::global_grad<D>(inv, a.dphi, j, qq, Q, gj);
+      double s = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if constexpr (FK == FEMHIP_KAPPA_LAPLACE)
+          s = s + (kap * gj[d]) * gi[d];
+        else
+          s = s + gj[d] * gi[d];
+      }
+      return s * dx;
+    };
+    const double r = 0.0 + np_pairwise<0, Q>(term);
+    const bool zero = r == 0.0;
+    const uint64_t c = (uint64_t)((int64_t)(j * NB + i) * a.E + e);
+    const bool bit = (a.mask[c >> 6] >> (c & 63)) & 1ull;
+    mism += zero != bit ? 1 : 0;
+    zeros += zero ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mism += __shfl_xor(mism, o, 64);
+    zeros += __shfl_xor(zeros, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sm[threadIdx.x >> 6] = mism;
+    sz[threadIdx.x >> 6] = zeros;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < kBlock / 64; ++w) {
+    mism += sm[w];
+    zeros += sz[w];
+  }
+  if (mism) atomicAdd(reinterpret_cast<unsigned long long*>(&st->mismatch), mism);
+  if (zeros) atomicAdd(reinterpret_cast<unsigned long long*>(&st->zeros), zeros);
+}
+
+// after the recheck: the verdict, and the queue counters and flags cleared
+// for the next pass
+__global__ void fused_recheck_verdict_kernel(femhip_fused_state_t* st, int64_t nzeros,
+                                             int32_t* verdict) {
+  const int64_t M = st->mismatch, Z = st->zeros;
+  const int32_t flags = st->flags;
+  verdict[0] = (flags == 0 && M == 0 && Z == nzeros) ? 1 : 0;
+  verdict[1] = (int32_t)(M < INT32_MAX ? M : INT32_MAX);
+  verdict[2] = (int32_t)(Z < INT32_MAX ? Z : INT32_MAX);
+  verdict[3] = flags;
+  st->nsusp = 0;
+  st->flags = 0;
+  st->mismatch = 0;
+  st->zeros = 0;
+}
+
+// the mass form has no zero entries on valid elements: the verdict only
+// needs the flags and an empty zero set
+__global__ void fused_verdict_kernel(femhip_fused_state_t* st, int64_t nzeros,
+                                     int32_t* verdict) {
+  const int32_t flags = st->flags;
+  const int64_t nq = st->nsusp;
+  verdict[0] = (flags == 0 && nq == 0 && nzeros == 0) ? 1 : 0;
+  verdict[1] = 0;
+  verdict[2] = 0;
+  verdict[3] = flags;
+  st->nsusp = 0;
+  st->flags = 0;
+}
+
+template <int D, int Q>
+int recheck_form(const RecheckArgs& a, int form, hipStream_t s) {
+  if (form == FEMHIP_LAPLACE)
+    fused_recheck_kernel<D, Q, FEMHIP_LAPLACE><<<kCheckBlocks, kBlock, 0, s>>>(a);
+  else
+    fused_recheck_kernel<D, Q, FEMHIP_KAPPA_LAPLACE><<<kCheckBlocks, kBlock, 0, s>>>(a);
+  FEMHIP_CHECK_LAUNCH();
+  fused_recheck_verdict_kernel<<<1, 1, 0, s>>>(a.st, a.nzeros, a.verdict);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+}  // namespace
+
+int fused_check_run(const femhip_fused_check_t* c, femhip_fused_state_t* st, int form, int dim,
+                    const double* p, int64_t nnodes, const double* kappa, hipStream_t s) {
+  FEMHIP_REQUIRE(c->t && c->dphi && c->W && c->zero_mask && c->nelems > 0, FEMHIP_EINVAL,
+                 "fused check: NULL table or no elements");
+  if (form == FEMHIP_MASS) {
+    fused_verdict_kernel<<<1, 1, 0, s>>>(st, c->nzeros, c->verdict);
+    FEMHIP_CHECK_LAUNCH();
+    return FEMHIP_OK;
+  }
+  FEMHIP_REQUIRE(form == FEMHIP_LAPLACE || form == FEMHIP_KAPPA_LAPLACE, FEMHIP_EUNSUPPORTED,
+                 "fused check: laplace, mass or kappa_laplace");
+  RecheckArgs a{c,        c->t,         p,  c->dphi,  c->W,      kappa, c->queue,
+                c->zero_mask, c->capacity, c->nelems, nnodes, c->nzeros, st, c->verdict};
+  if (dim == 2) {
+    switch (c->Q) {
+      case 1: return recheck_form<2, 1>(a, form, s);
+      case 3: return recheck_form<2, 3>(a, form, s);
+      case 4: return recheck_form<2, 4>(a, form, s);
+      case 6: return recheck_form<2, 6>(a, form, s);
+    }
+  } else {
+    switch (c->Q) {
+      case 1: return recheck_form<3, 1>(a, form, s);
+      case 4: return recheck_form<3, 4>(a, form, s);
+      case 5: return recheck_form<3, 5>(a, form, s);
+      case 11: return recheck_form<3, 11>(a, form, s);
+    }
+  }
+  set_error("fused check: unsupported number of quadrature points");
+  return FEMHIP_EUNSUPPORTED;
+}
+
+int fused_check_empty(const femhip_fused_check_t* c, femhip_fused_state_t* st, hipStream_t s) {
+  fused_verdict_kernel<<<1, 1, 0, s>>>(st, c->nzeros, c->verdict);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+}  // namespace femhip

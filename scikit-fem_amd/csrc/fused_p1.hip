@@ -1,44 +1,73 @@
-// Fused P1 assembly: one launch from node coordinates to CSR values.
+// Fused P1 assembly: one pass from node coordinates to CSR values.
 //
-// Workgroup = one row partition of the plan built by fused_plan.cpp.
-//   1. zero the partition's CSR rows in LDS, stage the coordinates of its
-//      local nodes (owned + halo) in LDS;
-//   2. per chunk of element records: every lane decodes one record and
-//      computes the element's local matrix in registers (the affine map,
-//      its determinant and the cross-product gradients, FP64);
-//   3. the chunk's colours are applied in rounds separated by barriers --
-//      within a round no two lanes touch the same owned row, so each LDS
-//      slot is summed in a fixed order (deterministic, no atomics);
-//   4. the rows are written to HBM once, 16 lanes per row, with
-//      non-temporal stores.
-// Bytes per element visit: one 8-16 byte record; per local node: one map
-// entry + its coordinates; per CSR slot: one 8-byte store.
+// Workgroup = one row partition of the plan built by fused_plan.cpp (or a
+// persistent grid walking the partitions).
+//   1. stage the coordinates of the partition's local nodes (owned + halo)
+//      and its row table in LDS, zero the rows' LDS accumulators;
+//   2. per chunk of element records (the next chunk's records load while
+//      this one is processed): every lane decodes one record and computes
+//      the element's local matrix in registers (edge vectors, cross-product
+//      gradients, one refined reciprocal of the determinant), and adds the
+//      off-diagonal entries of its owned rows into LDS;
+//   3. the rows are written to HBM once, 16 lanes per row, with
+//      non-temporal stores; for Laplace forms the diagonal is the negated
+//      sum of the row's off-diagonal entries (each element matrix has zero
+//      row sums), so diagonal contributions are never formed.
+//
+// Summation is deterministic without floating-point atomics: a
+// contribution v enters the LDS accumulators as the integer round(v * 2^s)
+// (int64, LDS integer atomics).  Integer addition is associative, so the
+// sums do not depend on the order in which lanes and waves arrive.  One
+// exponent s serves the whole launch: s = 49 - ilogb(M) rounded down to a
+// multiple of 4, M = the largest |contribution| of this input, so
+// 2^46 <= M 2^s < 2^50: no conversion leaves the 2^51 range of the
+// magic-number rounding, a slot of n contributions is within
+// n 2^-(s+1) <= n 7.2e-15 M of its exact sum, and sums stay below 2^62.
+// The pass uses the exponent the previous call measured; the last
+// workgroup to finish reduces this pass's M, and when its bucket differs
+// the companion launch (mode 1) repeats the pass with it -- otherwise that
+// launch returns at once.  The values are therefore a function of the
+// input alone.
+//
+// Validation of changed inputs (first-pass suspects): an off-diagonal
+// entry that the reference computes as exactly 0.0 (and eliminate_zeros
+// drops) comes out here within the rounding bound
+//   tau = 2^-43 a^(2d-2) / D (1 + 10 a^d / D) |coef|
+// of zero (a: the largest |edge component|, D = |det|; both arithmetics are
+// within half of it of the exact value), so every entry with |K_ij| <= tau
+// is queued as a suspect (element, row, column) and rechecked in the
+// reference's arithmetic afterwards; every other entry is nonzero in the
+// reference too.  Non-finite or vanishing element data sets a flag.
 #include <utility>
 
 #include "common.h"
+#include "fused_check.h"
 
 namespace femhip {
 namespace {
 
-constexpr int kFusedBlock = 512;
+constexpr double kFxMagic = 6755399441055744.0;  // 1.5 * 2^52
+static_assert(sizeof(femhip_fused_state_t) == 112, "femhip_fused_state_t layout");
 
-struct FusedArgs {
+struct FxArgs {
   const int32_t* parts;
   const int32_t* nodemap;
   const int32_t* rows;
-  const int32_t* chunks;
   const uint32_t* recs;
   const int32_t* rec_elem;
   const double* p;
   const double* kappa;
   double* out;
   double* load;      // optional: unit-load vector, one entry per node
-  int32_t* scale;    // fixed-point exponents per (form, partition), or NULL
+  femhip_fused_state_t* st;
+  uint64_t* susp;    // suspect queue (first pass), or NULL
+  int64_t susp_cap;
   int64_t nnodes;
   int32_t nparts;
   int32_t max_slots;
   int32_t max_nodes;
   int32_t max_rows;
+  int32_t mode;      // 0: first pass, 1: repeat with the measured exponent
 };
 
 template <int POS, int NB, int RW>
@@ -53,10 +82,9 @@ __device__ __forceinline__ uint32_t bits(const uint32_t (&w)[RW]) {
   }
 }
 
-// A field inside one word as one v_bfe_u32.  Written as inline assembly so
-// that a following shift by 3 (a slot index to a byte offset) folds into
-// the address add (v_lshl_add_u32) instead of being merged into a
-// shift-and-mask pair; fields that straddle two words use bits().
+// A field inside one word as one v_bfe_u32 (inline assembly, so that a
+// following shift by 3 folds into the address add); fields that straddle
+// two words use bits().
 template <int POS, int NB, int RW>
 __device__ __forceinline__ uint32_t field(const uint32_t (&w)[RW]) {
   constexpr int wi = POS >> 5, sh = POS & 31;
@@ -69,16 +97,10 @@ __device__ __forceinline__ uint32_t field(const uint32_t (&w)[RW]) {
   }
 }
 
-template <int BASE, int OFB, int RW, int... F>
+template <int BASE, int FB, int RW, int... F>
 __device__ __forceinline__ void decode_fields(const uint32_t (&w)[RW], uint32_t* o,
                                               std::integer_sequence<int, F...>) {
-  ((o[F] = field<BASE + F * OFB, OFB, RW>(w)), ...);
-}
-
-template <int BASE, int OFB, int RW, int... F>
-__device__ __forceinline__ void decode_offsets(const uint32_t (&w)[RW], uint32_t* o,
-                                               std::integer_sequence<int, F...>) {
-  ((o[F] = bits<BASE + F * OFB, OFB, RW>(w)), ...);
+  ((o[F] = field<BASE + F * FB, FB, RW>(w)), ...);
 }
 
 // Morton-adjacent partitions on the same XCD (workgroups are dealt to the
@@ -99,236 +121,6 @@ __device__ __forceinline__ double recip(double d) {
   return __fma_rn(r, e, r);
 }
 
-// Local matrix of a P1 simplex from its vertex coordinates in LDS: Laplace K_ab = coef * (n_a . n_b) / (d! |det|) with n_a the
-// cross-product (2-D: rotated-edge) gradients scaled by det; mass
-// |det| / (d! (d+1)(d+2)) * (1 + delta_ab).
-template <int D, int FORM>
-__device__ __forceinline__ double element_matrix(const double* X, const int (&lv)[D + 1],
-                                                 double coef, double (&K)[D + 1][D + 1]) {
-  // coordinates are staged AoS: 24 B (x y z) or 16 B (x y) per node
-  double x[D + 1][D];
-#pragma unroll
-  for (int v = 0; v < D + 1; ++v) {
-    if constexpr (D == 3) {
-      x[v][0] = X[3 * lv[v]];
-      x[v][1] = X[3 * lv[v] + 1];
-      x[v][2] = X[3 * lv[v] + 2];
-    } else {
-      const double2 xy = reinterpret_cast<const double2*>(X)[lv[v]];
-      x[v][0] = xy.x;
-      x[v][1] = xy.y;
-    }
-  }
-  if constexpr (D == 3) {
-    double c1[3], c2[3], c3[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      c1[d] = x[1][d] - x[0][d];
-      c2[d] = x[2][d] - x[0][d];
-      c3[d] = x[3][d] - x[0][d];
-    }
-    double n[4][3];
-    n[1][0] = c2[1] * c3[2] - c2[2] * c3[1];
-    n[1][1] = c2[2] * c3[0] - c2[0] * c3[2];
-    n[1][2] = c2[0] * c3[1] - c2[1] * c3[0];
-    n[2][0] = c3[1] * c1[2] - c3[2] * c1[1];
-    n[2][1] = c3[2] * c1[0] - c3[0] * c1[2];
-    n[2][2] = c3[0] * c1[1] - c3[1] * c1[0];
-    n[3][0] = c1[1] * c2[2] - c1[2] * c2[1];
-    n[3][1] = c1[2] * c2[0] - c1[0] * c2[2];
-    n[3][2] = c1[0] * c2[1] - c1[1] * c2[0];
-    const double det = c1[0] * n[1][0] + c1[1] * n[1][1] + c1[2] * n[1][2];
-    if constexpr (FORM == FEMHIP_MASS) {
-      const double m = fabs(det) * (1.0 / 120.0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
-    } else {
-      // vertices 1..3 directly; row and column 0 from the zero row sums
-      // (n_0 = -(n_1 + n_2 + n_3))
-      const double s = coef * recip(6.0 * fabs(det));
-#pragma unroll
-      for (int i = 1; i < 4; ++i)
-#pragma unroll
-        for (int j = i; j < 4; ++j) {
-          const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1] + n[i][2] * n[j][2]);
-          K[i][j] = v;
-          K[j][i] = v;
-        }
-#pragma unroll
-      for (int j = 1; j < 4; ++j) K[0][j] = K[j][0] = -(K[j][1] + K[j][2] + K[j][3]);
-      K[0][0] = -(K[0][1] + K[0][2] + K[0][3]);
-    }
-    return fabs(det);
-  } else {
-    const double c1x = x[1][0] - x[0][0], c1y = x[1][1] - x[0][1];
-    const double c2x = x[2][0] - x[0][0], c2y = x[2][1] - x[0][1];
-    const double det = c1x * c2y - c1y * c2x;
-    const double n[3][2] = {{0, 0}, {c2y, -c2x}, {-c1y, c1x}};
-    if constexpr (FORM == FEMHIP_MASS) {
-      const double m = fabs(det) * (1.0 / 24.0);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) K[i][j] = (i == j) ? 2.0 * m : m;
-    } else {
-      // vertices 1, 2 directly; row and column 0 from the zero row sums
-      const double s = coef * recip(2.0 * fabs(det));
-#pragma unroll
-      for (int i = 1; i < 3; ++i)
-#pragma unroll
-        for (int j = i; j < 3; ++j) {
-          const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1]);
-          K[i][j] = v;
-          K[j][i] = v;
-        }
-#pragma unroll
-      for (int j = 1; j < 3; ++j) K[0][j] = K[j][0] = -(K[j][1] + K[j][2]);
-      K[0][0] = -(K[0][1] + K[0][2]);
-    }
-    return fabs(det);
-  }
-}
-
-template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>
-__global__ void __launch_bounds__(kFusedBlock) p1_fused_kernel(FusedArgs a) {
-  constexpr int NV = D + 1;
-    constexpr uint32_t SENT = (1u << OFB) - 1u;
-  extern __shared__ double smem[];
-  double* acc = smem;
-  double* X = acc + ((a.max_slots + 3) & ~1);  // 16-byte aligned for b128 loads
-  double* bacc = X + D * a.max_nodes;
-  uint32_t* rinfo = reinterpret_cast<uint32_t*>(bacc + (LOAD ? a.max_rows : 0));
-
-  // persistent: each workgroup walks partitions blockIdx.x, +gridDim.x, ...
-  for (int pi = blockIdx.x; pi < a.nparts; pi += gridDim.x) {
-  if (pi != (int)blockIdx.x) __syncthreads();  // LDS reuse
-  const int part = xcd_remap(pi, a.nparts);
-  const int32_t* hdr = a.parts + 8 * (int64_t)part;
-  const int node_start = hdr[0], L = hdr[1], row_start = hdr[2], R = hdr[3];
-  const int nslots = hdr[4], chunk_start = hdr[5], nchunks = hdr[6];
-  const int tid = threadIdx.x;
-
-  for (int i = tid; i < nslots; i += kFusedBlock) acc[i] = 0.0;
-  for (int l = tid; l < L; l += kFusedBlock) {
-    const int64_t g = a.nodemap[node_start + l];
-#pragma unroll
-    for (int d = 0; d < D; ++d) X[D * l + d] = a.p[d * a.nnodes + g];
-  }
-  for (int r = tid; r < R; r += kFusedBlock) {
-    rinfo[r] = (uint32_t)a.rows[2 * (int64_t)(row_start + r) + 1];
-    if constexpr (LOAD) bacc[r] = 0.0;
-  }
-  __syncthreads();  // staged coordinates are read before the first round
-
-  // records of the next chunk are fetched while the current one is summed
-  uint32_t wn[RW];
-  int32_t rec_n = 0;
-  uint32_t cw_n = 0;
-  auto fetch = [&](int c) {
-    rec_n = a.chunks[2 * (int64_t)(chunk_start + c)];
-    cw_n = (uint32_t)a.chunks[2 * (int64_t)(chunk_start + c) + 1];
-    if (tid < (int)(cw_n & 0xffffu)) {
-      const uint32_t* rp = a.recs + (int64_t)(rec_n + tid) * RW;
-#pragma unroll
-      for (int k = 0; k < RW; ++k) wn[k] = __builtin_nontemporal_load(rp + k);
-    }
-  };
-  if (nchunks > 0) fetch(0);
-  double* const dummy = acc + a.max_slots;  // sink for pruned slots
-
-  for (int c = 0; c < nchunks; ++c) {
-    const int32_t rec0 = rec_n;
-    const uint32_t cw = cw_n;
-    const int count = (int)(cw & 0xffffu), nrounds = (int)(cw >> 16);
-    const bool active = tid < count;
-    uint32_t w[RW];
-#pragma unroll
-    for (int k = 0; k < RW; ++k) w[k] = wn[k];
-    if (c + 1 < nchunks) fetch(c + 1);
-    int lv[NV];
-    uint32_t offs[NV * (NV - 1)];
-    double K[NV][NV];
-    double vol = 0.0;
-    int rc = -1;
-    if (active) {
-      lv[0] = bits<0, IDB, RW>(w);
-      lv[1] = bits<IDB, IDB, RW>(w);
-      lv[2] = bits<2 * IDB, IDB, RW>(w);
-      if constexpr (NV == 4) lv[3] = bits<3 * IDB, IDB, RW>(w);
-      rc = (int)bits<NV * IDB + NV * (NV - 1) * OFB, 4, RW>(w);
-      decode_offsets<NV * IDB, OFB, RW>(w, offs, std::make_integer_sequence<int, NV * (NV - 1)>{});
-      vol = element_matrix<D, FORM>(X, lv, FORM == FEMHIP_KAPPA_LAPLACE ? a.kappa[a.rec_elem[rec0 + tid]] : 1.0, K);
-    }
-    for (int r = 0; r < nrounds; ++r) {
-      __syncthreads();
-      if (rc == r) {
-        // per owned vertex: its row's slots (all distinct), loads first,
-        // then adds and stores; rows of halo vertices are skipped
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-          if (lv[i] >= R) continue;
-          // unit load: int phi_i = |det| / (d! (d+1))
-          if constexpr (LOAD) bacc[lv[i]] += vol * (D == 3 ? 1.0 / 24.0 : 1.0 / 6.0);
-          const uint32_t info = rinfo[lv[i]];
-          double* row = acc + (info & 0xffffu);
-          const uint32_t dg = (info >> 16) & 0xffu;
-          double* ptr[NV];
-          ptr[i] = (dg != 0xffu) ? row + dg : dummy;
-          int q = 0;
-#pragma unroll
-          for (int j = 0; j < NV; ++j) {
-            if (j == i) continue;
-            const uint32_t off = offs[i * (NV - 1) + q];
-            ++q;
-            ptr[j] = (off != SENT) ? row + off : dummy;
-          }
-          double v[NV];
-#pragma unroll
-          for (int j = 0; j < NV; ++j) v[j] = *ptr[j];
-#pragma unroll
-          for (int j = 0; j < NV; ++j) *ptr[j] = v[j] + K[i][j];
-        }
-      }
-    }
-  }
-  __syncthreads();
-  // write back: 16 lanes per row
-  const int lane = tid & 15, grp = tid >> 4;
-  for (int r = grp; r < R; r += kFusedBlock / 16) {
-    const int64_t gstart = a.rows[2 * (int64_t)(row_start + r)];
-    const uint32_t info = rinfo[r];
-    const int len = (int)(info >> 24), base = (int)(info & 0xffffu);
-    for (int k = lane; k < len; k += 16)
-      __builtin_nontemporal_store(acc[base + k], a.out + gstart + k);
-  }
-  if constexpr (LOAD)
-    for (int r = tid; r < R; r += kFusedBlock) a.load[a.nodemap[node_start + r]] = bacc[r];
-  }  // partitions
-}
-
-// ---------------------------------------------------------------------
-// Fixed-point variant.  A contribution v enters the partition's LDS row
-// accumulators as the integer round(v * 2^s) (int64, two's complement),
-// summed with LDS integer atomics: integer addition is associative, so the
-// sums -- and the values written -- do not depend on the order in which
-// lanes and waves arrive (deterministic without colour rounds or barriers).
-//
-// s is chosen per partition (and form) from M = the largest |K_aa| over the
-// partition's elements: s = 47 - ilogb(M), so every contribution
-// |K_ab| <= M (the local matrices are semidefinite) is below 2^48 after
-// scaling.  The rounding error of one contribution is at most 2^-s / 2 <=
-// M * 2^-48, and M <= max|A| (a diagonal entry of A sums same-sign K_aa),
-// so a slot with n contributions is within n * 3.6e-15 * max|A| of its
-// exact sum.  The exponent of the previous call is kept in `scale`; the
-// pass measures M while it accumulates and is repeated with the exponent
-// of the measured M when the two differ (first call, or a changed input),
-// so the values are a function of the input alone.
-// ---------------------------------------------------------------------
-constexpr int kFxUnknown = INT32_MIN;
-constexpr double kFxMagic = 6755399441055744.0;  // 1.5 * 2^52
-
 // round(v * scale) as int64 for |v * scale| < 2^51: the fma rounds the
 // scaled value to an integer held in the low mantissa bits of the sum.
 __device__ __forceinline__ unsigned long long to_fixed(double v, double scale) {
@@ -336,13 +128,24 @@ __device__ __forceinline__ unsigned long long to_fixed(double v, double scale) {
   return (unsigned long long)(__double_as_longlong(y) - __double_as_longlong(kFxMagic));
 }
 
-__device__ __forceinline__ double from_fixed(unsigned long long v, double inv_scale) {
-  return (double)(long long)v * inv_scale;
+#ifndef FEMHIP_FUSED_NT_STORE
+#define FEMHIP_FUSED_NT_STORE 1
+#endif
+// CSR values are written once and not read by this kernel again
+__device__ __forceinline__ void store_value(double v, double* p) {
+  if constexpr (FEMHIP_FUSED_NT_STORE) __builtin_nontemporal_store(v, p);
+  else *p = v;
 }
 
-__device__ __forceinline__ int fx_exponent(double m) {
+__device__ __forceinline__ double from_fixed(long long v, double inv_scale) {
+  return (double)v * inv_scale;
+}
+
+// exponent bucket of the largest |contribution| m: 2^46 <= m 2^s < 2^50
+__device__ __forceinline__ int fx_bucket(double m) {
   if (!(m > 0.0) || !(m < INFINITY)) return 0;
-  const int s = 47 - ilogb(m);
+  int s = 49 - ilogb(m);
+  s = s >= 0 ? (s & ~3) : -((-s + 3) & ~3);
   return s > 1000 ? 1000 : (s < -1000 ? -1000 : s);
 }
 
@@ -350,8 +153,7 @@ __device__ __forceinline__ void lds_add(unsigned long long* p, unsigned long lon
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// max(m, |v|) in one instruction (the |v| as a source modifier); the
-// operands are finite, so the NaN canonicalisation of fmax is not needed
+// max(m, |v|) in one instruction (|v| as a source modifier)
 __device__ __forceinline__ double max_abs(double m, double v) {
   double r;
   asm("v_max_f64 %0, %1, |%2|" : "=v"(r) : "v"(m), "v"(v));
@@ -364,267 +166,470 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// One pass over a partition's element records: the local matrices, the
-// largest |K_aa| and load entry (mk, mv), and -- when SCATTER -- the
-// fixed-point contributions added into the LDS rows (pruned slots go to a
-// per-lane sink).
-// Element records are fetched kPrefetch chunks ahead of their use (deeper
-// rings measured no faster on the 1M-DoF tetrahedral mesh: the element
-// loop is bound by LDS and issue, not by the record loads).
-#ifndef FEMHIP_FUSED_PREFETCH
-#define FEMHIP_FUSED_PREFETCH 1
-#endif
-constexpr int kPrefetch = FEMHIP_FUSED_PREFETCH;
-static_assert(kPrefetch >= 1 && kPrefetch <= 3, "record prefetch depth 1 to 3");
+// Local matrix of a P1 simplex from its vertex coordinates in LDS.
+// Laplace: K_ab = coef (n_a . n_b) / (d! |det|) with n_a the cross-product
+// (2-D: rotated-edge) gradients scaled by det; row and column 0 from the
+// zero row sums; K[i][i] (i >= 1) are formed for that, K[0][0] is not.
+// Mass: |det| / (d! (d+1)(d+2)) (1 + delta_ab).  Also the suspect bound tau
+// of the header comment.
+template <int D, int FORM>
+struct Elem {
+  static constexpr int NV = D + 1;
+  double K[NV][NV];
+  double vol;   // |det| / d!
+  double tau;
+};
 
-template <int RW>
-struct RecRing {
-  // depth 1 alternates between two buffers (no register copies)
-  static constexpr int kSlots = kPrefetch > 1 ? kPrefetch : 2;
-  uint32_t w[kSlots][RW];
-  int32_t rec[kSlots];
-  int count[kSlots];
-  // records of chunk c of the partition's chunk list into slot S (lane tid:
-  // record tid of the chunk)
-  template <int S>
-  __device__ __forceinline__ void load(const FusedArgs& a, int chunk_start, int c, int tid) {
-    rec[S] = a.chunks[2 * (int64_t)(chunk_start + c)];
-    count[S] = a.chunks[2 * (int64_t)(chunk_start + c) + 1] & 0xffff;
-    if (tid < count[S]) {
-      const uint32_t* rp = a.recs + (int64_t)(rec[S] + tid) * RW;
+#ifndef FEMHIP_FUSED_XSOA
+#define FEMHIP_FUSED_XSOA 0
+#endif
+// staged coordinates: node-major (x y z per node) or one array per axis
+constexpr bool kXSoa = FEMHIP_FUSED_XSOA != 0;
+
+template <int D, int FORM>
+__device__ __forceinline__ void element_matrix(const double* X, int xs, const int (&lv)[D + 1],
+                                               double coef, bool want_tau, Elem<D, FORM>& el) {
+  constexpr int NV = D + 1;
+  double x[NV][D];
 #pragma unroll
-      for (int k = 0; k < RW; ++k) w[S][k] = __builtin_nontemporal_load(rp + k);
+  for (int v = 0; v < NV; ++v) {
+    if constexpr (kXSoa) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) x[v][d] = X[d * xs + lv[v]];
+    } else if constexpr (D == 3) {
+      x[v][0] = X[3 * lv[v]];
+      x[v][1] = X[3 * lv[v] + 1];
+      x[v][2] = X[3 * lv[v] + 2];
+    } else {
+      const double2 xy = reinterpret_cast<const double2*>(X)[lv[v]];
+      x[v][0] = xy.x;
+      x[v][1] = xy.y;
     }
   }
-  // chunks 0 .. kPrefetch-1
-  __device__ __forceinline__ void prime(const FusedArgs& a, int chunk_start, int nchunks,
-                                        int tid) {
-    if (nchunks > 0) load<0>(a, chunk_start, 0, tid);
-    if constexpr (kPrefetch > 1)
-      if (nchunks > 1) load<(kPrefetch > 1 ? 1 : 0)>(a, chunk_start, 1, tid);
-    if constexpr (kPrefetch > 2)
-      if (nchunks > 2) load<kPrefetch - 1>(a, chunk_start, 2, tid);
+  if constexpr (D == 3) {
+    double c[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      c[0][d] = x[1][d] - x[0][d];
+      c[1][d] = x[2][d] - x[0][d];
+      c[2][d] = x[3][d] - x[0][d];
+    }
+    double n[4][3];
+    n[1][0] = c[1][1] * c[2][2] - c[1][2] * c[2][1];
+    n[1][1] = c[1][2] * c[2][0] - c[1][0] * c[2][2];
+    n[1][2] = c[1][0] * c[2][1] - c[1][1] * c[2][0];
+    n[2][0] = c[2][1] * c[0][2] - c[2][2] * c[0][1];
+    n[2][1] = c[2][2] * c[0][0] - c[2][0] * c[0][2];
+    n[2][2] = c[2][0] * c[0][1] - c[2][1] * c[0][0];
+    n[3][0] = c[0][1] * c[1][2] - c[0][2] * c[1][1];
+    n[3][1] = c[0][2] * c[1][0] - c[0][0] * c[1][2];
+    n[3][2] = c[0][0] * c[1][1] - c[0][1] * c[1][0];
+    const double det = c[0][0] * n[1][0] + c[0][1] * n[1][1] + c[0][2] * n[1][2];
+    const double ad = fabs(det);
+    el.vol = ad * (1.0 / 6.0);
+    if constexpr (FORM == FEMHIP_MASS) {
+      const double m = ad * (1.0 / 120.0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) el.K[i][j] = (i == j) ? 2.0 * m : m;
+      el.tau = 0.0;   // mass entries never vanish on a valid element
+      if (!(ad > 0.0) || !(ad < INFINITY)) el.tau = INFINITY;
+    } else {
+      const double r = recip(6.0 * ad);
+      const double s = coef * r;
+#pragma unroll
+      for (int i = 1; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j) {
+          const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1] + n[i][2] * n[j][2]);
+          el.K[i][j] = v;
+          el.K[j][i] = v;
+        }
+#pragma unroll
+      for (int j = 1; j < 4; ++j) el.K[0][j] = el.K[j][0] = -(el.K[j][1] + el.K[j][2] + el.K[j][3]);
+      el.K[0][0] = -(el.K[0][1] + el.K[0][2] + el.K[0][3]);
+      // a = max |edge component|; tau = 2^-43 a^4 / D (1 + 10 a^3 / D) |coef|
+      el.tau = 0.0;
+      if (want_tau) {
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) a = max_abs(a, c[k][d]);
+        const double rd = 6.0 * r;  // 1 / D
+        const double a2 = a * a, a3 = a2 * a;
+        el.tau = fabs(coef) * (0x1p-43 * (a2 * a2) * rd) * __fma_rn(10.0 * a3, rd, 1.0);
+      }
+    }
+  } else {
+    const double c1x = x[1][0] - x[0][0], c1y = x[1][1] - x[0][1];
+    const double c2x = x[2][0] - x[0][0], c2y = x[2][1] - x[0][1];
+    const double det = c1x * c2y - c1y * c2x;
+    const double ad = fabs(det);
+    el.vol = ad * 0.5;
+    if constexpr (FORM == FEMHIP_MASS) {
+      const double m = ad * (1.0 / 24.0);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) el.K[i][j] = (i == j) ? 2.0 * m : m;
+      el.tau = 0.0;
+      if (!(ad > 0.0) || !(ad < INFINITY)) el.tau = INFINITY;
+    } else {
+      const double n[3][2] = {{0, 0}, {c2y, -c2x}, {-c1y, c1x}};
+      const double r = recip(2.0 * ad);
+      const double s = coef * r;
+#pragma unroll
+      for (int i = 1; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+          const double v = s * (n[i][0] * n[j][0] + n[i][1] * n[j][1]);
+          el.K[i][j] = v;
+          el.K[j][i] = v;
+        }
+#pragma unroll
+      for (int j = 1; j < 3; ++j) el.K[0][j] = el.K[j][0] = -(el.K[j][1] + el.K[j][2]);
+      el.K[0][0] = -(el.K[0][1] + el.K[0][2]);
+      el.tau = 0.0;
+      if (want_tau) {
+        double a = max_abs(max_abs(max_abs(0.0, c1x), c1y), max_abs(max_abs(0.0, c2x), c2y));
+        const double rd = 2.0 * r;
+        const double a2 = a * a;
+        el.tau = fabs(coef) * (0x1p-43 * a2 * rd) * __fma_rn(10.0 * a2, rd, 1.0);
+      }
+    }
+  }
+}
+
+// LDS layout: [max_slots accumulators | coordinates | load accumulators |
+//  2*NW reduction | row info | row starts | 4 ints]
+template <int D, bool LOAD, int NT>
+struct Lds {
+  unsigned long long* acc;
+  double* X;
+  unsigned long long* bacc;
+  double* red;
+  uint32_t* rinfo;
+  int32_t* rstart;
+  int32_t* misc;
+  __device__ Lds(double* smem, const FxArgs& a) {
+    constexpr int NW = NT / 64;
+    acc = reinterpret_cast<unsigned long long*>(smem);
+    X = smem + ((a.max_slots + 1) & ~1);
+    bacc = reinterpret_cast<unsigned long long*>(X + D * a.max_nodes);
+    red = reinterpret_cast<double*>(bacc + (LOAD ? a.max_rows : 0));
+    rinfo = reinterpret_cast<uint32_t*>(red + 2 * NW);
+    rstart = reinterpret_cast<int32_t*>(rinfo + a.max_rows);
+    misc = rstart + a.max_rows;
   }
 };
 
-// One pass over a partition's element records: the local matrices, the
-// largest |K_aa| and load entry (mk, mv), and -- when SCATTER -- the
-// fixed-point contributions added into the LDS rows.  `ring` holds chunks
-// 0..kPrefetch-1 on entry.
-template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD, bool SCATTER>
-__device__ __forceinline__ void fx_pass(const FusedArgs& a, int chunk_start, int nchunks,
-                                        int R, const double* X, const uint32_t* rinfo,
-                                        unsigned long long* acc, unsigned long long* bacc,
-                                        double sck, double scv, RecRing<RW>& ring,
-                                        double& mk, double& mv) {
-  constexpr int NV = D + 1;
-  constexpr double LOADC = (D == 3) ? 1.0 / 24.0 : 1.0 / 6.0;  // int phi = |det| LOADC
-  const int tid = threadIdx.x;
+// Suspects of one element (bit i*NV+j: |K_ij| <= tau, row i owned) into the
+// global queue (one queue reservation per lane that has any; rare).
+template <int NV>
+__device__ __forceinline__ void push_suspects(const FxArgs& a, uint32_t mask, int32_t e) {
+  const int cnt = __popc(mask);
+  long long pos = (long long)atomicAdd(reinterpret_cast<unsigned long long*>(&a.st->nsusp),
+                                       (unsigned long long)cnt);
+  if (pos + cnt > a.susp_cap) atomicOr(&a.st->flags, FEMHIP_FUSED_OVERFLOW);
+  while (mask) {
+    const int b = __ffs(mask) - 1;
+    mask &= mask - 1;
+    if (pos < a.susp_cap)
+      a.susp[pos] = (uint64_t)(uint32_t)e | ((uint64_t)(b / NV) << 32) |
+                    ((uint64_t)(b % NV) << 40);
+    ++pos;
+  }
+}
 
-  auto element = [&](const uint32_t (&w)[RW], int32_t rec0) {
-    int lv[NV];
-    uint32_t offs[NV * (NV - 1)];
-    double K[NV][NV];
-    lv[0] = bits<0, IDB, RW>(w);
-    lv[1] = bits<IDB, IDB, RW>(w);
-    lv[2] = bits<2 * IDB, IDB, RW>(w);
-    if constexpr (NV == 4) lv[3] = bits<3 * IDB, IDB, RW>(w);
-    decode_fields<NV * IDB, OFB, RW>(w, offs, std::make_integer_sequence<int, NV * (NV - 1)>{});
-    const double vol = element_matrix<D, FORM>(
-        X, lv, FORM == FEMHIP_KAPPA_LAPLACE ? a.kappa[a.rec_elem[rec0 + tid]] : 1.0, K);
+// Partition of the k-th take of this workgroup: one per workgroup, or a
+// strided walk of a persistent grid; consecutive workgroups (dealt to the
+// eight XCDs round-robin) get spatially adjacent partitions on one XCD.
+// Returns -1 when none is left.
+__device__ __forceinline__ int next_partition(const FxArgs& a, int k) {
+  const int pi = blockIdx.x + k * gridDim.x;
+  return pi < a.nparts ? xcd_remap(pi, a.nparts) : -1;
+}
+
+template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD, int NT>
+__global__ void __launch_bounds__(NT)
+    __attribute__((amdgpu_waves_per_eu(LOAD ? 4 : (NT == 1024 ? 8 : 6))))
+p1_fused_kernel(FxArgs a) {
+  constexpr int NV = D + 1;
+  constexpr int NW = NT / 64;
+  constexpr bool ROWSUM = FORM != FEMHIP_MASS;
+  constexpr double LOADC = (D == 3) ? 1.0 / 4.0 : 1.0 / 3.0;  // int phi = vol / (d+1)
+  extern __shared__ double smem[];
+  Lds<D, LOAD, NT> L(smem, a);
+  const int tid = threadIdx.x;
+  femhip_fused_state_t* st = a.st;
+  const bool first = a.mode == 0;
+  if (!first && !st->redo) return;  // uniform: the first pass's exponent was right
+  const bool known = first ? st->valid != 0 : true;
+  const bool scatter = known;
+  // the repeat launch runs with the exponents fused_exponent_kernel set
+  const int s = known ? st->s : 0, sv = known ? st->sv : 0;
+  const double sck = ldexp(1.0, s), scv = ldexp(1.0, sv);
+#ifdef FEMHIP_FUSED_NOSUSP
+  constexpr bool suspects = false;
+#else
+  const bool suspects = first && a.susp != nullptr;
+#endif
+  double mk = 0.0, mv = 0.0;
+  bool bad = false;
+
+  for (int k = 0;; ++k) {
+    const int part = next_partition(a, k);
+    if (part < 0) break;
+    if (k > 0) __syncthreads();  // LDS reuse
+    const int32_t* hdr = a.parts + 8 * (int64_t)part;
+    const int node_start = hdr[0], Ln = hdr[1], row_start = hdr[2], R = hdr[3];
+    const int nslots = hdr[4], rec_start = hdr[5], nrec = hdr[6];
+
+    // staging: coordinates (two nodes per thread and round, their loads
+    // issued together), row table, zeroed accumulators
+    for (int l0 = tid; l0 < Ln; l0 += 2 * NT) {
+      const int l1 = l0 + NT < Ln ? l0 + NT : l0;
+      const int64_t g0 = a.nodemap[node_start + l0], g1 = a.nodemap[node_start + l1];
+      double x0[D], x1[D];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) mk = max_abs(mk, K[i][i]);
-    if constexpr (LOAD) mv = fmax(mv, vol * LOADC);
-    if constexpr (SCATTER) {
+      for (int d = 0; d < D; ++d) {
+        x0[d] = a.p[d * a.nnodes + g0];
+        x1[d] = a.p[d * a.nnodes + g1];
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        L.X[kXSoa ? d * a.max_nodes + l0 : D * l0 + d] = x0[d];
+        L.X[kXSoa ? d * a.max_nodes + l1 : D * l1 + d] = x1[d];
+      }
+    }
+    for (int r = tid; r < R; r += NT) {
+      const int2 rw = reinterpret_cast<const int2*>(a.rows)[row_start + r];
+      L.rstart[r] = rw.x;
+      L.rinfo[r] = (uint32_t)rw.y;
+      if constexpr (LOAD) L.bacc[r] = 0ull;
+    }
+    if (scatter)
+      for (int i = tid; i < nslots; i += NT) L.acc[i] = 0ull;
+    if (nrec == 0) {
+      __syncthreads();
+      if (!scatter) continue;
+    }
+
+    // records: chunk c + 1 loads while chunk c is processed.  Every load is
+    // unconditional (an index past the partition's records is clamped to
+    // its last record), so the number of loads in flight is the same on
+    // every path and the wait for a chunk's records leaves the next
+    // chunk's loads outstanding.
+    const int nchunks = (nrec + NT - 1) / NT;
+    struct Chunk {
+      uint32_t w[RW];
+      double coef;
+    };
+    auto fetch = [&](int c, Chunk& ch) {
+      int r = c * NT + tid;
+      r = r < nrec ? r : nrec - 1;
+      const uint32_t* rp = a.recs + (int64_t)(rec_start + r) * RW;
+#pragma unroll
+      for (int q = 0; q < RW; ++q) ch.w[q] = __builtin_nontemporal_load(rp + q);
+      if constexpr (FORM == FEMHIP_KAPPA_LAPLACE) ch.coef = a.kappa[a.rec_elem[rec_start + r]];
+      else ch.coef = 1.0;
+    };
+    Chunk cur, nxt;
+    if (nrec > 0) fetch(0, cur);
+    __syncthreads();  // staged data before the first element
+
+    // one element per lane: local matrix, maxima, suspects (returned as a
+    // bit mask i*NV+j of the owned rows' entries), then the scatter
+    auto element = [&](const Chunk& ch, int c) -> uint32_t {
+      const int rec = c * NT + tid;
+      if (rec >= nrec) return 0u;
+      const uint32_t (&rw)[RW] = ch.w;
+      int lv[NV];
+      lv[0] = bits<0, IDB, RW>(rw);
+      lv[1] = bits<IDB, IDB, RW>(rw);
+      lv[2] = bits<2 * IDB, IDB, RW>(rw);
+      if constexpr (NV == 4) lv[3] = bits<3 * IDB, IDB, RW>(rw);
+      // row table entries of the vertices, read together (a halo vertex
+      // reads row 0's entry, which is not used)
+      uint32_t rinfo[NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) rinfo[i] = L.rinfo[lv[i] < R ? lv[i] : 0];
+      Elem<D, FORM> el;
+      element_matrix<D, FORM>(L.X, a.max_nodes, lv, ch.coef, suspects, el);
+      uint32_t mask = 0;
+      if constexpr (FORM == FEMHIP_MASS) {
+        mk = max_abs(mk, el.K[0][0]);
+        bad = bad || !(el.tau < INFINITY);
+      } else {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+          for (int j = i + 1; j < NV; ++j) mk = max_abs(mk, el.K[i][j]);
+        if (suspects) {
+          bool any = false, dz = false;
+#pragma unroll
+          for (int i = 0; i < NV; ++i) {
+            dz = dz || !(fabs(el.K[i][i]) > el.tau);
+#pragma unroll
+            for (int j = i + 1; j < NV; ++j) any = any || !(fabs(el.K[i][j]) > el.tau);
+          }
+          bad = bad || dz || !(el.tau < INFINITY);
+          if (any) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+#pragma unroll
+              for (int j = 0; j < NV; ++j)
+                if (i != j && lv[i] < R && !(fabs(el.K[i][j]) > el.tau))
+                  mask |= 1u << (i * NV + j);
+          }
+        }
+      }
+      if constexpr (LOAD) mv = fmax(mv, el.vol * LOADC);
+      if (!scatter) return mask;
+      // each off-diagonal pair converted once; per owned row its entries
       unsigned long long F[NV][NV];
 #pragma unroll
       for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int j = i; j < NV; ++j) F[i][j] = F[j][i] = to_fixed(K[i][j], sck);
-#pragma unroll
-      for (int i = 0; i < NV; ++i) {
-        if (lv[i] >= R) continue;  // halo vertex: its row belongs elsewhere
-        // pruned entries (a pruned diagonal too) carry the offset of the
-        // row's spare slot
-        const uint32_t info = rinfo[lv[i]];
-        unsigned long long* row = acc + (info & 0xffffu);
-        lds_add(row + ((info >> 16) & 0xffu), F[i][i]);
+        for (int j = i + (ROWSUM ? 1 : 0); j < NV; ++j)
+          F[i][j] = F[j][i] = to_fixed(el.K[i][j], sck);
+      unsigned long long fl = 0;
+      if constexpr (LOAD) fl = to_fixed(el.vol * LOADC, scv);
+      auto scatter_row = [&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if (lv[i] >= R) return;  // halo vertex: its row belongs elsewhere
+        unsigned long long* row = L.acc + (rinfo[i] & 0xffffu);
+        if constexpr (!ROWSUM) lds_add(row + ((rinfo[i] >> 16) & 0xffu), F[i][i]);
+        uint32_t offs[NV - 1];
+        decode_fields<NV * IDB + i * (NV - 1) * OFB, OFB, RW>(
+            rw, offs, std::make_integer_sequence<int, NV - 1>{});
         int q = 0;
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
           if (j == i) continue;
-          lds_add(row + offs[i * (NV - 1) + q], F[i][j]);
+          lds_add(row + offs[q], F[i][j]);
           ++q;
         }
-        if constexpr (LOAD) lds_add(bacc + lv[i], to_fixed(vol * LOADC, scv));
-      }
-    }
-  };
-  // chunk c uses ring slot c % kPrefetch, which is refilled with chunk
-  // c + kPrefetch before the element work of chunk c
-  auto step = [&](auto slot, int c) {
-    constexpr int S = decltype(slot)::value;
-    if (c >= nchunks) return;
-    uint32_t w[RW];
-#pragma unroll
-    for (int k = 0; k < RW; ++k) w[k] = ring.w[S][k];
-    const int32_t rec0 = ring.rec[S];
-    const bool active = tid < ring.count[S];
-    if (c + kPrefetch < nchunks) ring.template load<S>(a, chunk_start, c + kPrefetch, tid);
-    if (active) element(w, rec0);
-  };
-  if constexpr (kPrefetch == 1) {
-    // chunk c + 1 loads into one buffer while chunk c is processed from the
-    // other, which then receives chunk c + 2
+        if constexpr (LOAD) lds_add(L.bacc + lv[i], fl);
+      };
+      scatter_row(std::integral_constant<int, 0>{});
+      scatter_row(std::integral_constant<int, 1>{});
+      scatter_row(std::integral_constant<int, 2>{});
+      if constexpr (NV == 4) scatter_row(std::integral_constant<int, 3>{});
+      return mask;
+    };
+    auto run = [&](const Chunk& ch, int c) {
+      const uint32_t mask = element(ch, c);
+      if constexpr (FORM != FEMHIP_MASS)
+        if (mask) push_suspects<NV>(a, mask, a.rec_elem[rec_start + c * NT + tid]);
+    };
     for (int c = 0; c < nchunks; c += 2) {
-      if (c + 1 < nchunks) ring.template load<1>(a, chunk_start, c + 1, tid);
-      if (tid < ring.count[0]) element(ring.w[0], ring.rec[0]);
+      fetch(c + 1 < nchunks ? c + 1 : c, nxt);
+      run(cur, c);
       if (c + 1 >= nchunks) break;
-      if (c + 2 < nchunks) ring.template load<0>(a, chunk_start, c + 2, tid);
-      if (tid < ring.count[1]) element(ring.w[1], ring.rec[1]);
+      fetch(c + 2 < nchunks ? c + 2 : c + 1, cur);
+      run(nxt, c + 1);
     }
-  } else {
-    for (int c = 0; c < nchunks; c += kPrefetch) {
-      step(std::integral_constant<int, 0>{}, c);
-      step(std::integral_constant<int, (kPrefetch > 1 ? 1 : 0)>{}, c + 1);
-      if constexpr (kPrefetch > 2) step(std::integral_constant<int, kPrefetch - 1>{}, c + 2);
-    }
-  }
-}
+    if (!scatter) continue;
+    __syncthreads();  // every contribution has landed
+#ifdef FEMHIP_FUSED_NOWB
+    continue;
+#endif
 
-// Fixed-point kernel: one workgroup per partition by default (a smaller
-// grid walks the partitions persistently).
-// NT = 512: three workgroups per CU without the load vector (at most 80
-// VGPRs); NT = 1024: two workgroups per CU (at most 64 VGPRs)
-template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD, int NT>
-__global__ void __launch_bounds__(NT)
-    __attribute__((amdgpu_waves_per_eu(LOAD ? 4 : (NT == 1024 ? 8 : 6))))
-p1_fused_fx_kernel(FusedArgs a) {
-  constexpr int NW = NT / 64;
-  constexpr int FSLOT = FORM == FEMHIP_LAPLACE ? 0 : (FORM == FEMHIP_MASS ? 1 : 2);
-  extern __shared__ double smem[];
-  // [max_slots accumulators | coordinates | load acc | 2*NW reduction |
-  //  row info | row starts]
-  unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem);
-  double* X = smem + ((a.max_slots + 1) & ~1);
-  unsigned long long* bacc = reinterpret_cast<unsigned long long*>(X + D * a.max_nodes);
-  double* red = reinterpret_cast<double*>(bacc + (LOAD ? a.max_rows : 0));
-  uint32_t* rinfo = reinterpret_cast<uint32_t*>(red + 2 * NW);
-  int32_t* rstart = reinterpret_cast<int32_t*>(rinfo + a.max_rows);  // CSR row starts
-  const int tid = threadIdx.x;
-
-  for (int pi = blockIdx.x; pi < a.nparts; pi += gridDim.x) {
-    if (pi != (int)blockIdx.x) __syncthreads();  // LDS reuse
-    const int part = xcd_remap(pi, a.nparts);
-    const int32_t* hdr = a.parts + 8 * (int64_t)part;
-    const int node_start = hdr[0], L = hdr[1], row_start = hdr[2], R = hdr[3];
-    const int nslots = hdr[4], chunk_start = hdr[5], nchunks = hdr[6];
-    int32_t* sc = a.scale + 2 * ((int64_t)FSLOT * a.nparts + part);
-    int s = sc[0], sv = sc[1];
-    bool known = s != kFxUnknown && (!LOAD || sv != kFxUnknown);
-
-    for (int l = tid; l < L; l += NT) {
-      const int64_t g = a.nodemap[node_start + l];
-#pragma unroll
-      for (int d = 0; d < D; ++d) X[D * l + d] = a.p[d * a.nnodes + g];
-    }
-    for (int r = tid; r < R; r += NT) {
-      const int2 rw = reinterpret_cast<const int2*>(a.rows)[row_start + r];
-      rstart[r] = rw.x;
-      rinfo[r] = (uint32_t)rw.y;
-    }
-    RecRing<RW> ring;
-    ring.prime(a, chunk_start, nchunks, tid);  // overlaps the staging
-
-    for (;;) {
-      if (known) {
-        for (int i = tid; i < nslots; i += NT) acc[i] = 0ull;
-        if constexpr (LOAD)
-          for (int r = tid; r < R; r += NT) bacc[r] = 0ull;
-      }
-      __syncthreads();  // staging / zeroing before the first contribution
-      double mk = 0.0, mv = 0.0;
-      if (known)
-        fx_pass<D, RW, IDB, OFB, FORM, LOAD, true>(a, chunk_start, nchunks, R, X, rinfo, acc,
-                                                   bacc, ldexp(1.0, s), ldexp(1.0, sv), ring,
-                                                   mk, mv);
-      else
-        fx_pass<D, RW, IDB, OFB, FORM, LOAD, false>(a, chunk_start, nchunks, R, X, rinfo, acc,
-                                                    bacc, 1.0, 1.0, ring, mk, mv);
-      // partition maxima -> the exponents this pass should have used
-      mk = wave_max(mk);
-      if constexpr (LOAD) mv = wave_max(mv);
-      if ((tid & 63) == 0) {
-        red[tid >> 6] = mk;
-        red[NW + (tid >> 6)] = mv;
-      }
-      __syncthreads();  // also: every contribution of the pass has landed
-      double MK = red[0], MV = red[NW];
-#pragma unroll
-      for (int k = 1; k < NW; ++k) {
-        MK = fmax(MK, red[k]);
-        MV = fmax(MV, red[NW + k]);
-      }
-      const int s2 = fx_exponent(MK), sv2 = LOAD ? fx_exponent(MV) : sv;
-      if (known && s2 == s && sv2 == sv) break;
-      s = s2;
-      sv = sv2;
-      known = true;
-      ring.prime(a, chunk_start, nchunks, tid);
-      __syncthreads();  // reductions read before the next pass reuses red
-    }
-    // write back: 16 lanes per row
+    // write back: 16 lanes per row; Laplace forms: diagonal = -(sum of the
+    // row's other slots, the spare slot included)
     const double inv = ldexp(1.0, -s);
     const int lane = tid & 15, grp = tid >> 4;
     for (int r = grp; r < R; r += NT / 16) {
-      const int64_t gstart = rstart[r];
-      const uint32_t info = rinfo[r];
-      const int len = (int)(info >> 24), base = (int)(info & 0xffffu);
-      for (int k = lane; k < len; k += 16)
-        __builtin_nontemporal_store(from_fixed(acc[base + k], inv), a.out + gstart + k);
+      const int64_t gstart = L.rstart[r];
+      const uint32_t info = L.rinfo[r];
+      const int base = (int)(info & 0xffffu), dg = (int)((info >> 16) & 0xffu);
+      const int len = (int)((info >> 24) & 0x7fu);
+      const long long v0 = lane < len ? (long long)L.acc[base + lane] : 0;
+      const long long v1 = lane + 16 < len ? (long long)L.acc[base + lane + 16] : 0;
+      long long dsum = 0;
+      if constexpr (ROWSUM) {
+        long long t = (lane != dg ? v0 : 0) + (lane + 16 != dg ? v1 : 0);
+        if (lane == 0 && (info >> 31)) t += (long long)L.acc[base + len];
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o, 16);
+        dsum = -t;
+      }
+      if (lane < len)
+        store_value(from_fixed(ROWSUM && lane == dg ? dsum : v0, inv), a.out + gstart + lane);
+      if (lane + 16 < len)
+        store_value(from_fixed(ROWSUM && lane + 16 == dg ? dsum : v1, inv),
+                    a.out + gstart + lane + 16);
     }
     if constexpr (LOAD) {
       const double invv = ldexp(1.0, -sv);
       for (int r = tid; r < R; r += NT)
-        a.load[a.nodemap[node_start + r]] = from_fixed(bacc[r], invv);
-    }
-    if (tid == 0) {
-      sc[0] = s;
-      sc[1] = sv;
+        a.load[a.nodemap[node_start + r]] = from_fixed((long long)L.bacc[r], invv);
     }
   }  // partitions
+
+  // workgroup maxima -> the launch's maxima (read by the repeat launch and
+  // the finish kernel; no fence: the kernel boundary orders them)
+  if (!first) return;
+  mk = wave_max(mk);
+  if constexpr (LOAD) mv = wave_max(mv);
+  const bool wbad = __any(bad);
+  if ((tid & 63) == 0) {
+    L.red[tid >> 6] = mk;
+    L.red[NW + (tid >> 6)] = mv;
+    if (wbad && suspects) atomicOr(&st->flags, FEMHIP_FUSED_BAD);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double MK = L.red[0], MV = L.red[NW];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) {
+      MK = fmax(MK, L.red[q]);
+      MV = fmax(MV, L.red[NW + q]);
+    }
+    atomicMax(reinterpret_cast<unsigned long long*>(&st->maxk),
+              (unsigned long long)__double_as_longlong(MK));
+    if constexpr (LOAD)
+      atomicMax(reinterpret_cast<unsigned long long*>(&st->maxv),
+                (unsigned long long)__double_as_longlong(MV));
+  }
 }
 
+// Between the first pass and the repeat launch: the exponents the first
+// pass measured, and whether the pass must be repeated with them.
+__global__ void fused_exponent_kernel(femhip_fused_state_t* st, int with_load) {
+  const double M = __longlong_as_double((long long)st->maxk);
+  const double MV = __longlong_as_double((long long)st->maxv);
+  const int s2 = fx_bucket(M), sv2 = with_load ? fx_bucket(MV) : st->sv;
+  st->redo = (!st->valid || s2 != st->s || sv2 != st->sv) ? 1 : 0;
+  st->s = s2;
+  st->sv = sv2;
+  st->valid = 1;
+  st->maxk = 0;
+  st->maxv = 0;
+}
 
-// groups == 0: one workgroup per partition; fewer: persistent workgroups
 template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>
-void launch_one(const FusedArgs& a, int nt, size_t lds, int groups, hipStream_t s) {
-  if (a.scale == nullptr) {
-    const int g = (groups == 0 || groups > a.nparts) ? a.nparts : groups;
-    p1_fused_kernel<D, RW, IDB, OFB, FORM, LOAD><<<dim3(g), dim3(kFusedBlock), lds, s>>>(a);
-    return;
-  }
-  auto launch = [&](auto kernel, int threads) {
-    const int g = (groups == 0 || groups > a.nparts) ? a.nparts : groups;
-    kernel<<<dim3(g), dim3(threads), lds, s>>>(a);
-  };
+void launch_one(const FxArgs& a, int nt, size_t lds, unsigned grid, hipStream_t s) {
   if (nt == 1024)
-    launch(p1_fused_fx_kernel<D, RW, IDB, OFB, FORM, LOAD, 1024>, 1024);
+    p1_fused_kernel<D, RW, IDB, OFB, FORM, LOAD, 1024><<<grid, 1024, lds, s>>>(a);
   else
-    launch(p1_fused_fx_kernel<D, RW, IDB, OFB, FORM, LOAD, 512>, 512);
+    p1_fused_kernel<D, RW, IDB, OFB, FORM, LOAD, 512><<<grid, 512, lds, s>>>(a);
 }
 
 template <int D, int RW, int IDB, int OFB, bool LOAD>
-int launch_form(const FusedArgs& a, int form, int nt, size_t lds, int groups, hipStream_t s) {
+int launch_form(const FxArgs& a, int form, int nt, size_t lds, unsigned grid, hipStream_t s) {
   switch (form) {
     case FEMHIP_LAPLACE:
-      launch_one<D, RW, IDB, OFB, FEMHIP_LAPLACE, LOAD>(a, nt, lds, groups, s);
+      launch_one<D, RW, IDB, OFB, FEMHIP_LAPLACE, LOAD>(a, nt, lds, grid, s);
       break;
     case FEMHIP_MASS:
-      launch_one<D, RW, IDB, OFB, FEMHIP_MASS, LOAD>(a, nt, lds, groups, s);
+      launch_one<D, RW, IDB, OFB, FEMHIP_MASS, LOAD>(a, nt, lds, grid, s);
       break;
     case FEMHIP_KAPPA_LAPLACE:
-      launch_one<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD>(a, nt, lds, groups, s);
+      launch_one<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD>(a, nt, lds, grid, s);
       break;
     default:
       set_error("fused path: laplace, mass or kappa_laplace only");
@@ -635,16 +640,16 @@ int launch_form(const FusedArgs& a, int form, int nt, size_t lds, int groups, hi
 }
 
 template <bool LOAD>
-int dispatch_fused(const FusedArgs& a, const femhip_fused_plan_info_t& I, int form, int dim,
-                   size_t lds, int groups, hipStream_t st) {
-  if (dim == 3 && I.rec_words == 3 && I.idbits == 11 && I.offbits == 4)
-    return launch_form<3, 3, 11, 4, LOAD>(a, form, I.block, lds, groups, st);
+int dispatch_fused(const FxArgs& a, const femhip_fused_plan_info_t& I, int form, int dim,
+                   size_t lds, unsigned grid, hipStream_t st) {
+  if (dim == 3 && I.rec_words == 3 && I.idbits == 12 && I.offbits == 4)
+    return launch_form<3, 3, 12, 4, LOAD>(a, form, I.block, lds, grid, st);
   if (dim == 3 && I.rec_words == 4 && I.idbits == 16 && I.offbits == 5)
-    return launch_form<3, 4, 16, 5, LOAD>(a, form, I.block, lds, groups, st);
+    return launch_form<3, 4, 16, 5, LOAD>(a, form, I.block, lds, grid, st);
   if (dim == 2 && I.rec_words == 2 && I.idbits == 12 && I.offbits == 4)
-    return launch_form<2, 2, 12, 4, LOAD>(a, form, I.block, lds, groups, st);
+    return launch_form<2, 2, 12, 4, LOAD>(a, form, I.block, lds, grid, st);
   if (dim == 2 && I.rec_words == 3 && I.idbits == 16 && I.offbits == 5)
-    return launch_form<2, 3, 16, 5, LOAD>(a, form, I.block, lds, groups, st);
+    return launch_form<2, 3, 16, 5, LOAD>(a, form, I.block, lds, grid, st);
   set_error("fused: unsupported record format");
   return FEMHIP_EUNSUPPORTED;
 }
@@ -655,38 +660,60 @@ int dispatch_fused(const FusedArgs& a, const femhip_fused_plan_info_t& I, int fo
 using namespace femhip;
 
 extern "C" size_t femhip_fused_lds_bytes(const femhip_fused_plan_info_t* I, int32_t dim,
-                                         int32_t with_load, int32_t fixed_point) {
+                                         int32_t with_load) {
   if (!I) return 0;
   const size_t coords = 8 * (size_t)dim * I->max_nodes;
   const size_t load_acc = with_load ? 8 * (size_t)I->max_rows : 0;
-  const size_t rinfo = 4 * (size_t)I->max_rows;
-  if (fixed_point)  // accumulators, coordinates, load, reduction, row info + starts
-    return 8 * (((size_t)I->max_slots + 1) & ~(size_t)1) + coords + load_acc +
-           8 * 2 * (I->block / 64) + 2 * rinfo;
-  return 8 * (((size_t)I->max_slots + 3) & ~(size_t)1) + coords + load_acc + rinfo;
+  // accumulators, coordinates, load, reduction, row info + starts, 4 ints
+  return 8 * (((size_t)I->max_slots + 1) & ~(size_t)1) + coords + load_acc +
+         8 * 2 * (I->block / 64) + 8 * (size_t)I->max_rows + 16;
 }
 
-extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan, int32_t form, int32_t dim,
+extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan,
+                                     femhip_fused_state_t* state, int32_t form, int32_t dim,
                                      const double* p, int64_t nnodes, const double* kappa,
-                                     double* data, double* load, int32_t groups,
+                                     double* data, double* load,
+                                     const femhip_fused_check_t* check, int32_t groups,
                                      femhip_stream_t s) {
-  FEMHIP_REQUIRE(plan != nullptr && p != nullptr && data != nullptr, FEMHIP_EINVAL,
-                 "fused: NULL argument");
+  FEMHIP_REQUIRE(plan != nullptr && state != nullptr && p != nullptr && data != nullptr,
+                 FEMHIP_EINVAL, "fused: NULL argument");
   const femhip_fused_plan_info_t& I = plan->info;
-  FEMHIP_REQUIRE(I.block == kFusedBlock || (plan->scale != nullptr && I.block == 1024),
-                 FEMHIP_EINVAL,
-                 "fused: plan block must be 512 (or 1024 with fixed-point accumulation)");
+  FEMHIP_REQUIRE(I.block == 512 || I.block == 1024, FEMHIP_EINVAL,
+                 "fused: plan block must be 512 or 1024");
   FEMHIP_REQUIRE(form != FEMHIP_KAPPA_LAPLACE || (kappa && plan->rec_elem), FEMHIP_EINVAL,
                  "fused: kappa_laplace needs kappa and rec_elem");
   FEMHIP_REQUIRE(groups >= 0, FEMHIP_EINVAL, "fused: negative group count");
-  if (I.nparts == 0) return FEMHIP_OK;
-  const int ngroups = groups;  // resolved per kernel in launch_one
-  FusedArgs a{plan->parts, plan->nodemap, plan->rows, plan->chunks, plan->recs,
-              plan->rec_elem, p, kappa, data, load, plan->scale, nnodes, I.nparts,
-              I.max_slots, I.max_nodes, I.max_rows};
-  const size_t lds = femhip_fused_lds_bytes(&I, dim, load != nullptr, plan->scale != nullptr);
-  FEMHIP_REQUIRE(lds <= 160 * 1024, FEMHIP_EINVAL, "fused: plan exceeds LDS");
+  FEMHIP_REQUIRE(check == nullptr || (check->queue && check->capacity > 0 && check->verdict),
+                 FEMHIP_EINVAL, "fused: check needs a queue and a verdict");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
-  if (load) return dispatch_fused<true>(a, I, form, dim, lds, ngroups, st);
-  return dispatch_fused<false>(a, I, form, dim, lds, ngroups, st);
+  if (I.nparts == 0) {
+    if (check) return fused_check_empty(check, state, st);
+    return FEMHIP_OK;
+  }
+  // groups == 0: one workgroup per partition; else a persistent grid of
+  // `groups` workgroups walking the partitions
+  const unsigned grid = (unsigned)(groups > 0 && groups < I.nparts ? groups : I.nparts);
+  FxArgs a{plan->parts, plan->nodemap, plan->rows, plan->recs, plan->rec_elem, p, kappa,
+           data, load, state, check ? check->queue : nullptr,
+           check ? check->capacity : 0, nnodes, I.nparts, I.max_slots, I.max_nodes,
+           I.max_rows, 0};
+  const size_t lds = femhip_fused_lds_bytes(&I, dim, load != nullptr);
+  FEMHIP_REQUIRE(lds <= 160 * 1024, FEMHIP_EINVAL, "fused: plan exceeds LDS");
+  // first pass; exponents measured (repeat flag); repeat pass when the
+  // first one used other exponents (its workgroups return at once
+  // otherwise); work counters reset
+  for (int mode = 0; mode < 2; ++mode) {
+    a.mode = mode;
+    const int rc = load ? dispatch_fused<true>(a, I, form, dim, lds, grid, st)
+                        : dispatch_fused<false>(a, I, form, dim, lds, grid, st);
+    if (rc != FEMHIP_OK) return rc;
+    if (mode == 0) {
+      fused_exponent_kernel<<<1, 1, 0, st>>>(state, load != nullptr ? 1 : 0);
+      FEMHIP_CHECK_LAUNCH();
+    }
+  }
+
+  if (check) return fused_check_run(check, state, form, dim, p, nnodes, kappa, st);
+  return FEMHIP_OK;
 }
+

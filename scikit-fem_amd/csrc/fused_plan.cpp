@@ -2,27 +2,28 @@
 // CSR pattern).
 //
 // The fused kernel assembles P1 forms owner-computes style: rows are cut
-// into partitions of spatially close nodes (recursive coordinate
-// bisection of the node coordinates), one workgroup per partition accumulates the partition's
-// CSR rows in LDS and writes them once.  Every element with at least one
-// owned vertex is visited by that partition (halo elements are recomputed
-// by each partition they touch instead of exchanging partial sums).
-//
-// Determinism without floating-point atomics: the partition's elements are
-// greedily coloured so that two elements of one colour never share an
-// owned vertex, sorted by colour and cut into chunks of at most `block`
-// elements spanning at most 16 colours.  The kernel processes a chunk in
-// rounds, one colour per round with a barrier between rounds, so every LDS
-// slot receives its contributions in a fixed order.
+// into partitions of spatially close nodes (recursive coordinate bisection
+// of the node coordinates); one workgroup per partition accumulates the
+// partition's CSR rows in LDS and writes them once.  Every element with at
+// least one owned vertex is visited by that partition (halo elements are
+// recomputed by each partition they touch instead of exchanging partial
+// sums), so every off-diagonal contribution (e, i, j) of the reference's
+// COO list is formed by exactly one partition: the owner of row
+// element_dofs[i, e].
 //
 // Per element visit the plan stores one packed record: the local ids of the
-// element's vertices (partition-local numbering, owned nodes first) and, for
-// every owned vertex a and other vertex b, the offset of column b inside
-// row a of the CSR pattern, plus the colour relative to the chunk.  Rows are
-// laid out back to back in LDS: with the usual odd row lengths (15 in a
-// tetrahedral interior) consecutive rows start on different LDS banks.  A
-// row that lost entries to eliminate_zeros has one spare slot after its CSR
-// slots; its pruned entries (a pruned diagonal too) carry the offset
+// element's vertices (partition-local numbering, owned nodes first, each
+// group in global order) and, for every vertex a and other vertex b, the
+// offset of column b inside row a of the CSR pattern (unused when a is a
+// halo vertex).  A partition's records are contiguous; the kernel takes
+// them in chunks of `block`.  Records are ordered by a greedy colouring
+// (elements of one colour share no owned vertex, so the lanes of a chunk
+// of one colour add into distinct rows) or by element id.
+//
+// Rows are laid out back to back in LDS: with the usual odd row lengths (15
+// in a tetrahedral interior) consecutive rows start on different LDS banks.
+// A row that lost entries to eliminate_zeros has one spare slot after its
+// CSR slots; its pruned entries (a pruned diagonal too) carry the offset
 // len = the spare slot, which is never written out.
 //
 // The blocks are built independently by a pool of host threads
@@ -30,11 +31,11 @@
 // block order: the plan is the same for any number of threads.
 #include <algorithm>
 #include <atomic>
-#include <cstdlib>
-#include <thread>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/femhip.h"
@@ -49,8 +50,7 @@ struct Plan {
   femhip_fused_plan_info_t info;
   std::vector<int32_t> parts;    // 8 int32 per partition
   std::vector<int32_t> nodemap;  // global node id per local node
-  std::vector<int32_t> rows;     // 2 int32 per owned row: gstart, packed
-  std::vector<int32_t> chunks;   // 2 int32 per chunk: rec_start, count|nrounds<<16
+  std::vector<int32_t> rows;     // 2 int32 per owned row: CSR row start, packed info
   std::vector<uint32_t> recs;    // rec_words uint32 per record
   std::vector<int32_t> rec_elem; // global element id per record
 };
@@ -80,33 +80,37 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
                                        const int32_t* t, const double* p,
                                        const int32_t* indptr, const int32_t* indices,
                                        int32_t slot_cap, int32_t node_cap, int32_t block,
-                                       void** handle_out, femhip_fused_plan_info_t* info_out) {
+                                       int32_t order_mode, void** handle_out,
+                                       femhip_fused_plan_info_t* info_out) {
   using femhip::set_error;
   if (!(nverts == 3 && dim == 2) && !(nverts == 4 && dim == 3)) {
     set_error("fused plan: P1 triangles or tetrahedra only");
     return FEMHIP_EUNSUPPORTED;
   }
   if (!t || !p || !indptr || !indices || !handle_out || !info_out || E < 0 || Nv <= 0 ||
-      slot_cap <= 0 || node_cap <= 0 || block <= 0 || block > 1024) {
+      slot_cap <= 0 || node_cap <= 0 || block <= 0 || block > 1024 || block % 64 != 0 ||
+      order_mode < 0 || order_mode > 1) {
     set_error("fused plan: bad arguments");
     return FEMHIP_EINVAL;
   }
+  if (E >= INT32_MAX / 2 || Nv >= INT32_MAX / 2) {
+    set_error("fused plan: mesh exceeds int32 ids");
+    return FEMHIP_EOVERFLOW;
+  }
   const int NV = nverts;
   const int noff = NV * (NV - 1);
-  // record format: local ids and 4- or 5-bit offsets, 4-bit relative colour
   int max_row = 0;
   for (int64_t n = 0; n < Nv; ++n) max_row = std::max(max_row, indptr[n + 1] - indptr[n]);
   // offsets 0..len (len: the spare slot of a row with pruned entries)
-  int offbits = max_row <= 15 ? 4 : (max_row <= 31 ? 5 : 0);
+  const int offbits = max_row <= 15 ? 4 : (max_row <= 31 ? 5 : 0);
   if (offbits == 0) {
     set_error("fused plan: CSR rows longer than 31 entries");
     return FEMHIP_EUNSUPPORTED;
   }
-  const int idbits = (NV == 4) ? (offbits == 4 ? 11 : 16) : (offbits == 4 ? 12 : 16);
-  const int total_bits = NV * idbits + noff * offbits + 4;
+  const int idbits = offbits == 4 ? 12 : 16;
+  const int total_bits = NV * idbits + noff * offbits;
   const int rec_words = (total_bits + 31) / 32;
-  const int id_cap = 1 << idbits;
-  if (node_cap > id_cap) node_cap = id_cap;
+  node_cap = std::min(node_cap, 1 << idbits);
 
   // node -> element incidence
   std::vector<int64_t> inc_ptr(Nv + 1, 0);
@@ -128,14 +132,14 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   }
 
   // 1. recursive coordinate bisection of the nodes into blocks whose CSR
-  // rows fit `slot_cap` LDS slots; every split is weighted so the leaves
-  // come out close to the cap (fewer, larger blocks = less halo work).
+  // rows fit `slot_cap` LDS slots and whose owned + halo nodes fit
+  // `node_cap`; every split is weighted so the leaves come out close to the
+  // caps (fewer, larger blocks = less halo work).
   std::vector<int32_t> order(Nv);
   for (int64_t n = 0; n < Nv; ++n) order[n] = (int32_t)n;
   std::vector<std::pair<int64_t, int64_t>> leaves;  // [begin, end) in `order`
   std::vector<int64_t> cstamp(Nv, -1);
   int64_t check_id = 0;
-  // owned + halo node count of the block order[b, e)
   auto local_nodes = [&](int64_t b, int64_t e) {
     const int64_t id = check_id++;
     int64_t L = 0;
@@ -162,7 +166,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
       auto [b, e] = stack.back();
       stack.pop_back();
       int64_t slots = 0;
-      for (int64_t k = b; k < e; ++k) slots += indptr[order[k] + 1] - indptr[order[k]];
+      for (int64_t k = b; k < e; ++k) slots += indptr[order[k] + 1] - indptr[order[k]] + 1;
       int64_t L = -1;
       if (slots <= slot_cap) {
         L = local_nodes(b, e);
@@ -204,13 +208,13 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   // blocks are then concatenated in leaf order, so the plan does not depend
   // on the number of threads
   struct LeafOut {
-    std::vector<int32_t> nodemap, rows, chunks, rec_elem;  // chunks: leaf-relative rec0
+    std::vector<int32_t> nodemap, rows, rec_elem;
     std::vector<uint32_t> recs;
-    int L = 0, R = 0, lds = 0, ncolors = 0, nelems = 0, err = 0;
+    int L = 0, R = 0, lds = 0, nelems = 0, err = 0;
     std::string msg;
   };
   struct Scratch {
-    std::vector<int32_t> nstamp, lid, estamp, owned, elems, color;
+    std::vector<int32_t> nstamp, lid, estamp, owned, elems, color, locals;
     std::vector<uint64_t> used;
     Scratch(int64_t nv, int64_t ne) : nstamp(nv, -1), lid(nv, -1), estamp(ne, -1) {}
   };
@@ -218,50 +222,29 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   auto build_leaf = [&](size_t leaf, Scratch& S, LeafOut& O) {
     auto& owned = S.owned;
     auto& elems = S.elems;
-    auto& nstamp = S.nstamp;
     auto& lid = S.lid;
-    auto& estamp = S.estamp;
-    auto& used = S.used;
-    auto& color = S.color;
+    auto& locals = S.locals;
     const int32_t pid = (int32_t)leaf;
     owned.clear();
     elems.clear();
-    int nloc = 0;
-    for (int64_t k = leaves[leaf].first; k < leaves[leaf].second; ++k) {
-      const int32_t n = order[k];
-      owned.push_back(n);
-      if (nstamp[n] != pid) {
-        nstamp[n] = pid;
-        ++nloc;
-      }
-    }
+    for (int64_t k = leaves[leaf].first; k < leaves[leaf].second; ++k)
+      owned.push_back(order[k]);
     for (int32_t n : owned)
       for (int64_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
         const int32_t e = inc[k];
-        if (estamp[e] == pid) continue;
-        estamp[e] = pid;
+        if (S.estamp[e] == pid) continue;
+        S.estamp[e] = pid;
         elems.push_back(e);
-        for (int j = 0; j < NV; ++j) {
-          const int32_t v = t[(int64_t)j * E + e];
-          if (nstamp[v] != pid) {
-            nstamp[v] = pid;
-            ++nloc;
-          }
-        }
       }
-    if (nloc > node_cap || (int)owned.size() > 65535) {
-      O.err = FEMHIP_EUNSUPPORTED;
-      O.msg = "fused plan: a block exceeds the local node budget";
-      return;
-    }
     // local numbering: owned rows first, then halo nodes, each in global
     // order -- consecutive owned rows are then mostly consecutive CSR rows
     // (the write-back of neighbouring lanes lands in the same cache lines)
     // and the coordinate gathers of the staging step are coalesced
     std::sort(owned.begin(), owned.end());
+    std::sort(elems.begin(), elems.end());
     const int R = (int)owned.size();
     for (int r = 0; r < R; ++r) lid[owned[r]] = r;
-    std::vector<int32_t> locals(owned.begin(), owned.end());
+    locals.assign(owned.begin(), owned.end());
     for (int32_t e : elems)
       for (int j = 0; j < NV; ++j) {
         const int32_t v = t[(int64_t)j * E + e];
@@ -272,41 +255,50 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
       }
     std::sort(locals.begin() + R, locals.end());
     const int L = (int)locals.size();
-    for (int l = R; l < L; ++l) lid[locals[l]] = l;
-    // colouring: no two elements of a colour share an owned vertex
-    used.assign(R, 0);
-    color.assign(elems.size(), 0);
-    int ncolors = 0;
-    for (size_t i = 0; i < elems.size(); ++i) {
-      const int32_t e = elems[i];
-      uint64_t forbid = 0;
-      for (int j = 0; j < NV; ++j) {
-        const int l = lid[t[(int64_t)j * E + e]];
-        if (l < R) forbid |= used[l];
-      }
-      if (~forbid == 0) {
-        for (int32_t v : locals) lid[v] = -1;
-        O.err = FEMHIP_EUNSUPPORTED;
-        O.msg = "fused plan: more than 64 colours needed";
-        return;
-      }
-      const int c = __builtin_ctzll(~forbid);
-      color[i] = c;
-      ncolors = std::max(ncolors, c + 1);
-      for (int j = 0; j < NV; ++j) {
-        const int l = lid[t[(int64_t)j * E + e]];
-        if (l < R) used[l] |= (1ull << c);
-      }
+    auto release = [&]() {
+      for (int32_t v : locals) lid[v] = -1;
+    };
+    if (L > node_cap || R > 65535) {
+      release();
+      O.err = FEMHIP_EUNSUPPORTED;
+      O.msg = "fused plan: a block exceeds the local node budget";
+      return;
     }
-    // stable sort by colour
-    std::vector<int32_t> order(elems.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = (int32_t)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return color[a] < color[b]; });
+    for (int l = R; l < L; ++l) lid[locals[l]] = l;
+    // record order: colour classes (no two elements of a colour share an
+    // owned vertex), each in element order; element order when more than
+    // 64 colours would be needed or requested
+    std::vector<int32_t> ord(elems.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int32_t)i;
+    if (order_mode == 0) {
+      S.used.assign(R, 0);
+      S.color.assign(elems.size(), 0);
+      bool ok = true;
+      for (size_t i = 0; i < elems.size() && ok; ++i) {
+        const int32_t e = elems[i];
+        uint64_t forbid = 0;
+        for (int j = 0; j < NV; ++j) {
+          const int l = lid[t[(int64_t)j * E + e]];
+          if (l < R) forbid |= S.used[l];
+        }
+        if (~forbid == 0) {
+          ok = false;
+          break;
+        }
+        const int c = __builtin_ctzll(~forbid);
+        S.color[i] = c;
+        for (int j = 0; j < NV; ++j) {
+          const int l = lid[t[(int64_t)j * E + e]];
+          if (l < R) S.used[l] |= (1ull << c);
+        }
+      }
+      if (ok)
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int32_t a, int32_t b) { return S.color[a] < S.color[b]; });
+    }
     // rows of the partition; a row that loses an entry to eliminate_zeros
     // (a pruned off-diagonal pair of one of its elements, or its diagonal)
-    // gets one spare LDS slot after its CSR slots, which the pruned entries
-    // address (offset = row length), so the kernel adds without a test
+    // gets one spare LDS slot after its CSR slots
     auto in_row = [&](int32_t g, int32_t col) {
       const int32_t* s0 = indices + indptr[g];
       const int32_t* f0 = indices + indptr[g + 1];
@@ -323,73 +315,59 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
           if (!in_row(ga, t[(int64_t)bb * E + e])) spare[la] = 1;
       }
     int lds = 0;
-    std::vector<int> row_len(R, 0);
     for (int r = 0; r < R; ++r) {
       const int32_t g = owned[r];
       const int b = indptr[g], en = indptr[g + 1];
-      row_len[r] = en - b;
-      int diag = row_len[r];  // pruned diagonal: the spare slot
+      const int len = en - b;
+      int diag = len;  // pruned diagonal: the spare slot
       for (int k = b; k < en; ++k)
         if (indices[k] == g) diag = k - b;
       O.rows.push_back(b);
+      // base (16 bits) | diagonal offset (8) | length (7) | spare slot (1)
       O.rows.push_back((int32_t)((uint32_t)lds | ((uint32_t)diag << 16) |
-                                 ((uint32_t)row_len[r] << 24)));
-      lds += row_len[r] + spare[r];
+                                 ((uint32_t)len << 24) | ((uint32_t)spare[r] << 31)));
+      lds += len + spare[r];
     }
     if (lds > 65535) {
-      for (int32_t v : locals) lid[v] = -1;
+      release();
       O.err = FEMHIP_EUNSUPPORTED;
       O.msg = "fused plan: a block exceeds 65535 LDS slots";
       return;
     }
-    // chunks and records
-    size_t i = 0;
-    while (i < order.size()) {
-      const int c0 = color[order[i]];
-      size_t j = i;
-      while (j < order.size() && (int)(j - i) < block && color[order[j]] - c0 < 16) ++j;
-      const int nrounds = color[order[j - 1]] - c0 + 1;
-      const int32_t rec0 = (int32_t)O.rec_elem.size();  // leaf-relative
-      O.chunks.push_back(rec0);
-      O.chunks.push_back((int32_t)((j - i) | ((size_t)nrounds << 16)));
-      for (size_t q = i; q < j; ++q) {
-        const int32_t e = elems[order[q]];
-        uint32_t w[4] = {0, 0, 0, 0};
-        int32_t gv[4];
-        int lv[4];
-        for (int a = 0; a < NV; ++a) {
-          gv[a] = t[(int64_t)a * E + e];
-          lv[a] = lid[gv[a]];
-          put_bits(w, a * idbits, idbits, (uint32_t)lv[a]);
-        }
-        int bitpos = NV * idbits;
-        for (int a = 0; a < NV; ++a)
-          for (int b = 0; b < NV; ++b) {
-            if (b == a) continue;
-            uint32_t off = 0;  // halo row: never read
-            if (lv[a] < R) {
-              const int32_t g = gv[a];
-              const int32_t* s = indices + indptr[g];
-              const int32_t* f = indices + indptr[g + 1];
-              const int32_t* it = std::lower_bound(s, f, gv[b]);
-              // pruned pair: the row's spare slot (offset = row length)
-              off = (it != f && *it == gv[b]) ? (uint32_t)(it - s) : (uint32_t)(f - s);
-            }
-            put_bits(w, bitpos, offbits, off);
-            bitpos += offbits;
-          }
-        put_bits(w, bitpos, 4, (uint32_t)(color[order[q]] - c0));
-        for (int k = 0; k < rec_words; ++k) O.recs.push_back(w[k]);
-        O.rec_elem.push_back(e);
+    for (int32_t q : ord) {
+      const int32_t e = elems[q];
+      uint32_t w[4] = {0, 0, 0, 0};
+      int32_t gv[4];
+      int lv[4];
+      for (int a = 0; a < NV; ++a) {
+        gv[a] = t[(int64_t)a * E + e];
+        lv[a] = lid[gv[a]];
+        put_bits(w, a * idbits, idbits, (uint32_t)lv[a]);
       }
-      i = j;
+      int bitpos = NV * idbits;
+      for (int a = 0; a < NV; ++a)
+        for (int b = 0; b < NV; ++b) {
+          if (b == a) continue;
+          uint32_t off = 0;  // halo row: never read
+          if (lv[a] < R) {
+            const int32_t g = gv[a];
+            const int32_t* s = indices + indptr[g];
+            const int32_t* f = indices + indptr[g + 1];
+            const int32_t* it = std::lower_bound(s, f, gv[b]);
+            // pruned pair: the row's spare slot (offset = row length)
+            off = (it != f && *it == gv[b]) ? (uint32_t)(it - s) : (uint32_t)(f - s);
+          }
+          put_bits(w, bitpos, offbits, off);
+          bitpos += offbits;
+        }
+      for (int k = 0; k < rec_words; ++k) O.recs.push_back(w[k]);
+      O.rec_elem.push_back(e);
     }
     O.nodemap.assign(locals.begin(), locals.end());
-    for (int32_t v : locals) lid[v] = -1;
+    release();
     O.L = L;
     O.R = R;
     O.lds = lds;
-    O.ncolors = ncolors;
     O.nelems = (int32_t)elems.size();
   };
   {
@@ -416,38 +394,36 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
 
   Plan* P = new Plan();
   std::memset(&P->info, 0, sizeof(P->info));
-  int max_slots = 0, max_nodes = 0, max_rows = 0, max_colors = 0;
+  int max_slots = 0, max_nodes = 0, max_rows = 0, max_recs = 0;
   int64_t visits = 0;
   for (size_t leaf = 0; leaf < outs.size(); ++leaf) {
     LeafOut& O = outs[leaf];
     const int32_t node_start = (int32_t)P->nodemap.size();
     const int32_t row_start = (int32_t)(P->rows.size() / 2);
-    const int32_t chunk_start = (int32_t)(P->chunks.size() / 2);
-    const int32_t rec_base = (int32_t)P->rec_elem.size();
-    const int32_t nchunks = (int32_t)(O.chunks.size() / 2);
+    const int32_t rec_start = (int32_t)P->rec_elem.size();
     P->nodemap.insert(P->nodemap.end(), O.nodemap.begin(), O.nodemap.end());
     P->rows.insert(P->rows.end(), O.rows.begin(), O.rows.end());
-    for (int32_t c = 0; c < nchunks; ++c) {
-      P->chunks.push_back(O.chunks[2 * c] + rec_base);
-      P->chunks.push_back(O.chunks[2 * c + 1]);
-    }
     P->recs.insert(P->recs.end(), O.recs.begin(), O.recs.end());
     P->rec_elem.insert(P->rec_elem.end(), O.rec_elem.begin(), O.rec_elem.end());
-    P->parts.insert(P->parts.end(), {node_start, O.L, row_start, O.R, O.lds, chunk_start,
-                                     nchunks, O.nelems});
+    P->parts.insert(P->parts.end(),
+                    {node_start, O.L, row_start, O.R, O.lds, rec_start, O.nelems, 0});
     max_slots = std::max(max_slots, O.lds);
     max_nodes = std::max(max_nodes, O.L);
     max_rows = std::max(max_rows, O.R);
-    max_colors = std::max(max_colors, O.ncolors);
+    max_recs = std::max(max_recs, O.nelems);
     visits += O.nelems;
-    LeafOut().nodemap.swap(O.nodemap);  // release as we go
+    std::vector<int32_t>().swap(O.nodemap);  // release as we go
     std::vector<uint32_t>().swap(O.recs);
   }
-  const int32_t pid = (int32_t)outs.size();
+  if (visits >= INT32_MAX) {
+    delete P;
+    set_error("fused plan: more than 2^31-1 element visits");
+    return FEMHIP_EOVERFLOW;
+  }
   femhip_fused_plan_info_t& I = P->info;
-  I.nparts = pid;
+  I.nparts = (int32_t)outs.size();
   I.nrecs = (int64_t)P->rec_elem.size();
-  I.nchunks = (int32_t)(P->chunks.size() / 2);
+  I.max_recs = max_recs;
   I.nlocal = (int64_t)P->nodemap.size();
   I.nrows = (int32_t)(P->rows.size() / 2);
   I.max_slots = max_slots;
@@ -456,7 +432,6 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   I.rec_words = rec_words;
   I.idbits = idbits;
   I.offbits = offbits;
-  I.max_colors = max_colors;
   I.block = block;
   I.dup = E > 0 ? (double)visits / (double)E : 0.0;
   *info_out = I;
@@ -465,8 +440,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
 }
 
 extern "C" int femhip_fused_plan_export(void* handle, int32_t* parts, int32_t* nodemap,
-                                        int32_t* rows, int32_t* chunks, uint32_t* recs,
-                                        int32_t* rec_elem) {
+                                        int32_t* rows, uint32_t* recs, int32_t* rec_elem) {
   if (!handle) {
     femhip::set_error("fused plan: NULL handle");
     return FEMHIP_EINVAL;
@@ -478,7 +452,6 @@ extern "C" int femhip_fused_plan_export(void* handle, int32_t* parts, int32_t* n
   cp(parts, P->parts.data(), P->parts.size() * 4);
   cp(nodemap, P->nodemap.data(), P->nodemap.size() * 4);
   cp(rows, P->rows.data(), P->rows.size() * 4);
-  cp(chunks, P->chunks.data(), P->chunks.size() * 4);
   cp(recs, P->recs.data(), P->recs.size() * 4);
   cp(rec_elem, P->rec_elem.data(), P->rec_elem.size() * 4);
   return FEMHIP_OK;

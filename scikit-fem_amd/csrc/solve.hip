@@ -9,6 +9,7 @@
 // Every reduction runs in a fixed order (16 lanes per CSR row summed by a
 // fixed shuffle tree, per-block partial sums, one final block), so the
 // iterates are bitwise repeatable.
+#include <cmath>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -151,8 +152,8 @@ pcg_init_kernel(const int32_t* __restrict__ indptr, const int32_t* __restrict__ 
 }
 
 // final sums of per-block partials (NC per block), fixed order; MODE 0:
-// init (rz, rr, bb, beta = 0), MODE 1: alpha = rz / pq, MODE 2: beta =
-// rz_new / rz, rr
+// init (rz, rr, bb, beta = 0), MODE 1: alpha = rz / pq (0 when pq = 0),
+// MODE 2: beta = rz_new / rz (0 when rz = 0), rr
 template <int NC, int MODE>
 __global__ void __launch_bounds__(kBlock)
 pcg_reduce_kernel(const double* __restrict__ part, int64_t nblocks, double* __restrict__ sc) {
@@ -173,9 +174,11 @@ pcg_reduce_kernel(const double* __restrict__ part, int64_t nblocks, double* __re
     sc[4] = v2;
     sc[2] = 0.0;
   } else if constexpr (MODE == 1) {
-    sc[1] = sc[0] / v0;
+    // a converged residual (r = 0 exactly, so z = p = 0 and p.q = 0) makes
+    // the step a no-op instead of 0/0
+    sc[1] = v0 != 0.0 ? sc[0] / v0 : 0.0;
   } else {
-    sc[2] = v0 / sc[0];
+    sc[2] = sc[0] != 0.0 ? v0 / sc[0] : 0.0;
     sc[0] = v0;
     sc[3] = v1;
   }
@@ -404,5 +407,7 @@ extern "C" int femhip_pcg(const int32_t* indptr, const int32_t* indices, const d
   }
   *iters = it;
   *resid = rnorm;
+  FEMHIP_REQUIRE(std::isfinite(rnorm), FEMHIP_EINVAL,
+                 "pcg: the residual is not finite (matrix not symmetric positive definite?)");
   return FEMHIP_OK;
 }

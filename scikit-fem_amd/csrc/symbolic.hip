@@ -12,6 +12,7 @@
 // stable, so within a slot contributions stay in the reference's COO order
 // and the numeric reduction (femhip_segment_sum) is deterministic.
 #include <cstring>
+#include <string>
 
 #include <rocprim/rocprim.hpp>
 
@@ -137,43 +138,81 @@ segment_sum_kernel(const double* __restrict__ contrib, const int32_t* __restrict
   out[k] = acc;
 }
 
-// Order-independent signature of the exact-zero positions of contrib:
-// (number of zeros, wrapping sum of a 64-bit mix of their indices).
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdULL;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ULL;
-  x ^= x >> 33;
-  return x;
+// Exact-zero mask of contrib: bit (c & 63) of word c >> 6 is set iff
+// contrib[c] == 0.0 -- one wave ballot per 64 contributions.  A plan keeps
+// the mask of the contributions it was built from; another set of
+// contributions has the same pattern when its mask is equal (compared word
+// by word, exactly).
+__global__ void __launch_bounds__(kBlock)
+zero_mask_kernel(const double* __restrict__ contrib, int64_t n, uint64_t* __restrict__ mask) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t b = __ballot(c < n && contrib[c] == 0.0);
+  if ((threadIdx.x & 63) == 0 && c < n) mask[c >> 6] = b;
+}
+
+__global__ void zero_count_kernel(unsigned long long* __restrict__ count) { *count = 0; }
+
+// number of contributions whose zero bit differs from mask
+__global__ void __launch_bounds__(kBlock)
+zero_mask_mismatch_kernel(const double* __restrict__ contrib, int64_t n,
+                          const uint64_t* __restrict__ mask,
+                          unsigned long long* __restrict__ count) {
+  unsigned long long diff = 0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += stride) {
+    const int64_t c = base + threadIdx.x;
+    const uint64_t b = __ballot(c < n && contrib[c] == 0.0);
+    if ((threadIdx.x & 63) == 0 && c < n) diff += (unsigned long long)__popcll(b ^ mask[c >> 6]);
+  }
+  if ((threadIdx.x & 63) == 0 && diff != 0) atomicAdd(count, diff);
+}
+
+// out[0] = min, out[1] = max of idx[n] (initialised by range_init_kernel)
+__global__ void range_init_kernel(int32_t* __restrict__ out) {
+  out[0] = INT32_MAX;
+  out[1] = INT32_MIN;
 }
 
 __global__ void __launch_bounds__(kBlock)
-zero_signature_kernel(const double* __restrict__ contrib, int64_t n,
-                      unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long s_cnt[kBlock], s_sum[kBlock];
-  unsigned long long cnt = 0, sum = 0;
-  for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < n;
-       c += (int64_t)gridDim.x * kBlock) {
-    if (contrib[c] == 0.0) {
-      cnt += 1;
-      sum += mix64((uint64_t)c + 0x9e3779b97f4a7c15ULL);
-    }
+index_range_kernel(const int32_t* __restrict__ idx, int64_t n, int32_t* __restrict__ out) {
+  int32_t lo = INT32_MAX, hi = INT32_MIN;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * kBlock) {
+    const int32_t v = idx[k];
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
   }
-  s_cnt[threadIdx.x] = cnt;
-  s_sum[threadIdx.x] = sum;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      s_cnt[threadIdx.x] += s_cnt[threadIdx.x + w];
-      s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
-    }
-    __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    const int32_t a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
   }
-  if (threadIdx.x == 0) {
-    atomicAdd(&out[0], s_cnt[0]);
-    atomicAdd(&out[1], s_sum[0]);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(out, lo);
+    atomicMax(out + 1, hi);
   }
+}
+
+// Checks 0 <= idx[k] < bound for idx[n] on the device (scratch: 2 int32 of
+// device memory); synchronises the stream.
+int check_index_range(const int32_t* idx, int64_t n, int64_t bound, int32_t* scratch,
+                      hipStream_t st, const char* what) {
+  if (n == 0) return FEMHIP_OK;
+  range_init_kernel<<<1, 1, 0, st>>>(scratch);
+  FEMHIP_CHECK_LAUNCH();
+  int64_t blocks = ceil_div(n, kBlock);
+  if (blocks > 1024) blocks = 1024;
+  index_range_kernel<<<(unsigned)blocks, kBlock, 0, st>>>(idx, n, scratch);
+  FEMHIP_CHECK_LAUNCH();
+  int32_t h[2];
+  FEMHIP_TRY_HIP(hipMemcpyAsync(h, scratch, sizeof(h), hipMemcpyDeviceToHost, st));
+  FEMHIP_TRY_HIP(hipStreamSynchronize(st));
+  if (h[0] < 0 || (int64_t)h[1] >= bound) {
+    set_error(std::string(what) + ": index out of range [0, " + std::to_string(bound) +
+              "): min " + std::to_string(h[0]) + ", max " + std::to_string(h[1]));
+    return FEMHIP_EINVAL;
+  }
+  return FEMHIP_OK;
 }
 
 unsigned bits_for(uint64_t v) {
@@ -264,6 +303,16 @@ extern "C" int femhip_pattern_build(const int32_t* row_dofs, const int32_t* col_
   FEMHIP_REQUIRE(row_dofs && col_dofs && indices && perm && seg, FEMHIP_EINVAL,
                  "NULL input/output");
   Workspace w = carve(workspace, C);
+  // a DOF index outside the matrix would corrupt the sort keys and the
+  // indptr fill: refuse it before any indexing kernel runs
+  if (int rc = check_index_range(row_dofs, (int64_t)nb_row * nelems, nrows, w.scan, st,
+                                 "pattern: row DOF");
+      rc != FEMHIP_OK)
+    return rc;
+  if (int rc = check_index_range(col_dofs, (int64_t)nb_col * nelems, ncols, w.scan, st,
+                                 "pattern: column DOF");
+      rc != FEMHIP_OK)
+    return rc;
   const uint64_t sentinel = (uint64_t)nrows * (uint64_t)ncols;
   const unsigned end_bit = bits_for(sentinel);
   const unsigned grid = (unsigned)ceil_div(C, kBlock);
@@ -316,6 +365,9 @@ extern "C" int femhip_rows_plan_build(const int32_t* row_dofs, int32_t nb_row,
                  "workspace too small");
   FEMHIP_REQUIRE(row_dofs && perm, FEMHIP_EINVAL, "NULL input/output");
   Workspace w = carve(workspace, C);
+  if (int rc = check_index_range(row_dofs, C, nrows, w.scan, st, "rows plan: DOF");
+      rc != FEMHIP_OK)
+    return rc;
   const unsigned grid = (unsigned)ceil_div(C, kBlock);
   make_row_keys_kernel<<<grid, kBlock, 0, st>>>(row_dofs, nelems, C, w.keys_in, w.vals_in);
   FEMHIP_CHECK_LAUNCH();
@@ -340,17 +392,29 @@ extern "C" int femhip_segment_sum(const double* contrib, const int32_t* perm,
   return FEMHIP_OK;
 }
 
-extern "C" int femhip_zero_signature(const double* contrib, int64_t n, int64_t* out,
-                                     femhip_stream_t s) {
-  FEMHIP_REQUIRE(n >= 0 && out != nullptr, FEMHIP_EINVAL, "bad arguments");
-  hipStream_t st = reinterpret_cast<hipStream_t>(s);
-  FEMHIP_TRY_HIP(hipMemsetAsync(out, 0, 2 * sizeof(int64_t), st));
+extern "C" int femhip_zero_mask(const double* contrib, int64_t n, uint64_t* mask,
+                                femhip_stream_t s) {
+  FEMHIP_REQUIRE(n >= 0, FEMHIP_EINVAL, "zero mask: negative size");
   if (n == 0) return FEMHIP_OK;
-  FEMHIP_REQUIRE(contrib != nullptr, FEMHIP_EINVAL, "NULL contrib");
+  FEMHIP_REQUIRE(contrib != nullptr && mask != nullptr, FEMHIP_EINVAL, "zero mask: NULL argument");
+  zero_mask_kernel<<<(unsigned)ceil_div(n, kBlock), kBlock, 0,
+                     reinterpret_cast<hipStream_t>(s)>>>(contrib, n, mask);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+extern "C" int femhip_zero_mask_mismatch(const double* contrib, int64_t n, const uint64_t* mask,
+                                         int64_t* count, femhip_stream_t s) {
+  FEMHIP_REQUIRE(n >= 0 && count != nullptr, FEMHIP_EINVAL, "zero mask: bad arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(count);
+  zero_count_kernel<<<1, 1, 0, st>>>(cnt);
+  FEMHIP_CHECK_LAUNCH();
+  if (n == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(contrib != nullptr && mask != nullptr, FEMHIP_EINVAL, "zero mask: NULL argument");
   int64_t blocks = ceil_div(n, kBlock);
   if (blocks > 4096) blocks = 4096;
-  zero_signature_kernel<<<(unsigned)blocks, kBlock, 0, st>>>(
-      contrib, n, reinterpret_cast<unsigned long long*>(out));
+  zero_mask_mismatch_kernel<<<(unsigned)blocks, kBlock, 0, st>>>(contrib, n, mask, cnt);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }

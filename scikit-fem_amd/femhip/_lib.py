@@ -55,29 +55,34 @@ class FacetGeom(ctypes.Structure):
 
 
 class FusedInfo(ctypes.Structure):
-    _fields_ = [('nparts', _i32), ('nchunks', _i32), ('nrows', _i32), ('max_slots', _i32),
+    _fields_ = [('nparts', _i32), ('max_recs', _i32), ('nrows', _i32), ('max_slots', _i32),
                 ('max_nodes', _i32), ('max_rows', _i32), ('rec_words', _i32),
-                ('idbits', _i32), ('offbits', _i32), ('max_colors', _i32),
-                ('block', _i32), ('reserved', _i32), ('nrecs', _i64), ('nlocal', _i64),
+                ('idbits', _i32), ('offbits', _i32), ('block', _i32),
+                ('reserved0', _i32), ('reserved1', _i32), ('nrecs', _i64), ('nlocal', _i64),
                 ('dup', _f64)]
 
 
 class FusedDev(ctypes.Structure):
-    _fields_ = [('parts', _p), ('nodemap', _p), ('rows', _p), ('chunks', _p),
-                ('recs', _p), ('rec_elem', _p), ('scale', _p), ('info', FusedInfo)]
+    _fields_ = [('parts', _p), ('nodemap', _p), ('rows', _p), ('recs', _p),
+                ('rec_elem', _p), ('info', FusedInfo)]
 
 
-class SweepInfo(ctypes.Structure):
-    _fields_ = [('nblocks', _i32), ('nlevels', _i32), ('nchunks', _i32), ('nrows', _i32),
-                ('max_level_nodes', _i32), ('max_level_rows', _i32),
-                ('max_level_slots', _i32), ('max_colors', _i32), ('rec_words', _i32),
-                ('offbits', _i32), ('chunk', _i32), ('reserved', _i32), ('nrecs', _i64),
-                ('nlocal', _i64), ('dup', _f64)]
+class FusedState(ctypes.Structure):
+    _fields_ = [('s', _i32), ('sv', _i32), ('valid', _i32), ('redo', _i32),
+                ('maxk', ctypes.c_uint64), ('maxv', ctypes.c_uint64),
+                ('flags', _i32), ('reserved0', _i32), ('reserved1', ctypes.c_uint32 * 8),
+                ('nsusp', _i64), ('mismatch', _i64), ('zeros', _i64),
+                ('reserved', _i32 * 4)]
 
 
-class SweepDev(ctypes.Structure):
-    _fields_ = [('blocks', _p), ('levels', _p), ('nodemap', _p), ('rows', _p),
-                ('chunks', _p), ('recs', _p), ('rec_elem', _p), ('info', SweepInfo)]
+class FusedCheck(ctypes.Structure):
+    _fields_ = [('t', _p), ('dphi', _p), ('W', _p), ('nelems', _i64), ('Q', _i32),
+                ('reserved', _i32), ('zero_mask', _p), ('nzeros', _i64), ('queue', _p),
+                ('capacity', _i64), ('verdict', _p)]
+
+
+FUSED_BAD = 1
+FUSED_OVERFLOW = 2
 
 
 class FemhipError(RuntimeError):
@@ -104,30 +109,23 @@ SIGNATURES = {
     'femhip_rows_plan_build': (ctypes.c_int, [_p, _i32, _i64, _i64, _p, _p, _p,
                                               ctypes.c_size_t, _p]),
     'femhip_segment_sum': (ctypes.c_int, [_p, _p, _p, _i64, _p, _p]),
-    'femhip_zero_signature': (ctypes.c_int, [_p, _i64, _p, _p]),
+    'femhip_zero_mask': (ctypes.c_int, [_p, _i64, _p, _p]),
+    'femhip_zero_mask_mismatch': (ctypes.c_int, [_p, _i64, _p, _p, _p]),
     'femhip_fused_plan_build': (ctypes.c_int, [_i32, _i32, _i64, _i64, _p, _p, _p, _p,
-                                               _i32, _i32, _i32,
+                                               _i32, _i32, _i32, _i32,
                                                ctypes.POINTER(ctypes.c_void_p),
                                                ctypes.POINTER(FusedInfo)]),
-    'femhip_fused_plan_export': (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
+    'femhip_fused_plan_export': (ctypes.c_int, [_p, _p, _p, _p, _p, _p]),
     'femhip_fused_plan_free': (None, [_p]),
-    'femhip_fused_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(FusedInfo), _i32, _i32,
-                                                 _i32]),
-    'femhip_fused_assemble': (ctypes.c_int, [ctypes.POINTER(FusedDev), _i32, _i32, _p,
-                                             _i64, _p, _p, _p, _i32, _p]),
+    'femhip_fused_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(FusedInfo), _i32, _i32]),
+    'femhip_fused_assemble': (ctypes.c_int, [ctypes.POINTER(FusedDev), _p, _i32, _i32, _p,
+                                             _i64, _p, _p, _p, ctypes.POINTER(FusedCheck),
+                                             _i32, _p]),
     'femhip_batched_kappa_values': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _p, _i64, _p,
                                                    _p]),
     'femhip_hex2_elasticity_mfma': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space),
                                                    _f64, _f64, _p, _p, _i32, _p]),
     'femhip_segment_sum_em': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _p]),
-    'femhip_sweep_plan_build': (ctypes.c_int, [_i32, _i32, _i64, _i64, _p, _p, _p, _p,
-                                               _i32, _i32, ctypes.POINTER(ctypes.c_void_p),
-                                               ctypes.POINTER(SweepInfo)]),
-    'femhip_sweep_plan_export': (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p]),
-    'femhip_sweep_plan_free': (None, [_p]),
-    'femhip_sweep_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(SweepInfo), _i32, _i32]),
-    'femhip_sweep_assemble': (ctypes.c_int, [ctypes.POINTER(SweepDev), _i32, _i32, _p, _i64,
-                                             _p, _p, _p, _i32, _p]),
     'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
     'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
     'femhip_facet_local': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _i32, _p, _i32, _p, _p]),

@@ -1,54 +1,85 @@
-"""Autotuner of the fused P1 kernel's launch parameters.
+"""Autotuner of the fused P1 kernel's launch configuration.
 
-Tunables: the LDS budget per workgroup (CSR slots and local nodes), which
-sets the partition size -- elements per workgroup -- and the number of
-workgroups resident per CU; and the number of persistent workgroups.  The
-quadrature loops of the exact kernels are unrolled at compile time per
-(element, number of points), so they need no run-time choice.
+Tunables of ``FusedPlan`` / ``femhip_fused_assemble``:
 
-Results are cached in JSON (``FEMHIP_AUTOTUNE_CACHE``, default
-``~/.cache/femhip/autotune.json``) keyed by element kind, form and mesh-size
-bucket; ``DeviceBasis.fused_plan`` consults the cache.
+* the LDS budget per workgroup (CSR slots and local nodes) -- the partition
+  size, i.e. the elements each workgroup assembles, and how many
+  workgroups fit on a CU;
+* the workgroup size (elements per chunk: 512 or 1024);
+* the record order inside a partition (colour classes or element order);
+* the grid: one workgroup per partition, or a persistent grid of ``groups``
+  workgroups taking partitions from per-XCD work counters.
+
+The quadrature loops need no run-time choice: P1 gradients are constant per
+element (the fused kernel has no point loop) and the exact kernels unroll
+their point loop at compile time for each (element, number of points).
+
+``tune_fused`` times every candidate on the caller's basis and pattern and
+installs the fastest plan on the basis (later fused calls use it, with its
+grid).  Results are written to and read from a JSON cache only when one is
+named (``cache=`` or ``FEMHIP_AUTOTUNE_CACHE``); entries are keyed by the
+element kind, the form and a SHA-256 of the mesh connectivity and the CSR
+pattern, so a cached configuration is only ever applied to the mesh and
+pattern it was measured on (the partitioning changes the fixed-point
+rounding of the sums, within the stated tolerance, so results depend on the
+plan the cache selects).
 """
+import hashlib
 import json
-import math
 import os
 
+import numpy as np
 import torch
 
 from . import fused as _fused
 
-CACHE = os.environ.get('FEMHIP_AUTOTUNE_CACHE',
-                       os.path.join(os.path.expanduser('~'), '.cache', 'femhip',
-                                    'autotune.json'))
+CANDIDATES = [
+    # (slot_cap, node_cap, block, order, groups)
+    (4096, 768, 512, 1, 0),
+    (4096, 768, 512, 0, 0),
+    (4096, 768, 512, 1, 768),
+    (6144, 1152, 512, 1, 0),
+    (6144, 1152, 512, 1, 512),
+    (3072, 576, 512, 1, 0),
+    (8192, 1536, 1024, 1, 0),
+]
 
-CANDIDATES = [(4096, 768), (6144, 1024), (8192, 1280), (9000, 1500), (12000, 2000)]
+
+def _cache_path(cache):
+    return cache or os.environ.get('FEMHIP_AUTOTUNE_CACHE')
 
 
-def _key(kind, form, nnodes):
-    return '%s/%s/%d' % (kind, form, int(math.log2(max(nnodes, 1))))
+def plan_key(kind, form, t, indptr, indices):
+    h = hashlib.sha256()
+    for a in (t, indptr, indices):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return '%s/%s/%s' % (kind, form, h.hexdigest())
 
 
-def _load():
+def _load(path):
     try:
-        with open(CACHE) as f:
+        with open(path) as f:
             return json.load(f)
     except (OSError, ValueError):
         return {}
 
 
-def lookup(kind, form, nnodes):
-    """Cached (slot_cap, node_cap, groups) or None."""
-    v = _load().get(_key(kind, form, nnodes))
+def lookup(key, cache=None):
+    """Cached configuration (slot_cap, node_cap, block, order, groups) of a
+    mesh/pattern key, or None (also when no cache is named)."""
+    path = _cache_path(cache)
+    if not path:
+        return None
+    v = _load(path).get(key)
     return None if v is None else tuple(v['best'])
 
 
-def _store(key, rec):
-    db = _load()
+def _store(path, key, rec):
+    db = _load(path)
     db[key] = rec
     try:
-        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
-        with open(CACHE, 'w') as f:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        with open(path, 'w') as f:
             json.dump(db, f, indent=1)
     except OSError:
         pass
@@ -66,32 +97,46 @@ def _time(fn, steps):
     return s.elapsed_time(e) / steps
 
 
-def tune_fused(basis, form='laplace', candidates=None, groups=(0,), steps=20):
-    """Time the fused kernel for every candidate LDS budget (and persistent
-    group count) on ``basis``; keep and cache the fastest.  Returns
-    {'best': (slot_cap, node_cap, groups), 'ms': ..., 'trials': [...]}."""
+def tune_fused(basis, form='laplace', candidates=None, steps=20, cache=None):
+    """Time the fused kernel for every candidate configuration on ``basis``
+    (its validated pattern for ``form``), install the fastest plan on the
+    basis and, when a cache is named, store it there.  Returns {'best':
+    config, 'ms': ..., 'trials': [[config, ms, dup], ...]}."""
     from .models import laplace, mass
     f = {'laplace': laplace, 'mass': mass}[form]
     d = basis.device()
     f.assemble_device(basis)
-    plan = next(p for k, p in d.validated.items() if k[0] == form)
+    plan = d.fused_last.get(form)
+    if plan is None:
+        raise NotImplementedError("no fused pattern for %s on this basis" % form)
     ip, ix = plan.indptr.cpu().numpy(), plan.indices.cpu().numpy()
     out = torch.empty(plan.nnz, dtype=torch.float64, device=d.p.device)
+    ref = None
     trials, best = [], None
-    for sc, nc in (candidates or CANDIDATES):
+    for cfg in (candidates or CANDIDATES):
+        sc, nc, block, order, groups = cfg
         try:
-            fp = _fused.FusedPlan(basis.mesh, ip, ix, sc, nc)
-        except Exception:   # caps the mesh cannot satisfy
+            fp = _fused.FusedPlan(basis.mesh, ip, ix, sc, nc, block=block, order=order)
+        except (NotImplementedError, ValueError):   # caps the mesh cannot satisfy
             continue
-        for g in groups:
-            ms = _time(lambda: fp.assemble(form, d.p, out, groups=g), steps)
-            trials.append(((sc, nc, g), ms, fp.info.dup))
-            if best is None or ms < best[1]:
-                best = ((sc, nc, g), ms, fp)
+        if fp.lds_bytes() > 160 * 1024:
+            continue
+        ms = _time(lambda: fp.assemble(form, d.p, out, groups=groups), steps)
+        if ref is None:
+            ref = out.clone()
+        else:   # every configuration computes the same matrix (to rounding)
+            err = float((out - ref).abs().max() / ref.abs().max())
+            assert err <= 1e-12, (cfg, err)
+        trials.append((list(cfg), ms, fp.info.dup))
+        if best is None or ms < best[1]:
+            best = (cfg, ms, fp)
     if best is None:
         raise RuntimeError("no candidate configuration fits")
-    d.fused_plans[plan.signature] = best[2]
-    rec = {'best': list(best[0]), 'ms': best[1],
-           'trials': [[list(c), ms, dup] for c, ms, dup in trials]}
-    _store(_key(basis.mesh.kind, form, basis.N), rec)
+    fp = best[2]
+    fp.groups = best[0][4]
+    d.fused_plans[plan.signature] = fp
+    rec = {'best': list(best[0]), 'ms': best[1], 'trials': trials}
+    path = _cache_path(cache)
+    if path:
+        _store(path, plan_key(d.kind, form, basis.mesh.t, ip, ix), rec)
     return rec

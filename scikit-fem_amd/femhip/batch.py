@@ -5,6 +5,7 @@ mesh, assembled into one ``[S, nnz]`` value array that shares a single CSR
 pattern: the connectivity pattern of the mesh (no value-dependent pruning,
 so every sample's reference pattern is a subset of it; slots the reference
 drops for a particular sample hold values that are 0 up to rounding).
+``export`` turns one sample into the reference's pruned CSR matrix.
 
 ``MeshBatch`` -- many independent P1 meshes of uneven size assembled
 (Laplace + unit load) by ONE launch of the fused kernel over the
@@ -30,6 +31,28 @@ class KappaSamples:
         self.K1 = d.local_bilinear('laplace')   # unit-coefficient local entries
         self.nnz = self.plan.nnz
         self.indptr, self.indices = self.plan.indptr, self.plan.indices
+
+    def export(self, s, kappa_s, values):
+        """Sample ``s`` as the reference's CSR matrix (``DeviceCSR``): the
+        pattern is the one ``kappa_laplace.assemble`` returns for
+        ``kappa_s`` (its exact-zero entries found in the reference's
+        arithmetic), the values are row ``s`` of the batch output
+        ``values`` gathered into it."""
+        from .device import DeviceCSR
+        d = self.d
+        kappa_s = kappa_s.reshape(-1).to(torch.float64).contiguous()
+        plan = d.plan_for(d.local_bilinear('kappa_laplace', kappa_s))
+        keys_c = self._keys(self.indptr, self.indices)
+        keys_p = self._keys(plan.indptr, plan.indices)
+        pos = torch.searchsorted(keys_c, keys_p)
+        data = values[s].index_select(0, pos)
+        return DeviceCSR(plan.indptr, plan.indices, data, (d.N, d.N), plan)
+
+    def _keys(self, indptr, indices):
+        n = indptr.numel() - 1
+        rows = torch.repeat_interleave(torch.arange(n, device=indptr.device),
+                                       (indptr[1:] - indptr[:-1]).long())
+        return rows * n + indices.long()
 
     def assemble(self, kappa, out=None):
         """kappa: (S, E) float64 cuda tensor -> values (S, nnz)."""

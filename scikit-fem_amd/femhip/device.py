@@ -8,8 +8,8 @@ Generic path (every element / form):
 2. ``femhip_local_bilinear`` -- all local entries, bit-identical to the
    reference's NumPy values, in its COO order;
 3. symbolic plan -- CSR pattern of the nonzero entries + per-slot
-   contribution ranges, built once and revalidated on every call with an
-   order-independent signature of the zero positions;
+   contribution ranges, built once and revalidated on every call by an
+   exact comparison of the exact-zero positions with the plan's bit mask;
 4. ``femhip_segment_sum`` -- deterministic per-slot sums.
 
 The fused P1 path (``femhip.fused``) replaces 2-4 for P1 triangles and
@@ -27,6 +27,20 @@ def _dev():
     if not torch.cuda.is_available():
         raise RuntimeError("femhip needs a ROCm GPU (torch.cuda is unavailable)")
     return torch.device('cuda')
+
+
+def check_host_inputs(p, t, dofs, N):
+    """Validation before anything is uploaded or indexed on the device:
+    finite node coordinates, connectivity inside the node array and DOF
+    indices inside [0, N) (a DOF outside the matrix would corrupt the
+    symbolic phase).  Raises ValueError."""
+    if not np.all(np.isfinite(p)):
+        raise ValueError("mesh coordinates are not finite")
+    if t.size and (t.min() < 0 or t.max() >= p.shape[1]):
+        raise ValueError("element connectivity index out of range [0, %d)" % p.shape[1])
+    if dofs.size and (dofs.min() < 0 or dofs.max() >= N):
+        raise ValueError("element DOF index out of range [0, %d): min %d, max %d"
+                         % (N, int(dofs.min()), int(dofs.max())))
 
 
 def to_dev(a, dtype):
@@ -90,23 +104,28 @@ class DeviceCSR:
 
 class Plan:
     """Symbolic result for one (basis, form): the CSR pattern and the
-    reduction plan (``perm``/``seg``), plus the signature of the exact-zero
-    positions it was built from."""
+    reduction plan (``perm``/``seg``), the bit mask of the exact-zero
+    contributions it was built from (``mask``) and its serial number on the
+    basis (``signature``)."""
 
     def __init__(self, indptr, indices, perm, seg, nnz, signature):
         self.indptr, self.indices, self.perm, self.seg = indptr, indices, perm, seg
         self.nnz, self.signature = nnz, signature
         self.fused = None   # fused-kernel plan, built on demand
         self.host_pattern = None   # (indptr, indices) on the host, after the first transfer
+        self.mask = None    # exact-zero mask of the contributions
+        self.nzeros = 0     # number of exact-zero contributions
+        self.seen = 0       # exact-path calls that matched this pattern
 
 
 class DeviceBasis:
 
     def __init__(self, basis):
+        mesh, elem = basis.mesh, basis.elem
+        check_host_inputs(mesh.p, mesh.t, basis.element_dofs, basis.N)
         lib = _lib.load()
         self.lib = lib
         self.basis = basis
-        mesh, elem = basis.mesh, basis.elem
         self.kind = mesh.kind
         self.dim = mesh.dim
         self.E = mesh.nelements
@@ -130,10 +149,13 @@ class DeviceBasis:
                                 self.dphi.data_ptr())
         self._G = None
         self.p_version = 0
-        self.plans = {}          # zero signature -> Plan
-        self.validated = {}      # (form, params, coefficient token, p version) -> Plan
-        self.fused_plans = {}    # zero signature -> FusedPlan
-        self.unfusable = {}      # zero signature -> why the fused plan was refused
+        self.plans = {}          # serial number -> Plan
+        self.validated = {}      # (form, params, coefficient shape, p version) -> Plan
+        self.validated_coef = {}  # same key -> the coefficient it was validated with
+        self.fused_plans = {}    # plan serial -> FusedPlan
+        self.fused_last = {}     # form -> the last plan its fused path used
+        self.last_verdict = None
+        self.unfusable = {}      # plan serial -> why the fused plan was refused
         self.row_plan = None
 
     # -- coordinates ---------------------------------------------------------
@@ -196,7 +218,7 @@ class DeviceBasis:
         _lib.check(self.lib.femhip_pattern_workspace(ncontrib, ctypes.byref(nb)))
         return torch.empty(int(nb.value), dtype=torch.uint8, device=self.p.device), int(nb.value)
 
-    def build_plan(self, contrib, signature=None):
+    def build_plan(self, contrib, serial=None):
         C = self.nbfun * self.nbfun * self.E
         dev = self.p.device
         ws, wsb = self._workspace(C)
@@ -212,45 +234,67 @@ class DeviceBasis:
             _lib.stream_ptr()))
         n = int(nnz.value)
         del ws
-        return Plan(indptr, indices[:n].clone(), perm, seg[:n + 1].clone(), n, signature)
+        plan = Plan(indptr, indices[:n].clone(), perm, seg[:n + 1].clone(), n, serial)
+        # contrib None: the connectivity pattern (no contribution pruned)
+        if contrib is not None:
+            plan.mask = self.zero_mask(contrib)
+            # the kept contributions end at seg[nnz] (zeros sort last)
+            plan.nzeros = C - int(plan.seg[n].item())
+        return plan
 
-    def zero_signature(self, contrib):
-        out = torch.empty(2, dtype=torch.int64, device=self.p.device)
-        _lib.check(self.lib.femhip_zero_signature(_lib.ptr(contrib), contrib.numel(),
-                                                  _lib.ptr(out), _lib.stream_ptr()))
-        return tuple(int(v) for v in out.cpu().tolist())
+    def zero_mask(self, contrib):
+        """Exact-zero mask of the contributions (one bit each)."""
+        n = contrib.numel()
+        mask = torch.empty(max((n + 63) // 64, 1), dtype=torch.int64, device=self.p.device)
+        _lib.check(self.lib.femhip_zero_mask(_lib.ptr(contrib), n, _lib.ptr(mask),
+                                             _lib.stream_ptr()))
+        return mask
+
+    def zero_mismatch(self, contrib, mask):
+        """Number of contributions whose exact-zero bit differs from mask."""
+        out = torch.empty(1, dtype=torch.int64, device=self.p.device)
+        _lib.check(self.lib.femhip_zero_mask_mismatch(_lib.ptr(contrib), contrib.numel(),
+                                                      _lib.ptr(mask), _lib.ptr(out),
+                                                      _lib.stream_ptr()))
+        return int(out.item())
 
     def plan_for(self, contrib):
-        """Plan whose exact-zero positions match ``contrib`` (patterns are
-        shared by every form with the same zero set on this basis)."""
-        sig = self.zero_signature(contrib)
-        plan = self.plans.get(sig)
-        if plan is None:
-            plan = self.build_plan(contrib, sig)
-            self.plans[sig] = plan
+        """Plan whose exact-zero positions equal those of ``contrib``,
+        compared bit for bit with each known plan's mask (most recent first;
+        patterns are shared by every form with the same zero set on this
+        basis), else a new one."""
+        for serial in sorted(self.plans, reverse=True):
+            plan = self.plans[serial]
+            if self.zero_mismatch(contrib, plan.mask) == 0:
+                return plan
+        serial = len(self.plans)
+        plan = self.build_plan(contrib, serial)
+        self.plans[serial] = plan
         return plan
 
     def fused_plan(self, plan, form='laplace'):
-        """Fused plan for a CSR pattern (the autotuner's cached LDS budget
-        for this element kind / form / size when there is one)."""
-        import os
-        from .fused import FusedPlan, SweepPlan
+        """Fused plan for a CSR pattern (the configuration the autotuner
+        cached for this mesh and pattern when a cache is named, else the
+        default one)."""
+        from .fused import FusedPlan
         from . import autotune
         fp = self.fused_plans.get(plan.signature)
         if fp is None:
             torch.cuda.current_stream().synchronize()
             ip, ix = plan.indptr.cpu().numpy(), plan.indices.cpu().numpy()
-            variant = os.environ.get('FEMHIP_FUSED_VARIANT', 'blocks')
-            fp = None
-            if variant == 'sweep':
+            cfg = None
+            if autotune._cache_path(None):
+                cfg = autotune.lookup(autotune.plan_key(self.kind, form, self.basis.mesh.t,
+                                                        ip, ix))
+            if cfg is not None:
                 try:
-                    fp = SweepPlan(self.basis.mesh, ip, ix)
-                except NotImplementedError:
-                    fp = None      # mesh the sweep cannot level: block variant
+                    fp = FusedPlan(self.basis.mesh, ip, ix, cfg[0], cfg[1], block=cfg[2],
+                                   order=cfg[3])
+                    fp.groups = cfg[4]
+                except (NotImplementedError, ValueError):
+                    fp = None    # a cached configuration this build refuses
             if fp is None:
-                caps = autotune.lookup(self.kind, form, self.N)
-                args = () if caps is None else caps[:2]
-                fp = FusedPlan(self.basis.mesh, ip, ix, *args)
+                fp = FusedPlan(self.basis.mesh, ip, ix)
             self.fused_plans[plan.signature] = fp
         return fp
 
@@ -277,17 +321,61 @@ class DeviceBasis:
         return out
 
     # -- assembly ------------------------------------------------------------
-    def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, token=None,
-                          path='auto'):
+    def _validated_plan(self, vkey, kappa):
+        """The plan validated for these inputs, or None: the key holds the
+        form, its parameters and the coordinate version; a coefficient must
+        equal (bit for bit, on the device) the one the plan was validated
+        with."""
+        plan = self.validated.get(vkey)
+        if plan is None:
+            return None
+        ref = self.validated_coef.get(vkey)
+        if kappa is None:
+            return plan if ref is None else None
+        if ref is None or not torch.equal(ref, kappa):
+            return None
+        return plan
+
+    def _validate(self, vkey, plan, kappa):
+        self.validated[vkey] = plan
+        self.validated_coef[vkey] = None if kappa is None else kappa.detach().clone()
+
+    def fused_try(self, name, kappa=None, load=None):
+        """The fused kernel on the last pattern validated for ``name``, the
+        call itself validated against that pattern: the CSR values
+        (DeviceCSR) when this input prunes exactly the plan's entries,
+        else None (the verdict is kept in ``last_verdict``)."""
+        plan = self.fused_last.get(name)
+        if plan is None:
+            return None
+        fp = self.fused_plans[plan.signature]
+        out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
+                          device=self.p.device)[:plan.nnz]
+        chk = fp.check_struct(self, plan.mask, plan.nzeros)
+        fp.assemble(name, self.p, out, kappa, load=load, check=chk, groups=fp.groups)
+        v = fp.verdict.cpu().tolist()      # synchronises
+        self.last_verdict = v
+        if v[0] != 1:
+            return None
+        return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N), plan)
+
+    def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, path='auto'):
         """CSR assembly of a built-in form.
 
-        The first call for given inputs runs the exact path (local entries in
-        the reference's arithmetic -> pattern -> segmented sums) and returns
-        its result; it also validates the pattern for these inputs.  Later
-        calls with the same inputs go through the fused kernel when the
-        element/form has one (P1 triangles/tetrahedra), the MFMA kernel for
-        vector Q2-hex elasticity, else the exact path on the validated
-        pattern (no exact-zero signature pass for the scalar forms).
+        P1 triangles/tetrahedra (laplace, mass, per-element kappa_laplace):
+        the fused kernel on the last pattern validated for the form, the
+        call validated in the same launch sequence (entries within the
+        rounding bound of zero recomputed in the reference's arithmetic and
+        compared with the pattern's zero set); when the check fails, or on
+        the first call, the exact path (local entries in the reference's
+        arithmetic -> pattern matched by its exact-zero mask or built ->
+        segmented sums).  A pattern gets its fused plan the first time the
+        form has none, else when it is matched a second time.
+
+        Vector Q2-hex elasticity (MFMA kernel) and the other scalar forms
+        (exact path without the zero-mask pass) reuse the pattern validated
+        for the same form, parameters, coordinates and coefficient (compared
+        bit for bit on the device).
         """
         from . import fused
         fusable = (path != 'generic' and fused.eligible(self.basis, name)
@@ -295,17 +383,20 @@ class DeviceBasis:
         mfma = (path != 'generic' and name == 'linear_elasticity' and self.kind == 'hex'
                 and self.basis.elem.vector and self.basis.elem.nbs == 27 and self.Q == 27)
         # the other scalar forms: the exact path without the exact-zero
-        # signature (same inputs, same zeros)
+        # comparison (same inputs, same zeros)
         scalar = (path != 'generic' and not fusable and not self.basis.elem.vector
                   and name in ('laplace', 'mass', 'kappa_laplace', 'coef_mass'))
-        vkey = (name, lam, mu, token, self.p_version)
-        if (fusable or mfma or scalar) and vkey in self.validated:
-            plan = self.validated[vkey]
+        vkey = (name, lam, mu, None if kappa is None else tuple(kappa.shape), self.p_version)
+        if fusable:
+            A = self.fused_try(name, kappa)
+            if A is not None:
+                self.validated[vkey] = A.plan
+                return A
+        plan = self._validated_plan(vkey, kappa) if (mfma or scalar) else None
+        if plan is not None:
             out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
                               device=self.p.device)[:plan.nnz]
-            if fusable:
-                self.fused_plans[plan.signature].assemble(name, self.p, out, kappa)
-            elif mfma:
+            if mfma:
                 self.elasticity_mfma(lam, mu, plan, out)
             else:
                 self.segment_sum(self.local_bilinear(name, kappa), plan.perm, plan.seg,
@@ -315,16 +406,20 @@ class DeviceBasis:
         plan = self.plan_for(contrib)
         data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
         if fusable and plan.signature not in self.unfusable:
-            try:
-                self.fused_plan(plan, name)
-                self.validated[vkey] = plan
-            except (NotImplementedError, ValueError) as exc:
-                # a mesh the fused plan cannot encode (e.g. a CSR row longer
-                # than 31 entries, more than 64 colours): this pattern stays
-                # on the exact path, which is always available
-                self.unfusable[plan.signature] = str(exc)
+            plan.seen += 1
+            if plan.signature in self.fused_plans or name not in self.fused_last \
+                    or plan.seen >= 2:
+                try:
+                    self.fused_plan(plan, name)
+                    self.fused_last[name] = plan
+                    self.validated[vkey] = plan
+                except (NotImplementedError, ValueError) as exc:
+                    # a mesh the fused plan cannot encode (e.g. a CSR row
+                    # longer than 31 entries): this pattern stays on the
+                    # exact path, which is always available
+                    self.unfusable[plan.signature] = str(exc)
         elif mfma or scalar:
-            self.validated[vkey] = plan
+            self._validate(vkey, plan, kappa)
         return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N), plan)
 
     def elasticity_mfma(self, lam, mu, plan, out, loc=None):
@@ -379,10 +474,11 @@ class FacetDeviceBasis(DeviceBasis):
     "element" per facet (``element_dofs[:, tind]``)."""
 
     def __init__(self, basis):
+        mesh, elem = basis.mesh, basis.elem
+        check_host_inputs(mesh.p, mesh.t, basis.element_dofs, basis.N)
         lib = _lib.load()
         self.lib = lib
         self.basis = basis
-        mesh, elem = basis.mesh, basis.elem
         self.kind = mesh.kind
         self.dim = mesh.dim
         self.E = basis.nelems              # one local block per facet
@@ -406,7 +502,10 @@ class FacetDeviceBasis(DeviceBasis):
         self.p_version = 0
         self.plans = {}
         self.validated = {}
+        self.validated_coef = {}
         self.fused_plans = {}
+        self.fused_last = {}
+        self.last_verdict = None
         self.unfusable = {}
         self.row_plan = None
 
@@ -482,9 +581,8 @@ class FacetDeviceBasis(DeviceBasis):
             self._normals = out
         return self._normals
 
-    def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, token=None,
-                          path='auto'):
-        """Exact path: local entries -> pattern (cached by zero signature)
+    def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, path='auto'):
+        """Exact path: local entries -> pattern (matched by exact-zero mask)
         -> segmented sums."""
         if self.E == 0:
             self.local_bilinear(name, kappa)   # form check only

@@ -49,21 +49,21 @@ class Form:
             raise NotImplementedError("only float64 assembly is accelerated")
 
 
-_KAPPA_CACHE = {}
+_UPLOAD_CACHE = {}
 
 
 def _kappa_tensor(w, basis, name='kappa'):
     """Coefficient array of ``w[name]`` as a device tensor: (E,) per element
     or (E, Q) per quadrature point (the reference accepts an (E, Q) array or
-    a DiscreteField, ``form.py:92-121``), and a token identifying its
-    contents (content hash for host arrays, storage and version counter for
-    device tensors).  An (E, Q) array with identical columns is treated as
-    per-element."""
-    kt, token = _kappa_tensor_impl(w, basis, name)
-    return kt, token
+    a DiscreteField, ``form.py:92-121``); a 1-D array of length N is a DOF
+    vector, interpolated at the quadrature points of ``basis``
+    (``form.py:106-108``).  An (E, Q) array with identical columns is treated
+    as per-element.
 
-
-def _kappa_tensor_impl(w, basis, name):
+    Uploads of host arrays are cached by a 128-bit content digest (the
+    interpolated DOF vectors on the basis itself, with its coordinate
+    version).  Plan reuse does not rely on these caches: the assembly
+    compares the coefficient with the one its plan was validated for."""
     if name not in w:
         raise ValueError("Attribute '%s' not found in 'w'." % name)
     k = w[name]
@@ -72,38 +72,37 @@ def _kappa_tensor_impl(w, basis, name):
     E, Q = basis.nelems, basis.W.shape[0]
     if isinstance(k, torch.Tensor):
         kt = k.detach().to(dtype=torch.float64, device='cuda')
-        token = ('torch', kt.data_ptr(), int(k._version), tuple(kt.shape))
     else:
+        import hashlib
         k = np.asarray(k, dtype=np.float64)
         if k.ndim == 1:
             # a DOF vector: interpolated at the quadrature points
-            # (form.py:106-108)
+            # (form.py:106-108), cached per basis and coordinate version
             if k.shape[0] != basis.N:
                 raise ValueError("Input array has wrong size.")
-            import hashlib
-            token = ('dofs', k.shape, hashlib.blake2b(k.tobytes(), digest_size=16).hexdigest())
-            kt = _KAPPA_CACHE.get(token)
+            dev = basis.device()
+            key = (hashlib.blake2b(k.tobytes(), digest_size=16).hexdigest(), dev.p_version)
+            cache = getattr(dev, 'coef_cache', None)
+            kt = None if cache is None else cache.get(key)
             if kt is None:
-                kt = basis.interpolate_device(k, grad=False).value
-                _KAPPA_CACHE.clear()
-                _KAPPA_CACHE[token] = kt
-            return kt.reshape(E, Q).contiguous(), token
+                kt = basis.interpolate_device(k, grad=False).value.reshape(E, Q).contiguous()
+                dev.coef_cache = {key: kt}
+            return kt
         if k.ndim == 2 and k.shape == (E, Q) and Q > 1 and np.all(k == k[:, :1]):
             k = k[:, :1]
         k = np.ascontiguousarray(k)
-        import hashlib
-        token = ('np', k.shape, hashlib.blake2b(k.tobytes(), digest_size=16).hexdigest())
-        kt = _KAPPA_CACHE.get(token)
+        key = (k.shape, hashlib.blake2b(k.tobytes(), digest_size=16).hexdigest())
+        kt = _UPLOAD_CACHE.get(key)
         if kt is None:
             kt = torch.from_numpy(k).to('cuda')
-            _KAPPA_CACHE.clear()
-            _KAPPA_CACHE[token] = kt
+            _UPLOAD_CACHE.clear()
+            _UPLOAD_CACHE[key] = kt
     if kt.dim() == 2 and kt.shape == (E, 1):
-        return kt.reshape(E).contiguous(), token
+        return kt.reshape(E).contiguous()
     if kt.dim() == 2 and kt.shape == (E, Q):
-        return kt.reshape(E, Q).contiguous(), token
+        return kt.reshape(E, Q).contiguous()
     if kt.dim() == 1 and kt.shape[0] == E:
-        return kt.contiguous(), token
+        return kt.contiguous()
     raise ValueError("Quadrature mismatch: '%s' should have shape (E, Q) = "
                      "(%d, %d) or (E, 1)" % (name, E, Q))
 
@@ -128,9 +127,9 @@ class BilinearForm(Form):
             return A
         dev = ubasis.device()
         if name in ('kappa_laplace', 'coef_mass'):
-            kappa, token = _kappa_tensor({**self.params, **kwargs}, ubasis,
-                                         params.get('coef', 'kappa'))
-            A = dev.assemble_bilinear(name, kappa=kappa, token=token, path=path)
+            kappa = _kappa_tensor({**self.params, **kwargs}, ubasis,
+                                  params.get('coef', 'kappa'))
+            A = dev.assemble_bilinear(name, kappa=kappa, path=path)
         elif name == 'linear_elasticity':
             A = dev.assemble_bilinear(name, lam=params['Lambda'], mu=params['Mu'], path=path)
         else:
@@ -149,8 +148,8 @@ class BilinearForm(Form):
         dev = ubasis.device()
         kappa = None
         if name in ('kappa_laplace', 'coef_mass'):
-            kappa, _ = _kappa_tensor({**self.params, **kwargs}, ubasis,
-                                     params.get('coef', 'kappa'))
+            kappa = _kappa_tensor({**self.params, **kwargs}, ubasis,
+                                  params.get('coef', 'kappa'))
         c = dev.local_bilinear(name, kappa=kappa, lam=params.get('Lambda', 0.0),
                                mu=params.get('Mu', 0.0))
         nb = ubasis.Nbfun
@@ -198,7 +197,7 @@ class LinearForm(Form):
             raise ValueError("LinearForm takes one basis")
         name, scale, params = self.spec()
         if name == 'coef_load':
-            kappa, _ = _kappa_tensor({**self.params, **kwargs}, basis, params['coef'])
+            kappa = _kappa_tensor({**self.params, **kwargs}, basis, params['coef'])
             b = basis.device().assemble_linear('coef_load', kappa=kappa)
         elif name == 'unit_load':
             b = basis.device().assemble_linear('unit_load')

@@ -1,33 +1,34 @@
 """Fused P1 path: plan construction (host C++ via the C ABI) and launch.
 
 The plan depends only on the mesh topology, the node coordinates used for
-the Morton order (any order is valid; only locality depends on it) and the
-CSR pattern, so one plan serves every later assembly with that pattern --
-laplace, kappa_laplace with any kappa that prunes the same entries, and the
-adjoint's forward pass.
+the spatial partition (any coordinates give a valid plan; only locality
+depends on them) and the CSR pattern, so one plan serves every later
+assembly with that pattern -- laplace, kappa_laplace with any kappa that
+prunes the same entries, the adjoint's forward pass.  Whether a call's
+inputs prune the same entries is checked during the call itself
+(``check``): entries within the rounding bound of zero are recomputed in
+the reference's arithmetic and compared with the plan's zero set.
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
 
 from . import _lib
 
-import os
-
 # LDS budget per workgroup: three workgroups per CU (3 x 53 KB of 160 KB:
 # 4096 int64 row accumulators, 768 staged nodes); FEMHIP_FUSED_SLOT_CAP /
 # FEMHIP_FUSED_NODE_CAP override it (tuning).
 DEFAULT_SLOT_CAP = int(os.environ.get('FEMHIP_FUSED_SLOT_CAP', 4096))
 DEFAULT_NODE_CAP = int(os.environ.get('FEMHIP_FUSED_NODE_CAP', 768))
-# workgroup size: 512, or 1024 (fixed-point accumulation only)
+# workgroup size (elements per chunk): 512 or 1024
 BLOCK = int(os.environ.get('FEMHIP_FUSED_BLOCK', 512))
-# summation mode: 'fixed' (fixed-point LDS integer atomics) or 'rounds'
-# (floating-point colour rounds); both deterministic
-DEFAULT_ACCUM = os.environ.get('FEMHIP_FUSED_ACCUM', 'fixed')
-FX_UNKNOWN = -2 ** 31
+# record order inside a partition: 0 = colour classes, 1 = element order
+DEFAULT_ORDER = int(os.environ.get('FEMHIP_FUSED_ORDER', 1))
 
 _FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS, 'kappa_laplace': _lib.KAPPA_LAPLACE}
+STATE_BYTES = ctypes.sizeof(_lib.FusedState)
 
 
 def eligible(basis, name):
@@ -37,17 +38,16 @@ def eligible(basis, name):
 
 
 class FusedPlan:
-    """Device arrays of one fused plan plus its info struct."""
+    """Device arrays of one fused plan, its info struct, the per-form
+    accumulation states and the suspect queue of the validation."""
 
     def __init__(self, mesh, indptr, indices, slot_cap=DEFAULT_SLOT_CAP,
-                 node_cap=DEFAULT_NODE_CAP, accum=None, block=None):
+                 node_cap=DEFAULT_NODE_CAP, block=None, order=None):
         lib = _lib.load()
-        self.accum = accum or DEFAULT_ACCUM
-        if self.accum not in ('fixed', 'rounds'):
-            raise ValueError("accum must be 'fixed' or 'rounds'")
         block = int(block or BLOCK)
-        if block not in (512, 1024) or (block == 1024 and self.accum != 'fixed'):
-            raise ValueError("block must be 512, or 1024 with fixed-point accumulation")
+        if block not in (512, 1024):
+            raise ValueError("block must be 512 or 1024")
+        order = DEFAULT_ORDER if order is None else int(order)
         t = np.ascontiguousarray(mesh.t, dtype=np.int32)
         p = np.ascontiguousarray(mesh.p, dtype=np.float64)
         ip = np.ascontiguousarray(indptr, dtype=np.int32)
@@ -58,23 +58,25 @@ class FusedPlan:
             t.shape[0], p.shape[0], t.shape[1], p.shape[1],
             t.ctypes.data_as(ctypes.c_void_p), p.ctypes.data_as(ctypes.c_void_p),
             ip.ctypes.data_as(ctypes.c_void_p), ix.ctypes.data_as(ctypes.c_void_p),
-            slot_cap, node_cap, block, ctypes.byref(handle), ctypes.byref(info)))
+            int(slot_cap), int(node_cap), block, order, ctypes.byref(handle),
+            ctypes.byref(info)))
         try:
             parts = np.empty(8 * info.nparts, np.int32)
             nodemap = np.empty(info.nlocal, np.int32)
             rows = np.empty(2 * info.nrows, np.int32)
-            chunks = np.empty(2 * info.nchunks, np.int32)
             recs = np.empty(info.rec_words * info.nrecs, np.uint32)
             rec_elem = np.empty(info.nrecs, np.int32)
-            arrs = (parts, nodemap, rows, chunks, recs, rec_elem)
+            arrs = (parts, nodemap, rows, recs, rec_elem)
             _lib.check(lib.femhip_fused_plan_export(
                 handle, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs]))
         finally:
             lib.femhip_fused_plan_free(handle)
         self.caps = (slot_cap, node_cap)
+        self.order = order
+        self.groups = 0          # launch grid (the autotuner may set it)
         self.info = info
-        self.host = dict(parts=parts, nodemap=nodemap, rows=rows, chunks=chunks,
-                         recs=recs, rec_elem=rec_elem)
+        self.host = dict(parts=parts, nodemap=nodemap, rows=rows, recs=recs,
+                         rec_elem=rec_elem)
         self.nnz = int(indptr[-1])
         self.nnodes = p.shape[1]
         self.nelems = t.shape[1]
@@ -87,19 +89,22 @@ class FusedPlan:
         self.parts = torch.from_numpy(h['parts']).to(dev)
         self.nodemap = torch.from_numpy(h['nodemap']).to(dev)
         self.rows = torch.from_numpy(h['rows']).to(dev)
-        self.chunks = torch.from_numpy(h['chunks']).to(dev)
         self.recs = torch.from_numpy(h['recs'].view(np.int32)).to(dev)
         self.rec_elem = torch.from_numpy(h['rec_elem']).to(dev)
-        self.scale = None
-        if self.accum == 'fixed':
-            # exponent pair per (form, partition); owned by the kernel
-            self.scale = torch.full((6 * self.info.nparts,), FX_UNKNOWN, dtype=torch.int32,
-                                    device=dev)
+        self.states = {}
+        self.queue = None
+        self.verdict = torch.zeros(4, dtype=torch.int32, device=dev)
         self.dev = _lib.FusedDev(self.parts.data_ptr(), self.nodemap.data_ptr(),
-                                 self.rows.data_ptr(), self.chunks.data_ptr(),
-                                 self.recs.data_ptr(), self.rec_elem.data_ptr(),
-                                 None if self.scale is None else self.scale.data_ptr(),
-                                 self.info)
+                                 self.rows.data_ptr(), self.recs.data_ptr(),
+                                 self.rec_elem.data_ptr(), self.info)
+
+    def state(self, name):
+        """Accumulation state of form ``name`` (device, zeroed at creation)."""
+        st = self.states.get(name)
+        if st is None:
+            st = torch.zeros(STATE_BYTES, dtype=torch.uint8, device=self.parts.device)
+            self.states[name] = st
+        return st
 
     @classmethod
     def concat(cls, plans):
@@ -107,159 +112,97 @@ class FusedPlan:
         element ids are shifted so that the concatenated coordinate array,
         CSR value array and load vector of the meshes can be assembled in a
         single launch."""
-        fmt = {(q.info.rec_words, q.info.idbits, q.info.offbits, q.dim) for q in plans}
+        fmt = {(q.info.rec_words, q.info.idbits, q.info.offbits, q.dim, q.info.block)
+               for q in plans}
         if len(fmt) != 1:
             raise NotImplementedError("meshes need the same record format to be batched")
         out = cls.__new__(cls)
-        out.accum = plans[0].accum
-        parts, nodemap, rows, chunks, recs, rec_elem = [], [], [], [], [], []
-        off_node = off_nnz = off_map = off_row = off_chunk = off_rec = off_el = 0
+        out.order = plans[0].order
+        out.groups = 0
+        parts, nodemap, rows, recs, rec_elem = [], [], [], [], []
+        off_node = off_nnz = off_map = off_row = off_rec = off_el = 0
         info = _lib.FusedInfo()
-        for f in ('max_slots', 'max_nodes', 'max_rows', 'max_colors'):
+        for f in ('max_slots', 'max_nodes', 'max_rows', 'max_recs'):
             setattr(info, f, max(getattr(q.info, f) for q in plans))
         q0 = plans[0].info
         info.rec_words, info.idbits, info.offbits, info.block = (
             q0.rec_words, q0.idbits, q0.offbits, q0.block)
-        self_offsets = []
+        offsets = []
         for q in plans:
             h = q.host
             pt = h['parts'].reshape(-1, 8).copy()
             pt[:, 0] += off_map
             pt[:, 2] += off_row
-            pt[:, 5] += off_chunk
+            pt[:, 5] += off_rec
             parts.append(pt.ravel())
             nodemap.append(h['nodemap'] + off_node)
             rw = h['rows'].reshape(-1, 2).copy()
             rw[:, 0] += off_nnz
             rows.append(rw.ravel())
-            ch = h['chunks'].reshape(-1, 2).copy()
-            ch[:, 0] += off_rec
-            chunks.append(ch.ravel())
             recs.append(h['recs'])
             rec_elem.append(h['rec_elem'] + off_el)
-            self_offsets.append((off_node, off_nnz, off_el))
+            offsets.append((off_node, off_nnz, off_el))
             off_node += q.nnodes
             off_nnz += q.nnz
             off_el += q.nelems
             off_map += len(h['nodemap'])
             off_row += len(h['rows']) // 2
-            off_chunk += len(h['chunks']) // 2
             off_rec += len(h['rec_elem'])
         out.host = dict(parts=np.concatenate(parts), nodemap=np.concatenate(nodemap),
-                        rows=np.concatenate(rows), chunks=np.concatenate(chunks),
-                        recs=np.concatenate(recs), rec_elem=np.concatenate(rec_elem))
+                        rows=np.concatenate(rows), recs=np.concatenate(recs),
+                        rec_elem=np.concatenate(rec_elem))
         info.nparts = sum(q.info.nparts for q in plans)
-        info.nchunks = off_chunk
         info.nrows = off_row
         info.nrecs = off_rec
         info.nlocal = off_map
         info.dup = float(off_rec) / max(1, off_el)
         out.info = info
+        out.caps = plans[0].caps
         out.nnz, out.nnodes, out.nelems, out.dim = off_nnz, off_node, off_el, plans[0].dim
-        out.offsets = self_offsets
+        out.offsets = offsets
         out._upload()
         return out
 
     def lds_bytes(self, load=False):
         return int(_lib.load().femhip_fused_lds_bytes(ctypes.byref(self.info), self.dim,
-                                                      int(load), int(self.accum == 'fixed')))
+                                                      int(load)))
 
     def stats(self):
         i = self.info
-        return dict(accum=self.accum, lds_bytes=self.lds_bytes(),
-                    parts=i.nparts, records=int(i.nrecs), dup=i.dup,
-                    max_colors=i.max_colors, rec_bytes=4 * i.rec_words,
-                    max_slots=i.max_slots, max_nodes=i.max_nodes,
+        return dict(lds_bytes=self.lds_bytes(), parts=i.nparts, records=int(i.nrecs),
+                    dup=i.dup, rec_bytes=4 * i.rec_words, max_slots=i.max_slots,
+                    max_nodes=i.max_nodes, order=('colour', 'element')[self.order],
                     plan_bytes=int(sum(t.numel() * t.element_size() for t in (
-                        self.parts, self.nodemap, self.rows, self.chunks,
-                        self.recs, self.rec_elem))))
+                        self.parts, self.nodemap, self.rows, self.recs, self.rec_elem))))
 
-    def assemble(self, name, p, out, kappa=None, load=None, groups=0):
+    def check_struct(self, basis_dev, mask, nzeros):
+        """The validation arguments for ``assemble(check=...)``: the
+        reference tables of the basis, the plan's zero mask and count and
+        the suspect queue (allocated on first use)."""
+        d = basis_dev
+        if self.queue is None:
+            cap = max(1 << 17, self.nelems)
+            self.queue = torch.empty(cap, dtype=torch.int64, device=self.parts.device)
+        return _lib.FusedCheck(d.t.data_ptr(), d.dphi.data_ptr(), d.W.data_ptr(),
+                               self.nelems, d.Q, 0, mask.data_ptr(), int(nzeros),
+                               self.queue.data_ptr(), self.queue.numel(),
+                               self.verdict.data_ptr())
+
+    def assemble(self, name, p, out, kappa=None, load=None, groups=0, check=None):
         """out[nnz] <- CSR values of ``name`` for node coordinates ``p``
-        (and load[nnodes] <- the unit-load vector when given)."""
+        (and load[nnodes] <- the unit-load vector when given).  With
+        ``check`` (``check_struct``) the call is validated against the
+        plan's pattern and the verdict {valid, mismatches, zeros, flags} is
+        left in ``self.verdict`` on the device."""
         lib = _lib.load()
         assert out.numel() == self.nnz and out.dtype == torch.float64
         assert p.shape[1] == self.nnodes and p.is_contiguous()
         assert load is None or load.numel() == self.nnodes
         assert kappa is None or kappa.numel() == self.nelems
+        st = self.state(name)
         _lib.check(lib.femhip_fused_assemble(
-            ctypes.byref(self.dev), _FORM[name], p.shape[0], _lib.ptr(p), p.shape[1],
-            _lib.ptr(kappa), _lib.ptr(out), _lib.ptr(load), int(groups),
+            ctypes.byref(self.dev), _lib.ptr(st), _FORM[name], p.shape[0], _lib.ptr(p),
+            p.shape[1], _lib.ptr(kappa), _lib.ptr(out), _lib.ptr(load),
+            None if check is None else ctypes.byref(check), int(groups),
             _lib.stream_ptr()))
-        return out
-
-
-DEFAULT_BLOCK_ROWS = int(os.environ.get('FEMHIP_SWEEP_BLOCK_ROWS', 4096))
-DEFAULT_SWEEP_CHUNK = int(os.environ.get('FEMHIP_SWEEP_CHUNK', 512))
-
-
-def sweep_plan_arrays(mesh, indptr, indices, block_rows=None, chunk=None):
-    """Build a layer-sweep plan on the host; returns (info, dict of arrays)."""
-    lib = _lib.load()
-    t = np.ascontiguousarray(mesh.t, dtype=np.int32)
-    p = np.ascontiguousarray(mesh.p, dtype=np.float64)
-    ip = np.ascontiguousarray(indptr, dtype=np.int32)
-    ix = np.ascontiguousarray(indices, dtype=np.int32)
-    handle = ctypes.c_void_p()
-    info = _lib.SweepInfo()
-    _lib.check(lib.femhip_sweep_plan_build(
-        t.shape[0], p.shape[0], t.shape[1], p.shape[1],
-        t.ctypes.data_as(ctypes.c_void_p), p.ctypes.data_as(ctypes.c_void_p),
-        ip.ctypes.data_as(ctypes.c_void_p), ix.ctypes.data_as(ctypes.c_void_p),
-        int(block_rows or DEFAULT_BLOCK_ROWS), int(chunk or DEFAULT_SWEEP_CHUNK),
-        ctypes.byref(handle), ctypes.byref(info)))
-    try:
-        arrs = dict(blocks=np.empty(4 * info.nblocks, np.int32),
-                    levels=np.empty(6 * info.nlevels, np.int32),
-                    nodemap=np.empty(info.nlocal, np.int32),
-                    rows=np.empty(2 * info.nrows, np.int32),
-                    chunks=np.empty(2 * info.nchunks, np.int32),
-                    recs=np.empty(info.rec_words * info.nrecs, np.uint32),
-                    rec_elem=np.empty(info.nrecs, np.int32))
-        _lib.check(lib.femhip_sweep_plan_export(
-            handle, *[arrs[k].ctypes.data_as(ctypes.c_void_p) for k in
-                      ('blocks', 'levels', 'nodemap', 'rows', 'chunks', 'recs', 'rec_elem')]))
-    finally:
-        lib.femhip_sweep_plan_free(handle)
-    return info, arrs
-
-
-class SweepPlan:
-    """Layer-sweep plan of the fused P1 kernel, in device memory."""
-
-    kind = 'sweep'
-
-    def __init__(self, mesh, indptr, indices, block_rows=None, chunk=None):
-        info, h = sweep_plan_arrays(mesh, indptr, indices, block_rows, chunk)
-        self.info, self.host = info, h
-        self.caps = (int(block_rows or DEFAULT_BLOCK_ROWS), int(chunk or DEFAULT_SWEEP_CHUNK))
-        self.nnz = int(indptr[-1])
-        self.nnodes = mesh.p.shape[1]
-        self.nelems = mesh.t.shape[1]
-        self.dim = mesh.p.shape[0]
-        dev = torch.device('cuda')
-        self.t = {k: torch.from_numpy(v.view(np.int32) if v.dtype == np.uint32 else v).to(dev)
-                  for k, v in h.items()}
-        self.dev = _lib.SweepDev(*[self.t[k].data_ptr() for k in
-                                   ('blocks', 'levels', 'nodemap', 'rows', 'chunks', 'recs',
-                                    'rec_elem')], info)
-
-    def stats(self):
-        i = self.info
-        return dict(variant='sweep', blocks=i.nblocks, levels=i.nlevels, records=int(i.nrecs),
-                    dup=i.dup, max_colors=i.max_colors, rec_bytes=4 * i.rec_words,
-                    max_level_slots=i.max_level_slots, max_level_nodes=i.max_level_nodes,
-                    lds_bytes=int(_lib.load().femhip_sweep_lds_bytes(ctypes.byref(i), self.dim, 0)),
-                    nlocal_over_nodes=float(i.nlocal) / self.nnodes,
-                    plan_bytes=int(sum(t.numel() * t.element_size() for t in self.t.values())))
-
-    def assemble(self, name, p, out, kappa=None, load=None, groups=0):
-        lib = _lib.load()
-        assert out.numel() == self.nnz and out.dtype == torch.float64
-        assert p.shape[1] == self.nnodes and p.is_contiguous()
-        assert load is None or load.numel() == self.nnodes
-        assert kappa is None or kappa.numel() == self.nelems
-        _lib.check(lib.femhip_sweep_assemble(
-            ctypes.byref(self.dev), _FORM[name], p.shape[0], _lib.ptr(p), p.shape[1],
-            _lib.ptr(kappa), _lib.ptr(out), _lib.ptr(load), int(groups), _lib.stream_ptr()))
         return out

@@ -109,44 +109,6 @@ def test_tet65_kappa_digest():
     _check_digest(A, dig['laplace'])
 
 
-@pytest.mark.parametrize('kind,n,a,br,ck,form,load', [
-    ('tet', 12, 0.2, 200, 512, 'laplace', True),
-    ('tet', 12, 0.2, 4096, 1024, 'mass', False),
-    ('tri', 40, 0.2, 150, 512, 'laplace', True),
-    ('tet', 10, 0.0, 300, 512, 'kappa_laplace', False),
-])
-def test_sweep_variant(kind, n, a, br, ck, form, load):
-    """The layer-sweep kernel with small and large blocks, both workgroup
-    sizes, with and without the fused load vector."""
-    from femhip.fused import SweepPlan
-    m = _mesh(kind, n, a)
-    el = femhip.ElementTriP1() if kind == 'tri' else femhip.ElementTetP1()
-    b = femhip.Basis(m, el)
-    kw = {}
-    kappa = None
-    if form == 'kappa_laplace':
-        kk = np.random.default_rng(2).lognormal(0, 1, m.nelements)
-        kw['kappa'] = kk[:, None]
-        kappa = torch.tensor(kk, device='cuda')
-    f = {'laplace': laplace, 'mass': mass, 'kappa_laplace': kappa_laplace}[form]
-    A = f.assemble_device(b, path='generic', **kw)
-    sp = SweepPlan(m, A.indptr.cpu().numpy(), A.indices.cpu().numpy(), br, ck)
-    d = b.device()
-    out = torch.empty(A.nnz, dtype=torch.float64, device='cuda')
-    ld = torch.empty(b.N, dtype=torch.float64, device='cuda') if load else None
-    sp.assemble(form, d.p, out, kappa=kappa, load=ld)
-    out2 = torch.empty_like(out)
-    sp.assemble(form, d.p, out2, kappa=kappa, load=None, groups=7)
-    torch.cuda.synchronize()
-    ref = A.data.cpu().numpy()
-    got = out.cpu().numpy()
-    assert np.abs(got - ref).max() <= TOL * np.abs(ref).max()
-    assert np.array_equal(got, out2.cpu().numpy())
-    if load:
-        bref = unit_load.assemble(b)
-        assert np.abs(ld.cpu().numpy() - bref).max() <= TOL * np.abs(bref).max()
-
-
 @pytest.mark.parametrize('kind,n,a,caps,block', [
     ('tet', 12, 0.2, (6144, 1024), 512), ('tet', 12, 0.2, (600, 120), 512),
     ('tri', 40, 0.2, (6144, 1024), 512), ('tri', 40, 0.0, (500, 200), 512),
@@ -156,7 +118,8 @@ def test_fixed_point_accumulation(kind, n, a, caps, block):
     """Fixed-point LDS accumulation: every form, with and without the load
     vector, persistent workgroups; values within the tolerance of the exact
     path, bitwise repeatable, and independent of what the plan assembled
-    before (the per-partition exponents are re-derived from the input)."""
+    before (the exponent is re-derived from the input: a pass that ran with
+    another exponent is repeated)."""
     from femhip.fused import FusedPlan
     m = _mesh(kind, n, a)
     el = femhip.ElementTriP1() if kind == 'tri' else femhip.ElementTetP1()
@@ -167,39 +130,36 @@ def test_fixed_point_accumulation(kind, n, a, caps, block):
     k2 *= 1e6                         # a very different scale: exponents change
     plans = {}
 
-    def plans_for(A):
-        # one plan pair per CSR pattern (mass keeps entries Laplace prunes)
+    def plan_for(A):
+        # one plan per CSR pattern (mass keeps entries Laplace prunes)
         key = _sha(A.indices.cpu().numpy())
         if key not in plans:
             ip, ix = A.indptr.cpu().numpy(), A.indices.cpu().numpy()
-            plans[key] = (FusedPlan(m, ip, ix, *caps, accum='fixed', block=block),
-                          FusedPlan(m, ip, ix, *caps, accum='rounds'))
-            assert plans[key][0].info.nparts > 1
+            plans[key] = FusedPlan(m, ip, ix, *caps, block=block)
+            assert plans[key].info.nparts > 1
         return plans[key]
     bref = unit_load.assemble(b)
     results = {}
     for step, (form, kk, load, groups) in enumerate([
             ('laplace', None, True, 0), ('mass', None, False, 5),
             ('kappa_laplace', k1, False, 0), ('kappa_laplace', k2, True, 3),
-            ('kappa_laplace', k1, True, 0), ('laplace', None, False, 2)]):
+            ('kappa_laplace', k1, True, 0), ('laplace', None, False, 2),
+            ('kappa_laplace', k2, False, 9)]):
         print('step', step, form, flush=True)
         kt = None if kk is None else torch.tensor(kk, device='cuda')
         kw = {} if kk is None else {'kappa': kk[:, None]}
         f = {'laplace': laplace, 'mass': mass, 'kappa_laplace': kappa_laplace}[form]
         A = f.assemble_device(b, path='generic', **kw)
-        fx, rd = plans_for(A)
+        fx = plan_for(A)
         ref = A.data.cpu().numpy()
         out = torch.empty(fx.nnz, dtype=torch.float64, device='cuda')
         ld = torch.empty(b.N, dtype=torch.float64, device='cuda') if load else None
         fx.assemble(form, d.p, out, kappa=kt, load=ld, groups=groups)
         out2 = torch.empty_like(out)
         fx.assemble(form, d.p, out2, kappa=kt, groups=groups)
-        out3 = torch.empty_like(out)
-        rd.assemble(form, d.p, out3, kappa=kt)
         torch.cuda.synchronize()
         got = out.cpu().numpy()
         assert np.abs(got - ref).max() <= TOL * np.abs(ref).max()
-        assert np.abs(out3.cpu().numpy() - ref).max() <= TOL * np.abs(ref).max()
         assert np.array_equal(got, out2.cpu().numpy())
         if load:
             assert np.abs(ld.cpu().numpy() - bref).max() <= TOL * np.abs(bref).max()
@@ -207,6 +167,60 @@ def test_fixed_point_accumulation(kind, n, a, caps, block):
         if key in results:           # same input after other inputs: same bits
             assert np.array_equal(results[key], got)
         results[key] = got
+
+
+def _oracle(kind, p, t, form, kappa=None):
+    om = (O.MeshTri if kind == 'tri' else O.MeshTet)(p, t)
+    ob = O.CellBasis(om, O.ElementTriP1() if kind == 'tri' else O.ElementTetP1())
+    if form == 'kappa_laplace':
+        return O.assemble_named(form, ob, kappa=np.repeat(kappa[:, None], ob.X.shape[1], axis=1))
+    return O.assemble_named(form, ob)
+
+
+def _same(A, R):
+    ip, ix, d = (t.cpu().numpy() for t in (A.indptr, A.indices, A.data))
+    return (np.array_equal(ip, R.indptr) and np.array_equal(ix, R.indices)
+            and np.abs(d - R.data).max() <= TOL * np.abs(R.data).max())
+
+
+@pytest.mark.parametrize('kind,n,a', [('tet', 9, 0.2), ('tri', 33, 0.2), ('tet', 8, 0.0),
+                                      ('tri', 17, 0.0)])
+def test_changed_inputs_validated_in_the_fused_pass(kind, n, a):
+    """New coordinates and new kappa on every call go through the fused
+    kernel with the pattern check (entries near zero recomputed in the
+    reference's arithmetic): every result equals the oracle's on the same
+    inputs (pattern bit-exact, values within 1e-12); inputs that keep the
+    plan's zero set stay on the fused path, inputs that change it (an
+    unjittered mesh jittered, kappa with zeros) fall back to the exact path
+    and still match."""
+    m = _mesh(kind, n, a)
+    el = femhip.ElementTriP1() if kind == 'tri' else femhip.ElementTetP1()
+    b = femhip.Basis(m, el)
+    d = b.device()
+    h = 1.0 / (n - 1)
+    rng = np.random.default_rng(3)
+    laplace.assemble_device(b)             # exact path; the fused plan
+    A = laplace.assemble_device(b)
+    assert d.last_verdict is not None and d.last_verdict[0] == 1, d.last_verdict
+    assert _same(A, _oracle(kind, m.p, m.t, 'laplace'))
+    base = _mesh(kind, n, 0.0)
+    I = base.interior_nodes()
+    for step in range(3):
+        p = base.p.copy()
+        p[:, I] += rng.uniform(-0.2 * h, 0.2 * h, size=(p.shape[0], len(I)))
+        print('coords', kind, n, a, step, flush=True)
+        d.set_coordinates(torch.tensor(p, device='cuda'))
+        A = laplace.assemble_device(b)
+        assert _same(A, _oracle(kind, p, m.t, 'laplace')), step
+        if a > 0:   # jittered -> jittered: the zero set (boundary) is the same
+            assert d.last_verdict[0] == 1, d.last_verdict
+    for step in range(4):
+        kk = rng.lognormal(0, 1, m.nelements)
+        if step == 2:
+            kk[::5] = 0.0                  # zeros prune more entries
+        print('kappa', kind, n, a, step, flush=True)
+        A = kappa_laplace.assemble_device(b, kappa=torch.tensor(kk, device='cuda')[:, None])
+        assert _same(A, _oracle(kind, d.p.cpu().numpy(), m.t, 'kappa_laplace', kk)), step
 
 
 def test_unfusable_mesh_falls_back_to_exact_path():

@@ -96,3 +96,23 @@ def test_solve_tet_residual():
     r = (Ar @ un - f)[Ii]
     assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(bI.cpu().numpy())
     assert np.array_equal(un[D], xb[D])
+
+
+def test_pcg_diagonal_system_stays_finite():
+    """Jacobi-PCG on a diagonal system converges in one step to r = 0
+    exactly; the following iterations (up to the next residual check) must
+    leave x unchanged instead of forming 0/0.  Also the 1-DOF system."""
+    from femhip.device import DeviceCSR
+    for diag in ([2.0, 4.0, 8.0], [5.0]):
+        n = len(diag)
+        dev = 'cuda'
+        A = DeviceCSR(torch.arange(n + 1, dtype=torch.int32, device=dev),
+                      torch.arange(n, dtype=torch.int32, device=dev),
+                      torch.tensor(diag, dtype=torch.float64, device=dev), (n, n))
+        b = torch.tensor([1.0, -3.0, 0.5][:n], dtype=torch.float64, device=dev)
+        info = {}
+        print('pcg diag', n, flush=True)
+        u = solve_device(A, b, rtol=1e-12, check_every=16, info=info).cpu().numpy()
+        assert np.all(np.isfinite(u)), u
+        assert np.allclose(u, b.cpu().numpy() / np.array(diag), rtol=0, atol=1e-15)
+        assert info['residual'] == 0.0
