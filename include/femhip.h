@@ -297,20 +297,29 @@ typedef struct femhip_fused_dev {
 } femhip_fused_dev_t;
 
 /* Accumulation and validation state of one form on one plan, in device
- * memory (zeroed before first use, then owned by the kernels; calls that
- * share a state must not overlap). */
+ * memory (256 bytes, 128-byte aligned as every hipMalloc / torch allocation
+ * is; zeroed before first use, then owned by the kernels; calls that share
+ * a state must not overlap).  Two cache lines: the first holds what every
+ * workgroup of a pass reads, the second what they update atomically -- a
+ * line that takes device-wide atomics is slow to read with plain loads. */
 enum { FEMHIP_FUSED_BAD = 1, FEMHIP_FUSED_OVERFLOW = 2 };
 typedef struct femhip_fused_state {
-  int32_t s, sv;         /* fixed-point exponents of the last pass (values, load) */
+  /* read by every workgroup of a first pass */
+  int32_t s, sv;         /* fixed-point exponents for the next first pass     */
   int32_t valid;         /* s, sv have been measured                          */
-  int32_t redo;          /* the last first pass ran with other exponents      */
-  uint64_t maxk, maxv;   /* maxima of the running pass (bit patterns of doubles >= 0) */
-  int32_t flags;         /* FEMHIP_FUSED_BAD | FEMHIP_FUSED_OVERFLOW           */
+  int32_t s_first;       /* exponents the last first pass used (INT32_MIN:    */
+  int32_t sv_first;      /*   none)                                           */
   int32_t reserved0;
-  uint32_t reserved1[8];
+  uint64_t epoch;        /* tag of the next first pass's maxima               */
+  uint32_t reserved1[24];
+  /* updated atomically */
+  uint64_t maxk, maxv;   /* (epoch << 11) | exponent field of the pass's largest
+                            |contribution| / load entry (atomicMax)           */
+  int32_t flags;         /* FEMHIP_FUSED_BAD | FEMHIP_FUSED_OVERFLOW           */
+  int32_t reserved2;
   int64_t nsusp;         /* suspects queued by the last first pass            */
   int64_t mismatch, zeros; /* recheck counters                                */
-  int32_t reserved[4];
+  uint32_t reserved3[20];
 } femhip_fused_state_t;
 
 /* Validation of a pass against the pattern the plan was built for (the
@@ -343,8 +352,8 @@ size_t femhip_fused_lds_bytes(const femhip_fused_plan_info_t* info, int32_t dim,
  * p is [dim][nnodes] on the device.  Two launches (the second repeats the
  * pass only when the first measured another fixed-point exponent), then,
  * with check != NULL, the recheck of the suspects (verdict on the device).
- * groups = persistent workgroups taking partitions from work counters (0:
- * one workgroup per partition).  A plan may concatenate several meshes
+ * groups = a persistent grid of that many workgroups walking the
+ * partitions with a fixed stride (0: one workgroup per partition).  A plan may concatenate several meshes
  * (femhip.batch): then p, data and load are the concatenated arrays. */
 int femhip_fused_assemble(const femhip_fused_dev_t* plan, femhip_fused_state_t* state,
                           int32_t form, int32_t dim, const double* p, int64_t nnodes,

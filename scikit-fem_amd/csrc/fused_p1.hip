@@ -23,11 +23,13 @@
 // 2^46 <= M 2^s < 2^50: no conversion leaves the 2^51 range of the
 // magic-number rounding, a slot of n contributions is within
 // n 2^-(s+1) <= n 7.2e-15 M of its exact sum, and sums stay below 2^62.
-// The pass uses the exponent the previous call measured; the last
-// workgroup to finish reduces this pass's M, and when its bucket differs
-// the companion launch (mode 1) repeats the pass with it -- otherwise that
-// launch returns at once.  The values are therefore a function of the
-// input alone.
+// The pass uses the exponent the previous call measured and records the
+// exponent field of its own M (atomicMax, tagged with a pass epoch so that
+// no reset is needed); the companion launch (mode 1) derives this input's
+// exponent from it and repeats the pass when the two differ -- otherwise it
+// returns at once.  The values are therefore a function of the input
+// alone.  No workgroup waits on another (no fences, no last-block
+// reductions): the launch boundaries order the state updates.
 //
 // Validation of changed inputs (first-pass suspects): an off-diagonal
 // entry that the reference computes as exactly 0.0 (and eliminate_zeros
@@ -47,7 +49,10 @@ namespace femhip {
 namespace {
 
 constexpr double kFxMagic = 6755399441055744.0;  // 1.5 * 2^52
-static_assert(sizeof(femhip_fused_state_t) == 112, "femhip_fused_state_t layout");
+static_assert(sizeof(femhip_fused_state_t) == 256 &&
+                  offsetof(femhip_fused_state_t, maxk) == 128,
+              "femhip_fused_state_t layout: read line, then atomic line");
+constexpr unsigned kRepeatGrid = 768;   // 3 workgroups on each of 256 CUs
 
 struct FxArgs {
   const int32_t* parts;
@@ -128,25 +133,27 @@ __device__ __forceinline__ unsigned long long to_fixed(double v, double scale) {
   return (unsigned long long)(__double_as_longlong(y) - __double_as_longlong(kFxMagic));
 }
 
-#ifndef FEMHIP_FUSED_NT_STORE
-#define FEMHIP_FUSED_NT_STORE 1
-#endif
-// CSR values are written once and not read by this kernel again
-__device__ __forceinline__ void store_value(double v, double* p) {
-  if constexpr (FEMHIP_FUSED_NT_STORE) __builtin_nontemporal_store(v, p);
-  else *p = v;
-}
-
 __device__ __forceinline__ double from_fixed(long long v, double inv_scale) {
   return (double)v * inv_scale;
 }
 
-// exponent bucket of the largest |contribution| m: 2^46 <= m 2^s < 2^50
-__device__ __forceinline__ int fx_bucket(double m) {
-  if (!(m > 0.0) || !(m < INFINITY)) return 0;
-  int s = 49 - ilogb(m);
+// The pass maxima travel as (epoch << 11) | the 11-bit exponent field of
+// the double: atomicMax keeps the largest exponent of the newest epoch, so
+// a new pass overrides the previous pass's maximum without a reset.
+__device__ __forceinline__ unsigned long long exponent_field(double m) {
+  return ((unsigned long long)__double_as_longlong(fabs(m)) >> 52) & 0x7ffull;
+}
+
+// exponent bucket of the largest |contribution| from its exponent field e:
+// 2^(e-1023) <= m < 2^(e-1022), so s = 49 - (e - 1023) rounded down to a
+// multiple of 4 gives 2^46 <= m 2^s < 2^50 (e = 0: zero or subnormal; e =
+// 2047: not finite, flagged by the pass itself)
+__device__ __forceinline__ int fx_bucket_tagged(unsigned long long tagged) {
+  const int e = (int)(tagged & 0x7ffull);
+  if (e == 0 || e == 0x7ff) return 0;
+  int s = 49 - (e - 1023);
   s = s >= 0 ? (s & ~3) : -((-s + 3) & ~3);
-  return s > 1000 ? 1000 : (s < -1000 ? -1000 : s);
+  return s;
 }
 
 __device__ __forceinline__ void lds_add(unsigned long long* p, unsigned long long v) {
@@ -180,23 +187,14 @@ struct Elem {
   double tau;
 };
 
-#ifndef FEMHIP_FUSED_XSOA
-#define FEMHIP_FUSED_XSOA 0
-#endif
-// staged coordinates: node-major (x y z per node) or one array per axis
-constexpr bool kXSoa = FEMHIP_FUSED_XSOA != 0;
-
 template <int D, int FORM>
-__device__ __forceinline__ void element_matrix(const double* X, int xs, const int (&lv)[D + 1],
+__device__ __forceinline__ void element_matrix(const double* X, const int (&lv)[D + 1],
                                                double coef, bool want_tau, Elem<D, FORM>& el) {
   constexpr int NV = D + 1;
   double x[NV][D];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    if constexpr (kXSoa) {
-#pragma unroll
-      for (int d = 0; d < D; ++d) x[v][d] = X[d * xs + lv[v]];
-    } else if constexpr (D == 3) {
+    if constexpr (D == 3) {
       x[v][0] = X[3 * lv[v]];
       x[v][1] = X[3 * lv[v] + 1];
       x[v][2] = X[3 * lv[v] + 2];
@@ -365,17 +363,40 @@ p1_fused_kernel(FxArgs a) {
   const int tid = threadIdx.x;
   femhip_fused_state_t* st = a.st;
   const bool first = a.mode == 0;
-  if (!first && !st->redo) return;  // uniform: the first pass's exponent was right
-  const bool known = first ? st->valid != 0 : true;
+  int s, sv;
+  bool known;
+  if (first) {
+    // uniform values: read into scalar registers (workgroup 0 stores to
+    // the state below, so the compiler would otherwise keep them in vector
+    // registers and treat every branch on them as divergent)
+    known = __builtin_amdgcn_readfirstlane(st->valid) != 0;
+    s = known ? __builtin_amdgcn_readfirstlane(st->s) : 0;
+    sv = known ? __builtin_amdgcn_readfirstlane(st->sv) : 0;
+    if (blockIdx.x == 0 && tid == 0) {
+      st->s_first = known ? s : INT32_MIN;
+      st->sv_first = known ? sv : INT32_MIN;
+    }
+  } else {
+    // the exponents of this input (the first pass's maxima, final at the
+    // launch boundary); repeat only when the first pass used others.  The
+    // fields read here are not written by this launch.
+    const int s2 = fx_bucket_tagged(st->maxk), sv2 = LOAD ? fx_bucket_tagged(st->maxv) : 0;
+    const bool redo = st->s_first == INT32_MIN || s2 != st->s_first ||
+                      (LOAD && sv2 != st->sv_first);
+    if (blockIdx.x == 0 && tid == 0) {   // for the next first pass
+      st->s = s2;
+      if constexpr (LOAD) st->sv = sv2;
+      st->valid = 1;
+      st->epoch = (st->maxk >> 11) + 1;
+    }
+    if (!redo) return;  // uniform
+    known = true;
+    s = s2;
+    sv = sv2;
+  }
   const bool scatter = known;
-  // the repeat launch runs with the exponents fused_exponent_kernel set
-  const int s = known ? st->s : 0, sv = known ? st->sv : 0;
   const double sck = ldexp(1.0, s), scv = ldexp(1.0, sv);
-#ifdef FEMHIP_FUSED_NOSUSP
-  constexpr bool suspects = false;
-#else
   const bool suspects = first && a.susp != nullptr;
-#endif
   double mk = 0.0, mv = 0.0;
   bool bad = false;
 
@@ -400,8 +421,8 @@ p1_fused_kernel(FxArgs a) {
       }
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        L.X[kXSoa ? d * a.max_nodes + l0 : D * l0 + d] = x0[d];
-        L.X[kXSoa ? d * a.max_nodes + l1 : D * l1 + d] = x1[d];
+        L.X[D * l0 + d] = x0[d];
+        L.X[D * l1 + d] = x1[d];
       }
     }
     for (int r = tid; r < R; r += NT) {
@@ -423,21 +444,34 @@ p1_fused_kernel(FxArgs a) {
     // every path and the wait for a chunk's records leaves the next
     // chunk's loads outstanding.
     const int nchunks = (nrec + NT - 1) / NT;
+    // kappa-Laplace: the element ids run one chunk ahead of the records, so
+    // a chunk's coefficient gathers use ids that have already arrived
     struct Chunk {
       uint32_t w[RW];
       double coef;
+      int32_t elem_next;   // element id of this lane's record in the next chunk
     };
-    auto fetch = [&](int c, Chunk& ch) {
-      int r = c * NT + tid;
-      r = r < nrec ? r : nrec - 1;
-      const uint32_t* rp = a.recs + (int64_t)(rec_start + r) * RW;
+    auto clamp_rec = [&](int c) {
+      const int r = c * NT + tid;
+      return rec_start + (r < nrec ? r : nrec - 1);
+    };
+    auto fetch = [&](int c, Chunk& ch, const Chunk& prev) {
+      const uint32_t* rp = a.recs + (int64_t)clamp_rec(c) * RW;
 #pragma unroll
       for (int q = 0; q < RW; ++q) ch.w[q] = __builtin_nontemporal_load(rp + q);
-      if constexpr (FORM == FEMHIP_KAPPA_LAPLACE) ch.coef = a.kappa[a.rec_elem[rec_start + r]];
-      else ch.coef = 1.0;
+      if constexpr (FORM == FEMHIP_KAPPA_LAPLACE) {
+        ch.coef = a.kappa[prev.elem_next];
+        ch.elem_next = a.rec_elem[clamp_rec(c + 1)];
+      } else {
+        ch.coef = 1.0;
+      }
     };
     Chunk cur, nxt;
-    if (nrec > 0) fetch(0, cur);
+    if (nrec > 0) {
+      Chunk first_ids;
+      if constexpr (FORM == FEMHIP_KAPPA_LAPLACE) first_ids.elem_next = a.rec_elem[clamp_rec(0)];
+      fetch(0, cur, first_ids);
+    }
     __syncthreads();  // staged data before the first element
 
     // one element per lane: local matrix, maxima, suspects (returned as a
@@ -457,7 +491,7 @@ p1_fused_kernel(FxArgs a) {
 #pragma unroll
       for (int i = 0; i < NV; ++i) rinfo[i] = L.rinfo[lv[i] < R ? lv[i] : 0];
       Elem<D, FORM> el;
-      element_matrix<D, FORM>(L.X, a.max_nodes, lv, ch.coef, suspects, el);
+      element_matrix<D, FORM>(L.X, lv, ch.coef, suspects, el);
       uint32_t mask = 0;
       if constexpr (FORM == FEMHIP_MASS) {
         mk = max_abs(mk, el.K[0][0]);
@@ -526,20 +560,19 @@ p1_fused_kernel(FxArgs a) {
         if (mask) push_suspects<NV>(a, mask, a.rec_elem[rec_start + c * NT + tid]);
     };
     for (int c = 0; c < nchunks; c += 2) {
-      fetch(c + 1 < nchunks ? c + 1 : c, nxt);
+      fetch(c + 1, nxt, cur);
       run(cur, c);
       if (c + 1 >= nchunks) break;
-      fetch(c + 2 < nchunks ? c + 2 : c + 1, cur);
+      fetch(c + 2, cur, nxt);
       run(nxt, c + 1);
     }
     if (!scatter) continue;
     __syncthreads();  // every contribution has landed
-#ifdef FEMHIP_FUSED_NOWB
-    continue;
-#endif
 
-    // write back: 16 lanes per row; Laplace forms: diagonal = -(sum of the
-    // row's other slots, the spare slot included)
+    // write back: 16 lanes per row, neighbouring lanes on neighbouring
+    // slots (rows hold at most 32 slots).  Laplace forms: the diagonal is
+    // -(sum of the row's other slots, the spare slot included), reduced
+    // across the row's 16 lanes.
     const double inv = ldexp(1.0, -s);
     const int lane = tid & 15, grp = tid >> 4;
     for (int r = grp; r < R; r += NT / 16) {
@@ -558,10 +591,11 @@ p1_fused_kernel(FxArgs a) {
         dsum = -t;
       }
       if (lane < len)
-        store_value(from_fixed(ROWSUM && lane == dg ? dsum : v0, inv), a.out + gstart + lane);
+        __builtin_nontemporal_store(from_fixed(ROWSUM && lane == dg ? dsum : v0, inv),
+                                    a.out + gstart + lane);
       if (lane + 16 < len)
-        store_value(from_fixed(ROWSUM && lane + 16 == dg ? dsum : v1, inv),
-                    a.out + gstart + lane + 16);
+        __builtin_nontemporal_store(from_fixed(ROWSUM && lane + 16 == dg ? dsum : v1, inv),
+                                    a.out + gstart + lane + 16);
     }
     if constexpr (LOAD) {
       const double invv = ldexp(1.0, -sv);
@@ -570,8 +604,8 @@ p1_fused_kernel(FxArgs a) {
     }
   }  // partitions
 
-  // workgroup maxima -> the launch's maxima (read by the repeat launch and
-  // the finish kernel; no fence: the kernel boundary orders them)
+  // workgroup maxima -> the launch's maxima (read by the repeat launch; no
+  // fence: the kernel boundary orders them)
   if (!first) return;
   mk = wave_max(mk);
   if constexpr (LOAD) mv = wave_max(mv);
@@ -589,26 +623,11 @@ p1_fused_kernel(FxArgs a) {
       MK = fmax(MK, L.red[q]);
       MV = fmax(MV, L.red[NW + q]);
     }
-    atomicMax(reinterpret_cast<unsigned long long*>(&st->maxk),
-              (unsigned long long)__double_as_longlong(MK));
+    const unsigned long long ep = st->epoch << 11;
+    atomicMax(reinterpret_cast<unsigned long long*>(&st->maxk), ep | exponent_field(MK));
     if constexpr (LOAD)
-      atomicMax(reinterpret_cast<unsigned long long*>(&st->maxv),
-                (unsigned long long)__double_as_longlong(MV));
+      atomicMax(reinterpret_cast<unsigned long long*>(&st->maxv), ep | exponent_field(MV));
   }
-}
-
-// Between the first pass and the repeat launch: the exponents the first
-// pass measured, and whether the pass must be repeated with them.
-__global__ void fused_exponent_kernel(femhip_fused_state_t* st, int with_load) {
-  const double M = __longlong_as_double((long long)st->maxk);
-  const double MV = __longlong_as_double((long long)st->maxv);
-  const int s2 = fx_bucket(M), sv2 = with_load ? fx_bucket(MV) : st->sv;
-  st->redo = (!st->valid || s2 != st->s || sv2 != st->sv) ? 1 : 0;
-  st->s = s2;
-  st->sv = sv2;
-  st->valid = 1;
-  st->maxk = 0;
-  st->maxv = 0;
 }
 
 template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>
@@ -677,6 +696,8 @@ extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan,
                                      femhip_stream_t s) {
   FEMHIP_REQUIRE(plan != nullptr && state != nullptr && p != nullptr && data != nullptr,
                  FEMHIP_EINVAL, "fused: NULL argument");
+  FEMHIP_REQUIRE(reinterpret_cast<uintptr_t>(state) % 128 == 0, FEMHIP_EINVAL,
+                 "fused: state must be 128-byte aligned");
   const femhip_fused_plan_info_t& I = plan->info;
   FEMHIP_REQUIRE(I.block == 512 || I.block == 1024, FEMHIP_EINVAL,
                  "fused: plan block must be 512 or 1024");
@@ -699,18 +720,16 @@ extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan,
            I.max_rows, 0};
   const size_t lds = femhip_fused_lds_bytes(&I, dim, load != nullptr);
   FEMHIP_REQUIRE(lds <= 160 * 1024, FEMHIP_EINVAL, "fused: plan exceeds LDS");
-  // first pass; exponents measured (repeat flag); repeat pass when the
-  // first one used other exponents (its workgroups return at once
-  // otherwise); work counters reset
+  // first pass, then the repeat launch (a pass only when the first one
+  // used other exponents)
   for (int mode = 0; mode < 2; ++mode) {
     a.mode = mode;
-    const int rc = load ? dispatch_fused<true>(a, I, form, dim, lds, grid, st)
-                        : dispatch_fused<false>(a, I, form, dim, lds, grid, st);
+    // the repeat launch: a persistent grid of at most three workgroups per
+    // CU, which returns at once unless the exponents changed
+    const unsigned g = mode == 0 ? grid : (grid < kRepeatGrid ? grid : kRepeatGrid);
+    const int rc = load ? dispatch_fused<true>(a, I, form, dim, lds, g, st)
+                        : dispatch_fused<false>(a, I, form, dim, lds, g, st);
     if (rc != FEMHIP_OK) return rc;
-    if (mode == 0) {
-      fused_exponent_kernel<<<1, 1, 0, st>>>(state, load != nullptr ? 1 : 0);
-      FEMHIP_CHECK_LAUNCH();
-    }
   }
 
   if (check) return fused_check_run(check, state, form, dim, p, nnodes, kappa, st);

@@ -68,11 +68,13 @@ class FusedDev(ctypes.Structure):
 
 
 class FusedState(ctypes.Structure):
-    _fields_ = [('s', _i32), ('sv', _i32), ('valid', _i32), ('redo', _i32),
+    _fields_ = [('s', _i32), ('sv', _i32), ('valid', _i32), ('s_first', _i32),
+                ('sv_first', _i32), ('reserved0', _i32), ('epoch', ctypes.c_uint64),
+                ('reserved1', ctypes.c_uint32 * 24),
                 ('maxk', ctypes.c_uint64), ('maxv', ctypes.c_uint64),
-                ('flags', _i32), ('reserved0', _i32), ('reserved1', ctypes.c_uint32 * 8),
+                ('flags', _i32), ('reserved2', _i32),
                 ('nsusp', _i64), ('mismatch', _i64), ('zeros', _i64),
-                ('reserved', _i32 * 4)]
+                ('reserved3', ctypes.c_uint32 * 20)]
 
 
 class FusedCheck(ctypes.Structure):

@@ -94,18 +94,26 @@ def test_c_abi_null_arguments():
 
 def test_fused_structs_match_the_header():
     """The ctypes mirrors of the fused path's structs have the sizes the C
-    side asserts (femhip_fused_state_t: 112 bytes)."""
-    assert ctypes.sizeof(_lib.FusedState) == 112
+    side asserts (femhip_fused_state_t: 256 bytes, the atomically updated
+    fields on the second 128-byte line)."""
+    assert ctypes.sizeof(_lib.FusedState) == 256
+    assert _lib.FusedState.maxk.offset == 128
     assert ctypes.sizeof(_lib.FusedInfo) == 12 * 4 + 3 * 8
     assert ctypes.sizeof(_lib.FusedCheck) == 3 * 8 + 8 + 8 + 8 + 8 + 8 + 8 + 8
 
 
 def test_fused_assemble_argument_checks():
     lib = _lib.load()
-    st = (ctypes.c_uint8 * 112)()
+    buf = (ctypes.c_uint8 * 512)()
+    st = ctypes.c_void_p((ctypes.addressof(buf) + 127) // 128 * 128)   # 128-byte aligned
     assert lib.femhip_fused_assemble(None, st, 0, 3, _fake(1), 10, None, _fake(2), None,
                                      None, 0, None) == _lib.EINVAL
     info = _lib.FusedInfo()
+    info.block = 512
+    dev = _lib.FusedDev(None, None, None, None, None, info)
+    rc = lib.femhip_fused_assemble(ctypes.byref(dev), ctypes.c_void_p(st.value + 8), 0, 3,
+                                   _fake(1), 10, None, _fake(2), None, None, 0, None)
+    assert rc == _lib.EINVAL and b"aligned" in lib.femhip_last_error()
     info.block = 256
     dev = _lib.FusedDev(None, None, None, None, None, info)
     rc = lib.femhip_fused_assemble(ctypes.byref(dev), st, 0, 3, _fake(1), 10, None, _fake(2),
