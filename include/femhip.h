@@ -361,6 +361,25 @@ int femhip_fused_assemble(const femhip_fused_dev_t* plan, femhip_fused_state_t* 
                           const femhip_fused_check_t* check, int32_t groups,
                           femhip_stream_t s);
 
+/* femhip_fused_assemble without any allocation, caller's or library's.  It
+ * has no counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The accumulation states (one
+ * per form) and the suspect queue live in a static device array of the
+ * library (3 x 256 B + 65,536 suspect entries = 525,056 bytes per device,
+ * held until the process ends), so the caller passes only the plan, the
+ * inputs and the outputs (check: the tables, the zero mask and the
+ * verdict; its queue and capacity fields are ignored).  Returns
+ * FEMHIP_EINVAL for a check whose pattern has more exact zeros than the
+ * static queue holds (femhip_fused_static_queue()).  Every call on a device
+ * shares that device's array and keeps the last call's contents; calls
+ * that overlap on one device race. */
+int64_t femhip_fused_static_queue(void);
+int femhip_fused_assemble_static(const femhip_fused_dev_t* plan, int32_t form, int32_t dim,
+                                 const double* p, int64_t nnodes, const double* kappa,
+                                 double* data, double* load, const femhip_fused_check_t* check,
+                                 int32_t groups, femhip_stream_t s);
+
 /* out[S][nseg] <- kappa-weighted segmented sums  sum_k kappa[s][c_k % E] *
  * K1[c_k]  over the plan (perm, seg): many per-element coefficient samples
  * (kappa[S][E]) assembled on one shared pattern from the kappa = 1 local

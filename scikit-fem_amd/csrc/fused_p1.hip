@@ -736,3 +736,84 @@ extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan,
   return FEMHIP_OK;
 }
 
+
+// ---------------------------------------------------------------------
+// Allocation-free variant: the accumulation states (one per form) and the
+// suspect queue live in `scratch`, a static device array of this library,
+// so a caller that cannot allocate (code under HIP graph capture, serving
+// loops) passes only the plan, the inputs and the outputs.  Every call on a
+// device shares that device's array and keeps the last call's contents:
+// a state carries the last exponent (a different plan's or input's is a
+// valid first guess -- the repeat pass corrects it), and the verdict
+// kernels reset the counters a checked call uses.  Calls that overlap on
+// one device race.
+// ---------------------------------------------------------------------
+#include <mutex>
+
+namespace femhip {
+namespace {
+
+constexpr int64_t kStaticQueue = 1 << 16;   // suspect entries
+constexpr size_t kStateSlot = 256;          // bytes per form state
+static_assert(sizeof(femhip_fused_state_t) == kStateSlot, "state slot");
+constexpr size_t kScratchBytes = 3 * kStateSlot + 8 * (size_t)kStaticQueue;  // 525,056
+
+// zero-initialised with the code object: every state starts unmeasured
+// and with its counters at zero
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on that device (each device holds its own copy of the array;
+// the array's name is not a device address in host code).
+int scratch_ptr(unsigned char** out) {
+  int dev = 0;
+  FEMHIP_TRY_HIP(hipGetDevice(&dev));
+  FEMHIP_REQUIRE(dev >= 0 && dev < kMaxDevices, FEMHIP_EUNSUPPORTED,
+                 "fused static: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    FEMHIP_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FEMHIP_REQUIRE(p != nullptr, FEMHIP_EHIP, "fused static: no device address for scratch");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return FEMHIP_OK;
+}
+
+}  // namespace
+}  // namespace femhip
+
+extern "C" int64_t femhip_fused_static_queue(void) { return femhip::kStaticQueue; }
+
+extern "C" int femhip_fused_assemble_static(const femhip_fused_dev_t* plan, int32_t form,
+                                            int32_t dim, const double* p, int64_t nnodes,
+                                            const double* kappa, double* data, double* load,
+                                            const femhip_fused_check_t* check, int32_t groups,
+                                            femhip_stream_t s) {
+  using namespace femhip;
+  FEMHIP_REQUIRE(form == FEMHIP_LAPLACE || form == FEMHIP_MASS || form == FEMHIP_KAPPA_LAPLACE,
+                 FEMHIP_EUNSUPPORTED, "fused static: laplace, mass or kappa_laplace only");
+  FEMHIP_REQUIRE(check == nullptr || check->verdict != nullptr, FEMHIP_EINVAL,
+                 "fused static: the check needs a verdict output");
+  FEMHIP_REQUIRE(check == nullptr || (check->nzeros >= 0 && check->nzeros <= kStaticQueue),
+                 FEMHIP_EINVAL,
+                 "fused static: the pattern's exact zeros exceed the static suspect queue "
+                 "(65536 entries); use femhip_fused_assemble with a larger queue");
+  unsigned char* base = nullptr;
+  if (const int rc = scratch_ptr(&base); rc != FEMHIP_OK) return rc;
+  femhip_fused_state_t* state =
+      reinterpret_cast<femhip_fused_state_t*>(base + kStateSlot * (size_t)form);
+  femhip_fused_check_t chk{};
+  if (check) {
+    chk = *check;
+    chk.queue = reinterpret_cast<uint64_t*>(base + 3 * kStateSlot);
+    chk.capacity = kStaticQueue;
+  }
+  return femhip_fused_assemble(plan, state, form, dim, p, nnodes, kappa, data, load,
+                               check ? &chk : nullptr, groups, s);
+}

@@ -123,6 +123,10 @@ SIGNATURES = {
     'femhip_fused_assemble': (ctypes.c_int, [ctypes.POINTER(FusedDev), _p, _i32, _i32, _p,
                                              _i64, _p, _p, _p, ctypes.POINTER(FusedCheck),
                                              _i32, _p]),
+    'femhip_fused_static_queue': (_i64, []),
+    'femhip_fused_assemble_static': (ctypes.c_int, [ctypes.POINTER(FusedDev), _i32, _i32, _p,
+                                                    _i64, _p, _p, _p,
+                                                    ctypes.POINTER(FusedCheck), _i32, _p]),
     'femhip_batched_kappa_values': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _p, _i64, _p,
                                                    _p]),
     'femhip_hex2_elasticity_mfma': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space),
