@@ -120,7 +120,8 @@ def cpu_configs():
     """One-time timings of the oracle on every benchmark configuration (the
     reference's thread knob at the host's cores), progress on stderr:
     P1 triangles 129^2 (laplace + unit_load), P1 tetrahedra 101^3 jittered
-    (laplace, mass, unit_load; median of 3), P2 tetrahedra 41^3 jittered,
+    (laplace, mass with nthreads = cores and = 0, unit_load; each the median
+    of 3), P2 tetrahedra 41^3 jittered,
     vector Q2 hexahedral elasticity 20^3 jittered (the 40^3 case is 8x the
     elements), kappa-Laplace on P1 tetrahedra 65^3, one kappa sample on P2
     triangles 257^2 and the 64 P1 triangle meshes (per-mesh p50/p99)."""
@@ -151,9 +152,11 @@ def cpu_configs():
     m = O.MeshTet(mm.p, mm.t)
     tb, b = timed(lambda: O.CellBasis(m, O.ElementTetP1()))
     r = {"elements": b.nelems, "basis_s": tb}
-    for name, f in (('laplace', O.form_laplace), ('mass', O.form_mass)):
-        r[name + '_s'] = _median_s(lambda: O.bilinear_assemble(f, b, nthreads=cores), 3)
-        log('cpu config tet1_n101j', name, 'done')
+    for nth, tag in ((cores, ''), (0, '_nthreads0')):
+        for name, f in (('laplace', O.form_laplace), ('mass', O.form_mass)):
+            r[name + tag + '_s'] = _median_s(
+                lambda: O.bilinear_assemble(f, b, nthreads=nth), 3)
+            log('cpu config tet1_n101j', name, 'nthreads', nth, 'done')
     r['unit_load_s'] = _median_s(lambda: O.linear_assemble(O.form_unit_load, b), 3)
     res["tet1_n101j"] = r
     del b
@@ -322,6 +325,29 @@ def bench_adjoint(n=65):
     ms_f = time_stream(fwd, 10, stream)
     ms_b = time_stream(bwd, 10, stream)
     ms_fb = time_stream(fwdbwd, 10, stream)
+    # the launch sequences alone (C ABI, no autograd bookkeeping): the
+    # forward's fused pass and the fused adjoint
+    import ctypes
+    from femhip import _lib
+    from femhip.autograd import _ctx
+    c = _ctx(b)
+    fo = torch.empty_like(data)
+    fl = torch.empty_like(load)
+    dk = torch.empty_like(kap)
+    dX = torch.empty_like(p)
+
+    def fwd_launch():
+        c.fp.assemble('kappa_laplace', p, fo, kap, load=fl)
+
+    def adj_launch():
+        _lib.check(c.d.lib.femhip_fused_adjoint(
+            ctypes.byref(c.fp.dev), _lib.ptr(c.fp.state('adjoint')), 3, _lib.ptr(p), p.shape[1],
+            _lib.ptr(c.plan.indptr), _lib.ptr(c.diag_slot), _lib.ptr(kap), _lib.ptr(gd),
+            _lib.ptr(gl), _lib.ptr(dk), _lib.ptr(dX), _lib.stream_ptr()))
+    fwd_launch()
+    adj_launch()
+    ms_fl = time_stream(fwd_launch, 20, stream)
+    ms_al = time_stream(adj_launch, 20, stream)
     # algorithmic bytes: connectivity, coordinates and kappa read, values and
     # load written (forward); + dL/dA, dL/db read, dkappa and dX written
     # (adjoint; dL/dA on the CSR pattern)
@@ -333,7 +359,10 @@ def bench_adjoint(n=65):
             "forward_loss_backward_ms": ms_fb,
             "forward_frac_of_8TBps": fwd_bytes / (ms_f * 1e-3) / HBM_PEAK,
             "adjoint_alg_bytes": bwd_bytes,
-            "adjoint_frac_of_8TBps": bwd_bytes / (ms_b * 1e-3) / HBM_PEAK}
+            "adjoint_frac_of_8TBps": bwd_bytes / (ms_b * 1e-3) / HBM_PEAK,
+            "forward_launch_ms": ms_fl, "adjoint_launch_ms": ms_al,
+            "adjoint_launch_over_forward_launch": ms_al / ms_fl,
+            "adjoint_launch_frac_of_8TBps": bwd_bytes / (ms_al * 1e-3) / HBM_PEAK}
 
 
 def bench_changed_inputs(basis, calls=10):

@@ -272,7 +272,9 @@ int femhip_segment_sum(const double* contrib, const int32_t* perm,
 typedef struct femhip_fused_plan_info {
   int32_t nparts, max_recs, nrows, max_slots;
   int32_t max_nodes, max_rows, rec_words, idbits;
-  int32_t offbits, block, reserved0, reserved1;
+  int32_t offbits, block;
+  int32_t pruned;       /* 1: the pattern lacks a pair of some element      */
+  int32_t reserved1;
   int64_t nrecs, nlocal;
   double dup;           /* element visits / elements (halo recomputation)  */
 } femhip_fused_plan_info_t;
@@ -407,29 +409,21 @@ int femhip_enforce_rows(const int32_t* indptr, const int32_t* indices, double* d
  * linear_form.py:18-49) with respect to per-element kappa and the node
  * coordinates Mesh.p.
  *
- * femhip_invert_plan: slot_of[c] = CSR slot of contribution c (entries of
- * contributions not in any slot are left untouched: initialise to -1).
- * femhip_p1_adjoint: per element, from grad_data[nnz] (via slot_of) and
- * grad_load[nnodes] (either may be NULL): dkappa[e] (may be NULL) and the
- * per-vertex coordinate gradients gX[dim][(d+1)*nelems] (contribution
- * layout a*nelems + e, summed per node with femhip_segment_sum and the rows
- * plan of the mesh connectivity).  kappa NULL means kappa = 1.
+ * femhip_fused_adjoint: on a fused plan of the connectivity pattern (no
+ * pruned pair: info.pruned == 0; indptr its row starts, diag_slot[nnodes]
+ * the CSR slot of each row's diagonal), from grad_data[nnz] and
+ * grad_load[nnodes] (either may be NULL): dkappa[nelems] (may be NULL) and
+ * dX[dim][nnodes], the node sums formed in the partitions' LDS (fixed
+ * point, one exponent per launch, deterministic).  kappa NULL means
+ * kappa = 1.  state: a femhip_fused_state_t of its own (not the forward's).
+ * Two launches, as femhip_fused_assemble.
  * --------------------------------------------------------------------- */
-int femhip_invert_plan(const int32_t* perm, const int32_t* seg, int64_t nseg,
-                       int32_t* slot_of, femhip_stream_t s);
-int femhip_p1_adjoint(int32_t dim, int64_t nelems, int64_t nnodes, const double* p,
-                      const int32_t* t, const double* kappa, const int32_t* slot_of,
-                      const double* grad_data, const double* grad_load, double* dkappa,
-                      double* gX, femhip_stream_t s);
-
-/* Node sums of the adjoint's per-vertex contributions, up to three
- * components in one pass: out[c*out_stride + s] = sum_{k in [seg[s],
- * seg[s+1])} contrib[c*in_stride + perm[k]] (perm order, deterministic).
- * Multi-column counterpart of femhip_segment_sum; no reference counterpart
- * (it accumulates dL/dx per mesh node). */
-int femhip_segment_sum_multi(const double* contrib, int32_t ncomp, int64_t in_stride,
-                             const int32_t* perm, const int32_t* seg, int64_t nseg,
-                             double* out, int64_t out_stride, femhip_stream_t s);
+size_t femhip_fused_adjoint_lds_bytes(const femhip_fused_plan_info_t* info, int32_t dim);
+int femhip_fused_adjoint(const femhip_fused_dev_t* plan, femhip_fused_state_t* state,
+                         int32_t dim, const double* p, int64_t nnodes, const int32_t* indptr,
+                         const int32_t* diag_slot, const double* kappa,
+                         const double* grad_data, const double* grad_load, double* dkappa,
+                         double* dX, femhip_stream_t s);
 
 /* ---------------------------------------------------------------------
  * The assembled system on the device, for the solve.

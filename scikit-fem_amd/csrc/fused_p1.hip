@@ -14,22 +14,13 @@
 //      sum of the row's off-diagonal entries (each element matrix has zero
 //      row sums), so diagonal contributions are never formed.
 //
-// Summation is deterministic without floating-point atomics: a
-// contribution v enters the LDS accumulators as the integer round(v * 2^s)
-// (int64, LDS integer atomics).  Integer addition is associative, so the
-// sums do not depend on the order in which lanes and waves arrive.  One
-// exponent s serves the whole launch: s = 49 - ilogb(M) rounded down to a
-// multiple of 4, M = the largest |contribution| of this input, so
-// 2^46 <= M 2^s < 2^50: no conversion leaves the 2^51 range of the
-// magic-number rounding, a slot of n contributions is within
-// n 2^-(s+1) <= n 7.2e-15 M of its exact sum, and sums stay below 2^62.
-// The pass uses the exponent the previous call measured and records the
-// exponent field of its own M (atomicMax, tagged with a pass epoch so that
-// no reset is needed); the companion launch (mode 1) derives this input's
-// exponent from it and repeats the pass when the two differ -- otherwise it
-// returns at once.  The values are therefore a function of the input
-// alone.  No workgroup waits on another (no fences, no last-block
-// reductions): the launch boundaries order the state updates.
+// Summation is deterministic without floating-point atomics: contributions
+// enter the LDS accumulators as int64 fixed-point values (LDS integer
+// atomics), one exponent per launch (fused_common.h).  Integer addition is
+// associative, so the sums do not depend on the order in which lanes and
+// waves arrive; a slot of n contributions is within n 2^-(s+1) <=
+// n 7.2e-15 M of its exact sum (M: the largest |contribution|), and sums
+// stay below 2^62.  The values are a function of the input alone.
 //
 // Validation of changed inputs (first-pass suspects): an off-diagonal
 // entry that the reference computes as exactly 0.0 (and eliminate_zeros
@@ -40,19 +31,11 @@
 // is queued as a suspect (element, row, column) and rechecked in the
 // reference's arithmetic afterwards; every other entry is nonzero in the
 // reference too.  Non-finite or vanishing element data sets a flag.
-#include <utility>
-
-#include "common.h"
 #include "fused_check.h"
+#include "fused_common.h"
 
 namespace femhip {
 namespace {
-
-constexpr double kFxMagic = 6755399441055744.0;  // 1.5 * 2^52
-static_assert(sizeof(femhip_fused_state_t) == 256 &&
-                  offsetof(femhip_fused_state_t, maxk) == 128,
-              "femhip_fused_state_t layout: read line, then atomic line");
-constexpr unsigned kRepeatGrid = 768;   // 3 workgroups on each of 256 CUs
 
 struct FxArgs {
   const int32_t* parts;
@@ -75,103 +58,6 @@ struct FxArgs {
   int32_t mode;      // 0: first pass, 1: repeat with the measured exponent
 };
 
-template <int POS, int NB, int RW>
-__device__ __forceinline__ uint32_t bits(const uint32_t (&w)[RW]) {
-  static_assert(NB <= 32, "field too wide");
-  constexpr int wi = POS >> 5, sh = POS & 31;
-  if constexpr (sh + NB <= 32) {
-    return (w[wi] >> sh) & ((1u << NB) - 1u);
-  } else {
-    const uint64_t v = (uint64_t)w[wi] | ((uint64_t)w[wi + 1] << 32);
-    return (uint32_t)(v >> sh) & ((1u << NB) - 1u);
-  }
-}
-
-// A field inside one word as one v_bfe_u32 (inline assembly, so that a
-// following shift by 3 folds into the address add); fields that straddle
-// two words use bits().
-template <int POS, int NB, int RW>
-__device__ __forceinline__ uint32_t field(const uint32_t (&w)[RW]) {
-  constexpr int wi = POS >> 5, sh = POS & 31;
-  if constexpr (sh + NB <= 32 && sh > 0) {
-    uint32_t r;
-    asm("v_bfe_u32 %0, %1, %2, %3" : "=v"(r) : "v"(w[wi]), "I"(sh), "I"(NB));
-    return r;
-  } else {
-    return bits<POS, NB, RW>(w);
-  }
-}
-
-template <int BASE, int FB, int RW, int... F>
-__device__ __forceinline__ void decode_fields(const uint32_t (&w)[RW], uint32_t* o,
-                                              std::integer_sequence<int, F...>) {
-  ((o[F] = field<BASE + F * FB, FB, RW>(w)), ...);
-}
-
-// Morton-adjacent partitions on the same XCD (workgroups are dealt to the
-// eight XCDs round-robin).
-__device__ __forceinline__ int xcd_remap(int i, int n) {
-  const int q = n / 8, r = n % 8;
-  const int x = i % 8, j = i / 8;
-  return x * q + (x < r ? x : r) + j;
-}
-
-// 1/d: hardware reciprocal estimate refined by two Newton steps
-// r <- r + r (1 - d r) (error squared per step; ~1 ulp at the end).
-__device__ __forceinline__ double recip(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = __fma_rn(-d, r, 1.0);
-  r = __fma_rn(r, e, r);
-  e = __fma_rn(-d, r, 1.0);
-  return __fma_rn(r, e, r);
-}
-
-// round(v * scale) as int64 for |v * scale| < 2^51: the fma rounds the
-// scaled value to an integer held in the low mantissa bits of the sum.
-__device__ __forceinline__ unsigned long long to_fixed(double v, double scale) {
-  const double y = __fma_rn(v, scale, kFxMagic);
-  return (unsigned long long)(__double_as_longlong(y) - __double_as_longlong(kFxMagic));
-}
-
-__device__ __forceinline__ double from_fixed(long long v, double inv_scale) {
-  return (double)v * inv_scale;
-}
-
-// The pass maxima travel as (epoch << 11) | the 11-bit exponent field of
-// the double: atomicMax keeps the largest exponent of the newest epoch, so
-// a new pass overrides the previous pass's maximum without a reset.
-__device__ __forceinline__ unsigned long long exponent_field(double m) {
-  return ((unsigned long long)__double_as_longlong(fabs(m)) >> 52) & 0x7ffull;
-}
-
-// exponent bucket of the largest |contribution| from its exponent field e:
-// 2^(e-1023) <= m < 2^(e-1022), so s = 49 - (e - 1023) rounded down to a
-// multiple of 4 gives 2^46 <= m 2^s < 2^50 (e = 0: zero or subnormal; e =
-// 2047: not finite, flagged by the pass itself)
-__device__ __forceinline__ int fx_bucket_tagged(unsigned long long tagged) {
-  const int e = (int)(tagged & 0x7ffull);
-  if (e == 0 || e == 0x7ff) return 0;
-  int s = 49 - (e - 1023);
-  s = s >= 0 ? (s & ~3) : -((-s + 3) & ~3);
-  return s;
-}
-
-__device__ __forceinline__ void lds_add(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// max(m, |v|) in one instruction (|v| as a source modifier)
-__device__ __forceinline__ double max_abs(double m, double v) {
-  double r;
-  asm("v_max_f64 %0, %1, |%2|" : "=v"(r) : "v"(m), "v"(v));
-  return r;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
 
 // Local matrix of a P1 simplex from its vertex coordinates in LDS.
 // Laplace: K_ab = coef (n_a . n_b) / (d! |det|) with n_a the cross-product
@@ -341,21 +227,11 @@ __device__ __forceinline__ void push_suspects(const FxArgs& a, uint32_t mask, in
   }
 }
 
-// Partition of the k-th take of this workgroup: one per workgroup, or a
-// strided walk of a persistent grid; consecutive workgroups (dealt to the
-// eight XCDs round-robin) get spatially adjacent partitions on one XCD.
-// Returns -1 when none is left.
-__device__ __forceinline__ int next_partition(const FxArgs& a, int k) {
-  const int pi = blockIdx.x + k * gridDim.x;
-  return pi < a.nparts ? xcd_remap(pi, a.nparts) : -1;
-}
-
 template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD, int NT>
 __global__ void __launch_bounds__(NT)
     __attribute__((amdgpu_waves_per_eu(LOAD ? 4 : (NT == 1024 ? 8 : 6))))
 p1_fused_kernel(FxArgs a) {
   constexpr int NV = D + 1;
-  constexpr int NW = NT / 64;
   constexpr bool ROWSUM = FORM != FEMHIP_MASS;
   constexpr double LOADC = (D == 3) ? 1.0 / 4.0 : 1.0 / 3.0;  // int phi = vol / (d+1)
   extern __shared__ double smem[];
@@ -363,45 +239,17 @@ p1_fused_kernel(FxArgs a) {
   const int tid = threadIdx.x;
   femhip_fused_state_t* st = a.st;
   const bool first = a.mode == 0;
-  int s, sv;
-  bool known;
-  if (first) {
-    // uniform values: read into scalar registers (workgroup 0 stores to
-    // the state below, so the compiler would otherwise keep them in vector
-    // registers and treat every branch on them as divergent)
-    known = __builtin_amdgcn_readfirstlane(st->valid) != 0;
-    s = known ? __builtin_amdgcn_readfirstlane(st->s) : 0;
-    sv = known ? __builtin_amdgcn_readfirstlane(st->sv) : 0;
-    if (blockIdx.x == 0 && tid == 0) {
-      st->s_first = known ? s : INT32_MIN;
-      st->sv_first = known ? sv : INT32_MIN;
-    }
-  } else {
-    // the exponents of this input (the first pass's maxima, final at the
-    // launch boundary); repeat only when the first pass used others.  The
-    // fields read here are not written by this launch.
-    const int s2 = fx_bucket_tagged(st->maxk), sv2 = LOAD ? fx_bucket_tagged(st->maxv) : 0;
-    const bool redo = st->s_first == INT32_MIN || s2 != st->s_first ||
-                      (LOAD && sv2 != st->sv_first);
-    if (blockIdx.x == 0 && tid == 0) {   // for the next first pass
-      st->s = s2;
-      if constexpr (LOAD) st->sv = sv2;
-      st->valid = 1;
-      st->epoch = (st->maxk >> 11) + 1;
-    }
-    if (!redo) return;  // uniform
-    known = true;
-    s = s2;
-    sv = sv2;
-  }
-  const bool scatter = known;
+  const FxPass pass = fx_begin<LOAD>(st, a.mode);
+  if (!pass.run) return;  // uniform
+  const int s = pass.s, sv = pass.sv;
+  const bool scatter = pass.scatter;
   const double sck = ldexp(1.0, s), scv = ldexp(1.0, sv);
   const bool suspects = first && a.susp != nullptr;
   double mk = 0.0, mv = 0.0;
   bool bad = false;
 
   for (int k = 0;; ++k) {
-    const int part = next_partition(a, k);
+    const int part = next_partition(a.nparts, k);
     if (part < 0) break;
     if (k > 0) __syncthreads();  // LDS reuse
     const int32_t* hdr = a.parts + 8 * (int64_t)part;
@@ -604,30 +452,7 @@ p1_fused_kernel(FxArgs a) {
     }
   }  // partitions
 
-  // workgroup maxima -> the launch's maxima (read by the repeat launch; no
-  // fence: the kernel boundary orders them)
-  if (!first) return;
-  mk = wave_max(mk);
-  if constexpr (LOAD) mv = wave_max(mv);
-  const bool wbad = __any(bad);
-  if ((tid & 63) == 0) {
-    L.red[tid >> 6] = mk;
-    L.red[NW + (tid >> 6)] = mv;
-    if (wbad && suspects) atomicOr(&st->flags, FEMHIP_FUSED_BAD);
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double MK = L.red[0], MV = L.red[NW];
-#pragma unroll
-    for (int q = 1; q < NW; ++q) {
-      MK = fmax(MK, L.red[q]);
-      MV = fmax(MV, L.red[NW + q]);
-    }
-    const unsigned long long ep = st->epoch << 11;
-    atomicMax(reinterpret_cast<unsigned long long*>(&st->maxk), ep | exponent_field(MK));
-    if constexpr (LOAD)
-      atomicMax(reinterpret_cast<unsigned long long*>(&st->maxv), ep | exponent_field(MV));
-  }
+  if (first) fx_end<NT, LOAD>(st, mk, mv, bad, suspects, L.red);
 }
 
 template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>

@@ -14,8 +14,9 @@
 // Per element visit the plan stores one packed record: the local ids of the
 // element's vertices (partition-local numbering, owned nodes first, each
 // group in global order) and, for every vertex a and other vertex b, the
-// offset of column b inside row a of the CSR pattern (unused when a is a
-// halo vertex).  A partition's records are contiguous; the kernel takes
+// offset of column b inside row a of the CSR pattern (a halo vertex's row
+// too: the values kernel never reads those, the adjoint reads its upstream
+// gradient there).  A partition's records are contiguous; the kernel takes
 // them in chunks of `block`.  Records are ordered by a greedy colouring
 // (elements of one colour share no owned vertex, so the lanes of a chunk
 // of one colour add into distinct rows) or by element id.
@@ -211,6 +212,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
     std::vector<int32_t> nodemap, rows, rec_elem;
     std::vector<uint32_t> recs;
     int L = 0, R = 0, lds = 0, nelems = 0, err = 0;
+    bool pruned = false;   // some owned row lacks a pair of one of its elements
     std::string msg;
   };
   struct Scratch {
@@ -327,6 +329,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
       O.rows.push_back((int32_t)((uint32_t)lds | ((uint32_t)diag << 16) |
                                  ((uint32_t)len << 24) | ((uint32_t)spare[r] << 31)));
       lds += len + spare[r];
+      O.pruned = O.pruned || spare[r];
     }
     if (lds > 65535) {
       release();
@@ -348,15 +351,15 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
       for (int a = 0; a < NV; ++a)
         for (int b = 0; b < NV; ++b) {
           if (b == a) continue;
-          uint32_t off = 0;  // halo row: never read
-          if (lv[a] < R) {
-            const int32_t g = gv[a];
-            const int32_t* s = indices + indptr[g];
-            const int32_t* f = indices + indptr[g + 1];
-            const int32_t* it = std::lower_bound(s, f, gv[b]);
-            // pruned pair: the row's spare slot (offset = row length)
-            off = (it != f && *it == gv[b]) ? (uint32_t)(it - s) : (uint32_t)(f - s);
-          }
+          // the pair's offset in row a's CSR slots, for halo rows too (the
+          // adjoint reads their upstream values from global memory); a
+          // pruned pair: the row's spare slot (offset = row length)
+          const int32_t g = gv[a];
+          const int32_t* s = indices + indptr[g];
+          const int32_t* f = indices + indptr[g + 1];
+          const int32_t* it = std::lower_bound(s, f, gv[b]);
+          const uint32_t off =
+              (it != f && *it == gv[b]) ? (uint32_t)(it - s) : (uint32_t)(f - s);
           put_bits(w, bitpos, offbits, off);
           bitpos += offbits;
         }
@@ -396,8 +399,10 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   std::memset(&P->info, 0, sizeof(P->info));
   int max_slots = 0, max_nodes = 0, max_rows = 0, max_recs = 0;
   int64_t visits = 0;
+  bool pruned = false;
   for (size_t leaf = 0; leaf < outs.size(); ++leaf) {
     LeafOut& O = outs[leaf];
+    pruned = pruned || O.pruned;
     const int32_t node_start = (int32_t)P->nodemap.size();
     const int32_t row_start = (int32_t)(P->rows.size() / 2);
     const int32_t rec_start = (int32_t)P->rec_elem.size();
@@ -433,6 +438,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   I.idbits = idbits;
   I.offbits = offbits;
   I.block = block;
+  I.pruned = pruned ? 1 : 0;
   I.dup = E > 0 ? (double)visits / (double)E : 0.0;
   *info_out = I;
   *handle_out = P;

@@ -58,7 +58,7 @@ class FusedInfo(ctypes.Structure):
     _fields_ = [('nparts', _i32), ('max_recs', _i32), ('nrows', _i32), ('max_slots', _i32),
                 ('max_nodes', _i32), ('max_rows', _i32), ('rec_words', _i32),
                 ('idbits', _i32), ('offbits', _i32), ('block', _i32),
-                ('reserved0', _i32), ('reserved1', _i32), ('nrecs', _i64), ('nlocal', _i64),
+                ('pruned', _i32), ('reserved1', _i32), ('nrecs', _i64), ('nlocal', _i64),
                 ('dup', _f64)]
 
 
@@ -124,6 +124,9 @@ SIGNATURES = {
                                              _i64, _p, _p, _p, ctypes.POINTER(FusedCheck),
                                              _i32, _p]),
     'femhip_fused_static_queue': (_i64, []),
+    'femhip_fused_adjoint_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(FusedInfo), _i32]),
+    'femhip_fused_adjoint': (ctypes.c_int, [ctypes.POINTER(FusedDev), _p, _i32, _p, _i64, _p,
+                                            _p, _p, _p, _p, _p, _p, _p]),
     'femhip_fused_assemble_static': (ctypes.c_int, [ctypes.POINTER(FusedDev), _i32, _i32, _p,
                                                     _i64, _p, _p, _p,
                                                     ctypes.POINTER(FusedCheck), _i32, _p]),
@@ -133,7 +136,6 @@ SIGNATURES = {
                                                    _f64, _f64, _p, _p, _i32, _p]),
     'femhip_segment_sum_em': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _p]),
     'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
-    'femhip_invert_plan': (ctypes.c_int, [_p, _p, _i64, _p, _p]),
     'femhip_facet_local': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _i32, _p, _i32, _p, _p]),
     'femhip_facet_interpolate': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p, _p]),
     'femhip_facet_global_coordinates': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p]),
@@ -143,7 +145,6 @@ SIGNATURES = {
     'femhip_facet_normals': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p]),
     'femhip_interpolate': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space), _p, _p,
                                           _p, _p, _p]),
-    'femhip_segment_sum_multi': (ctypes.c_int, [_p, _i32, _i64, _p, _p, _i64, _p, _i64, _p]),
     'femhip_csr_select_workspace': (ctypes.c_int, [_i64, ctypes.POINTER(ctypes.c_size_t)]),
     'femhip_csr_select_pattern': (ctypes.c_int, [_p, _p, _p, _i64, _p, _p,
                                                  ctypes.POINTER(_i64), _p, ctypes.c_size_t,
@@ -155,8 +156,6 @@ SIGNATURES = {
     'femhip_pcg': (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _f64, _f64, _i64, _i32, _p,
                                   ctypes.c_size_t, ctypes.POINTER(_i64),
                                   ctypes.POINTER(_f64), _p]),
-    'femhip_p1_adjoint': (ctypes.c_int, [_i32, _i64, _i64, _p, _p, _p, _p, _p, _p, _p,
-                                         _p, _p]),
 }
 
 _lib = None

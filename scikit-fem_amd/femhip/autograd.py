@@ -1,5 +1,5 @@
 """Differentiable P1 assembly: ``torch.autograd.Function`` whose backward is
-the hand-written adjoint kernel (``femhip_p1_adjoint``).
+the hand-written fused adjoint kernel (``femhip_fused_adjoint``).
 
     data, b, (indptr, indices) = kappa_laplace_with_load(basis, kappa, p)
 
@@ -8,11 +8,13 @@ of the mesh (every (row, col) pair coupled by an element, no exact-zero
 pruning: a slot that happens to be 0 for these coordinates still has a
 nonzero derivative with respect to them), ``b`` the unit-load vector.  The
 backward maps (dL/ddata, dL/db) to dL/dkappa (per element) and dL/dp (node
-coordinates), recomputing the element geometry instead of storing it; node
-sums use the deterministic rows plan (no atomics).
+coordinates) in one pass over the forward's fused plan, recomputing the
+element geometry instead of storing it; the node sums are formed in the
+plan's partitions (deterministic fixed point, no per-element array).
 """
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -20,8 +22,8 @@ from . import fused as _fused
 
 
 class _Ctx:
-    """Per-basis state of the differentiable path (connectivity plan, its
-    fused plan, slot map and rows plan)."""
+    """Per-basis state of the differentiable path: the connectivity pattern,
+    its fused plan and the CSR slot of each row's diagonal."""
 
     def __init__(self, basis):
         d = basis.device()
@@ -29,14 +31,17 @@ class _Ctx:
             raise NotImplementedError("the adjoint supports P1 triangles/tetrahedra")
         self.d = d
         self.plan = d.build_plan(None)            # connectivity pattern
-        self.fp = _fused.FusedPlan(basis.mesh, self.plan.indptr.cpu().numpy(),
-                                   self.plan.indices.cpu().numpy())
-        C = d.nbfun * d.nbfun * d.E
-        self.slot_of = torch.full((C,), -1, dtype=torch.int32, device=d.p.device)
-        _lib.check(d.lib.femhip_invert_plan(_lib.ptr(self.plan.perm), _lib.ptr(self.plan.seg),
-                                            self.plan.nnz, _lib.ptr(self.slot_of),
-                                            _lib.stream_ptr()))
-        self.rows = d.rows_plan()
+        indptr = self.plan.indptr.cpu().numpy()
+        indices = self.plan.indices.cpu().numpy()
+        self.fp = _fused.FusedPlan(basis.mesh, indptr, indices)
+        if self.fp.info.pruned:
+            raise RuntimeError("the connectivity pattern lacks an element pair")
+        n = len(indptr) - 1
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(indptr))
+        k = np.flatnonzero(indices == rows)
+        if len(k) != n:
+            raise RuntimeError("a row of the connectivity pattern has no diagonal")
+        self.diag_slot = torch.tensor(k.astype(np.int32), device=d.p.device)
 
 
 def _ctx(basis):
@@ -66,23 +71,16 @@ class _KappaLaplaceLoad(torch.autograd.Function):
         kappa, p = ctx.saved_tensors
         c = _ctx(ctx.basis)
         d = c.d
-        E, nv, dim = d.E, d.nbfun, d.dim
-        dev = p.device
         gdata = None if gdata is None else gdata.contiguous()
         gload = None if gload is None else gload.contiguous()
-        dk = torch.empty(E, dtype=torch.float64, device=dev)
-        gX = torch.empty(dim * nv * E, dtype=torch.float64, device=dev)
-        _lib.check(d.lib.femhip_p1_adjoint(
-            dim, E, p.shape[1], _lib.ptr(p), _lib.ptr(d.t), _lib.ptr(kappa),
-            _lib.ptr(c.slot_of), _lib.ptr(gdata), _lib.ptr(gload), _lib.ptr(dk),
-            _lib.ptr(gX), _lib.stream_ptr()))
-        perm, seg = c.rows
+        dk = (torch.empty(d.E, dtype=torch.float64, device=p.device)
+              if ctx.needs_input_grad[0] else None)
         dX = torch.empty_like(p)
-        _lib.check(d.lib.femhip_segment_sum_multi(
-            _lib.ptr(gX), dim, nv * E, _lib.ptr(perm), _lib.ptr(seg), p.shape[1],
-            _lib.ptr(dX), p.shape[1], _lib.stream_ptr()))
-        return dk if ctx.needs_input_grad[0] else None, \
-            dX if ctx.needs_input_grad[1] else None, None
+        _lib.check(d.lib.femhip_fused_adjoint(
+            ctypes.byref(c.fp.dev), _lib.ptr(c.fp.state('adjoint')), d.dim, _lib.ptr(p),
+            p.shape[1], _lib.ptr(c.plan.indptr), _lib.ptr(c.diag_slot), _lib.ptr(kappa),
+            _lib.ptr(gdata), _lib.ptr(gload), _lib.ptr(dk), _lib.ptr(dX), _lib.stream_ptr()))
+        return dk, dX if ctx.needs_input_grad[1] else None, None
 
 
 def kappa_laplace_with_load(basis, kappa, p=None):

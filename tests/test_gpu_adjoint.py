@@ -108,3 +108,24 @@ def test_reference_gradient_65():
     dX = p.grad.cpu().numpy()[:, z['nodes']].T
     ref = z['dX']
     assert np.abs(dX - ref).max() <= 1e-7 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('kind,n', [('tri', 17), ('tet', 7)])
+def test_adjoint_is_bitwise_repeatable(kind, n):
+    """The node sums are fixed point with one exponent per launch: two
+    backward passes on the same upstream give identical gradients, also
+    after a pass on another upstream changed the exponent."""
+    m, b, kappa, u, f, g, p = _setup(kind, n, 0.2, seed=7)
+    kappa.requires_grad_(True)
+    p.requires_grad_(True)
+    data, load, _ = kappa_laplace_with_load(b, kappa, p)
+    gd = torch.randn_like(data)
+    gl = torch.randn_like(load)
+    r1 = torch.autograd.grad((data, load), (kappa, p), (gd, gl), retain_graph=True)
+    torch.autograd.grad((data, load), (kappa, p), (1e6 * gd, gl), retain_graph=True)
+    r2 = torch.autograd.grad((data, load), (kappa, p), (gd, gl), retain_graph=True)
+    assert torch.equal(r1[0], r2[0]) and torch.equal(r1[1], r2[1])
+    # linearity in the upstream (to the fixed-point rounding)
+    r3 = torch.autograd.grad((data, load), (kappa, p), (2 * gd, 2 * gl), retain_graph=True)
+    assert (r3[1] - 2 * r1[1]).abs().max() <= 1e-12 * r1[1].abs().max()
+    assert (r3[0] - 2 * r1[0]).abs().max() <= 1e-13 * r1[0].abs().max()
