@@ -237,6 +237,13 @@ def bench_batched_kappa(S=256, n=257):
     stream = torch.cuda.current_stream()
     ks.assemble(kt, out)
     ms_batch = time_stream(lambda: ks.assemble(kt, out), 5, stream)
+    # the same launch captured in a HIP graph and replayed
+    kg = kt.clone()
+    og = torch.empty_like(out)
+    ks.capture(kg, og)
+    ms_graph = time_stream(ks.replay, 5, stream)
+    torch.cuda.synchronize()
+    assert torch.equal(og, out)
     # plain: one drop-in GPU assembly per sample (exact path; kappa changes
     # every call, so each call also validates its pattern)
     nplain = 8
@@ -249,6 +256,7 @@ def bench_batched_kappa(S=256, n=257):
     ms_plain = (time.perf_counter() - t0) * 1e3 / nplain
     return {"samples": S, "elements": m.nelements, "dofs": b.N, "nnz": ks.nnz,
             "batched_ms_total": ms_batch, "batched_ms_per_sample": ms_batch / S,
+            "graph_replay_ms_total": ms_graph,
             "plain_ms_per_sample": ms_plain,
             "speedup_per_sample": ms_plain / (ms_batch / S),
             "batched_GBps_written": S * ks.nnz * 8 / (ms_batch * 1e-3) / 1e9}
@@ -550,6 +558,18 @@ def bench_p2_tet(n=41):
             "max_rel_err_vs_first_call": err}
 
 
+def bench_autotune(basis):
+    """The fused kernel's autotuner on the 101^3 Laplace: every candidate
+    launch configuration timed, the fastest installed on the basis (the
+    headline above uses the default configuration)."""
+    from femhip import autotune
+    rec = autotune.tune_fused(basis, 'laplace', steps=10)
+    default = next(ms for cfg, ms, dup in rec['trials']
+                   if tuple(cfg) == tuple(autotune.CANDIDATES[0]))
+    return {"best": rec['best'], "best_ms": rec['ms'], "default_ms": default,
+            "trials": [[cfg, round(ms, 4), round(dup, 3)] for cfg, ms, dup in rec['trials']]}
+
+
 def bench_elasticity(n=41):
     """linear_elasticity on ElementVector(ElementHex2), 27 points, MeshHex
     (n-1)^3 jittered 0.15h: MFMA local matrices + segmented sums."""
@@ -613,7 +633,7 @@ def main():
     ap.add_argument('--extra', default=None,
                     help='comma-separated subset of the extra measurements '
                          '(changed, interpolate, facet, device_solve, batched_kappa, '
-                         'mesh_batch, adjoint, elasticity, p2_tet)')
+                         'mesh_batch, adjoint, elasticity, p2_tet, autotune)')
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -746,6 +766,7 @@ def main():
             ('adjoint', "adjoint_kappa_tet65", bench_adjoint),
             ('elasticity', "elasticity_vhex2_40", bench_elasticity),
             ('p2_tet', "laplace_p2tet41", bench_p2_tet),
+            ('autotune', "autotune_tet101", lambda: bench_autotune(basis)),
         )
         chosen = None if args.extra is None else set(args.extra.split(','))
         for short, name, fn in extras:

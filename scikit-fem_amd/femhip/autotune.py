@@ -8,7 +8,7 @@ Tunables of ``FusedPlan`` / ``femhip_fused_assemble``:
 * the workgroup size (elements per chunk: 512 or 1024);
 * the record order inside a partition (colour classes or element order);
 * the grid: one workgroup per partition, or a persistent grid of ``groups``
-  workgroups taking partitions from per-XCD work counters.
+  workgroups walking the partitions with a fixed stride.
 
 The quadrature loops need no run-time choice: P1 gradients are constant per
 element (the fused kernel has no point loop) and the exact kernels unroll

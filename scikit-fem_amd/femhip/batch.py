@@ -66,6 +66,28 @@ class KappaSamples:
             self.nnz, E, _lib.ptr(kappa), S, _lib.ptr(out), _lib.stream_ptr()))
         return out
 
+    def capture(self, kappa, out):
+        """Capture ``assemble(kappa, out)`` in a HIP graph.  ``kappa`` (S, E)
+        and ``out`` (S, nnz) become the graph's buffers: copy new samples
+        into ``kappa`` and call ``replay`` (the launch reads and writes
+        those addresses on every replay)."""
+        assert kappa.is_contiguous() and out.is_contiguous()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.assemble(kappa, out)            # warm-up outside the graph
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.assemble(kappa, out)
+        self.graph = g
+        self.graph_buffers = (kappa, out)
+        return g
+
+    def replay(self):
+        self.graph.replay()
+        return self.graph_buffers[1]
+
 
 class MeshBatch:
     """Laplace + unit load on many P1 meshes in one launch."""

@@ -100,3 +100,27 @@ def test_mesh_batch_graph_replays():
         torch.cuda.synchronize()
         assert torch.equal(d, d0) and torch.equal(l, l0), rep
         _ = (torch.ones(1000, device='cuda') * rep).sum().item()   # eager work
+
+
+def test_kappa_samples_graph_replays():
+    """The captured κ-sample launch: new samples copied into the graph's
+    input buffer before each replay; every replay equals the eager launch on
+    the same samples (eager work and synchronisations between replays)."""
+    x = np.linspace(0, 1, 13)
+    m = femhip.MeshTri.init_tensor(x, x)
+    b = femhip.Basis(m, femhip.ElementTriP2())
+    ks = KappaSamples(b)
+    S = 4
+    rng = np.random.default_rng(11)
+    kap = torch.tensor(rng.lognormal(0, 0.5, (S, m.nelements)), device='cuda')
+    out = torch.empty((S, ks.nnz), dtype=torch.float64, device='cuda')
+    ks.capture(kap, out)
+    for r in range(4):
+        print('replay', r, flush=True)
+        new = torch.tensor(rng.lognormal(0, 0.5, (S, m.nelements)), device='cuda')
+        kap.copy_(new)
+        ks.replay()
+        torch.cuda.synchronize()
+        ref = ks.assemble(new.clone())
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), r
