@@ -520,11 +520,12 @@ def bench_facet(m):
 
 def bench_p2_tet(n=41):
     """Laplace on ElementTetP2, MeshTet 41^3 jittered (384,000 tets,
-    531,441 DoF, 11 points at intorder 4): the exact path's steady state
-    (local entries in the reference's arithmetic + segmented sums over the
-    cached pattern).  Bytes: the local kernel writes 8 B x 100 entries per
-    element (and reads the mapping), the sums read 12 B per contribution
-    and write 8 B per slot."""
+    531,441 DoF, 11 points at intorder 4), later calls: the fused rows
+    kernel (entries in the reference's arithmetic summed into LDS rows, the
+    pattern checked in the same pass, verdict read back by the drop-in
+    call), beside the exact path it replaces (local entries + segmented
+    sums over the cached pattern).  Algorithmic bytes: 40 B element DOFs per
+    element + 24 B coordinates per mesh vertex + 8 B per CSR value."""
     import torch
     import femhip
     from femhip.models import laplace
@@ -537,25 +538,58 @@ def bench_p2_tet(n=41):
     torch.cuda.synchronize()
     first = (time.perf_counter() - t0) * 1e3
     d = b.device()
-    plan = next(iter(d.plans.values()))
+    plan = d.rows_last['laplace']
+    fp = d.rows_plans[plan.signature]
     E, nb = d.E, d.nbfun
-    contrib = torch.empty(nb * nb * E, dtype=torch.float64, device='cuda')
-    out = torch.empty(plan.nnz, dtype=torch.float64, device='cuda')
     stream = torch.cuda.current_stream()
+    out = torch.empty(plan.nnz, dtype=torch.float64, device='cuda')
+    fp.assemble('laplace', d, out)
+    fused_ms = time_stream(lambda: fp.assemble('laplace', d, out), 10, stream)
+    torch.cuda.synchronize()
+    verdict = fp.verdict.cpu().tolist()
+    err = float((out - A.data).abs().max() / A.data.abs().max())
+    # the exact path's steady state on the same pattern
+    contrib = torch.empty(nb * nb * E, dtype=torch.float64, device='cuda')
+    out2 = torch.empty(plan.nnz, dtype=torch.float64, device='cuda')
     d.local_bilinear('laplace', out=contrib)
     local_ms = time_stream(lambda: d.local_bilinear('laplace', out=contrib), 10, stream)
-    sum_ms = time_stream(lambda: d.segment_sum(contrib, plan.perm, plan.seg, plan.nnz, out=out),
-                         10, stream)
+    sum_ms = time_stream(lambda: d.segment_sum(contrib, plan.perm, plan.seg, plan.nnz,
+                                               out=out2), 10, stream)
     torch.cuda.synchronize()
-    err = float((out - A.data).abs().max() / A.data.abs().max())
-    C = nb * nb * E
+    A2 = laplace.assemble_device(b)          # the drop-in call (fused, validated)
+    torch.cuda.synchronize()
+    dropin = bool(d.last_verdict and d.last_verdict[0] == 1 and A2.nnz == A.nnz)
+    # new interior coordinates on every drop-in call (validated in the pass)
+    rng = np.random.default_rng(31)
+    I = femhip.MeshTet.init_tensor(x, x, x).interior_nodes()
+    base = femhip.MeshTet.init_tensor(x, x, x).p
+    coords = []
+    for _ in range(4):
+        p = base.copy()
+        p[:, I] += rng.uniform(-0.2 / (n - 1), 0.2 / (n - 1), size=(3, len(I)))
+        coords.append(torch.tensor(p, device='cuda'))
+    p0 = d.p
+    fused_calls = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for pc in coords:
+        d.set_coordinates(pc)
+        laplace.assemble_device(b)
+        fused_calls += int(d.last_verdict is not None and d.last_verdict[0] == 1)
+    torch.cuda.synchronize()
+    new_coords_ms = (time.perf_counter() - t0) * 1e3 / len(coords)
+    d.set_coordinates(p0)
+    alg = 40 * E + 24 * m.p.shape[1] + 8 * plan.nnz
     return {"elements": E, "dofs": b.N, "nnz": plan.nnz, "points": d.Q,
-            "first_call_ms": first, "local_ms": local_ms, "segment_sum_ms": sum_ms,
-            "assembly_ms": local_ms + sum_ms,
-            "elements_per_s": E / ((local_ms + sum_ms) * 1e-3),
-            "local_GBps_written": 8 * C / (local_ms * 1e-3) / 1e9,
-            "sum_GBps": (12 * C + 8 * plan.nnz) / (sum_ms * 1e-3) / 1e9,
-            "max_rel_err_vs_first_call": err}
+            "first_call_ms": first, "fused_rows_ms": fused_ms,
+            "elements_per_s": E / (fused_ms * 1e-3),
+            "alg_bytes": alg, "frac_of_8TBps": alg / (fused_ms * 1e-3) / HBM_PEAK,
+            "verdict": verdict, "fused_plan": fp.stats(),
+            "max_rel_err_vs_first_call": err,
+            "dropin_fused": dropin, "new_coordinates_ms": new_coords_ms,
+            "new_coordinates_fused_fraction": fused_calls / len(coords),
+            "exact_path_local_ms": local_ms, "exact_path_segment_sum_ms": sum_ms,
+            "exact_path_ms": local_ms + sum_ms}
 
 
 def bench_autotune(basis):

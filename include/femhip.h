@@ -274,7 +274,7 @@ typedef struct femhip_fused_plan_info {
   int32_t max_nodes, max_rows, rec_words, idbits;
   int32_t offbits, block;
   int32_t pruned;       /* 1: the pattern lacks a pair of some element      */
-  int32_t reserved1;
+  int32_t nrowrecs;     /* rows plans: (visit, owned DOF) records           */
   int64_t nrecs, nlocal;
   double dup;           /* element visits / elements (halo recomputation)  */
 } femhip_fused_plan_info_t;
@@ -287,6 +287,25 @@ int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t nelems, int64_t
 int femhip_fused_plan_export(void* handle, int32_t* parts, int32_t* nodemap, int32_t* rows,
                              uint32_t* recs, int32_t* rec_elem);
 void femhip_fused_plan_free(void* handle);
+
+/* Rows plan of a P2 simplex mesh (nverts 6 / 10 DOFs per element, `p` the
+ * DOF locations used only for the partition): the same partitions, node
+ * maps and row tables as a fused plan of the DOF graph, records holding the
+ * element's local DOF ids (16 bits each) and one row record per (visit,
+ * owned DOF) of six words -- (visit << 4) | local DOF, then 7-bit column
+ * offsets four per word: of the element's DOFs in the DOF's row, and of the
+ * DOF in the rows of the element's other owned DOFs (127: not in the
+ * pattern), the DOF's local row in the top nibbles of words 1-4 --
+ * exported by femhip_rows_fused_plan_export (info.nrowrecs records;
+ * parts[8k + 5..7] = first row record, row records, first visit).  Freed with
+ * femhip_fused_plan_free; the other arrays by femhip_fused_plan_export. */
+int femhip_rows_fused_plan_build(int32_t nverts, int32_t dim, int64_t nelems, int64_t ndofs,
+                                 const int32_t* t, const double* p, const int32_t* indptr,
+                                 const int32_t* indices, int32_t slot_cap, int32_t node_cap,
+                                 int32_t block, void** handle,
+                                 femhip_fused_plan_info_t* info);
+int femhip_rows_fused_plan_export(void* handle, int32_t* rowrecs);
+
 
 /* The exported plan in device memory. */
 typedef struct femhip_fused_dev {
@@ -381,6 +400,25 @@ int femhip_fused_assemble_static(const femhip_fused_dev_t* plan, int32_t form, i
                                  const double* p, int64_t nnodes, const double* kappa,
                                  double* data, double* load, const femhip_fused_check_t* check,
                                  int32_t groups, femhip_stream_t s);
+
+/* LDS bytes per workgroup that femhip_rows_fused_assemble asks for. */
+size_t femhip_rows_fused_lds_bytes(const femhip_fused_plan_info_t* info);
+
+/* data[nnz] <- CSR values of `form` (FEMHIP_LAPLACE or FEMHIP_MASS) for a
+ * P2 simplex basis (g, sp as for femhip_local_bilinear; G its
+ * femhip_geometry) on the pattern the rows plan was built for: the local
+ * entries in the reference's arithmetic (BilinearForm._assemble,
+ * bilinear_form.py:58-128) summed in LDS rows (COOData._assemble_scipy_csr,
+ * coo_data.py:28-36), no local-matrix array.  The pattern is checked
+ * exactly in the same pass: with check != NULL, check->verdict (device
+ * int32[4]) <- {valid, 0, 0, flags} (its other fields unused; nzeros = 0).
+ * Two launches, as femhip_fused_assemble.  rowrecs: the exported row
+ * records. */
+int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const uint32_t* rowrecs,
+                               femhip_fused_state_t* state, int32_t form,
+                               const femhip_geom_t* g, const femhip_space_t* sp,
+                               const double* G, double* data,
+                               const femhip_fused_check_t* check, femhip_stream_t s);
 
 /* out[S][nseg] <- kappa-weighted segmented sums  sum_k kappa[s][c_k % E] *
  * K1[c_k]  over the plan (perm, seg): many per-element coefficient samples

@@ -47,6 +47,10 @@ void set_error(const std::string& msg);
 
 namespace {
 
+// rows plans: words per row record (visit << 4 | DOF, then 2 x 10 7-bit
+// column offsets)
+constexpr int kRowRecWords = 6;
+
 struct Plan {
   femhip_fused_plan_info_t info;
   std::vector<int32_t> parts;    // 8 int32 per partition
@@ -54,6 +58,7 @@ struct Plan {
   std::vector<int32_t> rows;     // 2 int32 per owned row: CSR row start, packed info
   std::vector<uint32_t> recs;    // rec_words uint32 per record
   std::vector<int32_t> rec_elem; // global element id per record
+  std::vector<uint32_t> rowrecs; // rows plans: kRowRecWords words per owned row visit
 };
 
 // Writes `nbits` (<= 32) of `v` at bit position `pos` of a little-endian
@@ -77,15 +82,22 @@ int plan_threads(int64_t nblocks) {
 
 }  // namespace
 
-extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, int64_t Nv,
-                                       const int32_t* t, const double* p,
-                                       const int32_t* indptr, const int32_t* indices,
-                                       int32_t slot_cap, int32_t node_cap, int32_t block,
-                                       int32_t order_mode, void** handle_out,
-                                       femhip_fused_plan_info_t* info_out) {
+namespace {
+
+// mode 0: P1 plans (records with the column offsets of every vertex pair);
+// mode 1: rows plans for P2 simplices (records with the local DOF ids only,
+// plus one row record per (visit, owned DOF); columns are found in LDS)
+int build_impl(int mode, int32_t nverts, int32_t dim, int64_t E, int64_t Nv, const int32_t* t,
+               const double* p, const int32_t* indptr, const int32_t* indices,
+               int32_t slot_cap, int32_t node_cap, int32_t block, int32_t order_mode,
+               void** handle_out, femhip_fused_plan_info_t* info_out) {
   using femhip::set_error;
-  if (!(nverts == 3 && dim == 2) && !(nverts == 4 && dim == 3)) {
+  if (mode == 0 && !(nverts == 3 && dim == 2) && !(nverts == 4 && dim == 3)) {
     set_error("fused plan: P1 triangles or tetrahedra only");
+    return FEMHIP_EUNSUPPORTED;
+  }
+  if (mode == 1 && !(nverts == 6 && dim == 2) && !(nverts == 10 && dim == 3)) {
+    set_error("rows plan: P2 triangles or tetrahedra only");
     return FEMHIP_EUNSUPPORTED;
   }
   if (!t || !p || !indptr || !indices || !handle_out || !info_out || E < 0 || Nv <= 0 ||
@@ -99,13 +111,17 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
     return FEMHIP_EOVERFLOW;
   }
   const int NV = nverts;
-  const int noff = NV * (NV - 1);
+  const int noff = mode == 0 ? NV * (NV - 1) : 0;
   int max_row = 0;
   for (int64_t n = 0; n < Nv; ++n) max_row = std::max(max_row, indptr[n + 1] - indptr[n]);
   // offsets 0..len (len: the spare slot of a row with pruned entries)
-  const int offbits = max_row <= 15 ? 4 : (max_row <= 31 ? 5 : 0);
-  if (offbits == 0) {
+  const int offbits = mode == 1 ? 0 : (max_row <= 15 ? 4 : (max_row <= 31 ? 5 : 0));
+  if (mode == 0 && offbits == 0) {
     set_error("fused plan: CSR rows longer than 31 entries");
+    return FEMHIP_EUNSUPPORTED;
+  }
+  if (mode == 1 && max_row > 126) {
+    set_error("rows plan: CSR rows longer than 126 entries");
     return FEMHIP_EUNSUPPORTED;
   }
   const int idbits = offbits == 4 ? 12 : 16;
@@ -210,7 +226,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   // on the number of threads
   struct LeafOut {
     std::vector<int32_t> nodemap, rows, rec_elem;
-    std::vector<uint32_t> recs;
+    std::vector<uint32_t> recs, rowrecs, rowkeys;
     int L = 0, R = 0, lds = 0, nelems = 0, err = 0;
     bool pruned = false;   // some owned row lacks a pair of one of its elements
     std::string msg;
@@ -326,9 +342,12 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
         if (indices[k] == g) diag = k - b;
       O.rows.push_back(b);
       // base (16 bits) | diagonal offset (8) | length (7) | spare slot (1)
+      // rows plans: no spare slot (the kernel finds the columns by search
+      // and flags a nonzero entry outside the pattern)
+      const uint32_t sp = mode == 0 ? (uint32_t)spare[r] : 0u;
       O.rows.push_back((int32_t)((uint32_t)lds | ((uint32_t)diag << 16) |
-                                 ((uint32_t)len << 24) | ((uint32_t)spare[r] << 31)));
-      lds += len + spare[r];
+                                 ((uint32_t)len << 24) | (sp << 31)));
+      lds += len + (int)sp;
       O.pruned = O.pruned || spare[r];
     }
     if (lds > 65535) {
@@ -337,15 +356,53 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
       O.msg = "fused plan: a block exceeds 65535 LDS slots";
       return;
     }
+    int32_t visit = 0;
     for (int32_t q : ord) {
       const int32_t e = elems[q];
-      uint32_t w[4] = {0, 0, 0, 0};
-      int32_t gv[4];
-      int lv[4];
+      uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      int32_t gv[10];
+      int lv[10];
       for (int a = 0; a < NV; ++a) {
         gv[a] = t[(int64_t)a * E + e];
         lv[a] = lid[gv[a]];
         put_bits(w, a * idbits, idbits, (uint32_t)lv[a]);
+      }
+      if (mode == 1) {
+        // one row record per owned DOF a: (visit << 4) | a, then 7-bit
+        // offsets, four per word: column b in row a (field b < NV), and
+        // column a in row b for every owned b != a (field NV + b; 127: the
+        // pair is not in the pattern)
+        auto offset_in = [&](int32_t g, int32_t col) -> uint32_t {
+          const int32_t* s0 = indices + indptr[g];
+          const int32_t* f0 = indices + indptr[g + 1];
+          const int32_t* it = std::lower_bound(s0, f0, col);
+          return (it != f0 && *it == col) ? (uint32_t)(it - s0) : 127u;
+        };
+        for (int a = 0; a < NV; ++a) {
+          if (lv[a] >= R) continue;
+          uint32_t rw[kRowRecWords] = {};
+          rw[0] = ((uint32_t)visit << 4) | (uint32_t)a;
+          // field k: word 1 + k / 4, bits 7 (k % 4): no field straddles a word
+          auto put_field = [&](int k, uint32_t v) {
+            put_bits(rw + 1, 32 * (k >> 2) + 7 * (k & 3), 7, v);
+          };
+          for (int b = 0; b < NV; ++b) {
+            put_field(b, offset_in(gv[a], gv[b]));
+            put_field(NV + b, (b != a && lv[b] < R) ? offset_in(gv[b], gv[a]) : 127u);
+          }
+          // the local row of DOF a in the top nibbles of offset words 1-4
+          for (int q = 0; q < 4; ++q) rw[1 + q] |= (((uint32_t)lv[a] >> (4 * q)) & 15u) << 28;
+          // sort key (not stored): the trial functions the record computes
+          uint32_t need = 0;
+          for (int b = 0; b < NV; ++b)
+            if (lv[b] >= R || b >= a) need |= 1u << b;
+          O.rowkeys.push_back(((uint32_t)a << 16) | need);
+          O.rowrecs.insert(O.rowrecs.end(), rw, rw + kRowRecWords);
+        }
+        for (int k = 0; k < rec_words; ++k) O.recs.push_back(w[k]);
+        O.rec_elem.push_back(e);
+        ++visit;
+        continue;
       }
       int bitpos = NV * idbits;
       for (int a = 0; a < NV; ++a)
@@ -365,6 +422,21 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
         }
       for (int k = 0; k < rec_words; ++k) O.recs.push_back(w[k]);
       O.rec_elem.push_back(e);
+    }
+    if (mode == 1) {
+      // row records ordered by (local DOF index, trial functions computed):
+      // a wave's lanes then mostly run the same entries
+      const size_t n = O.rowrecs.size() / kRowRecWords;
+      std::vector<uint32_t> idx(n);
+      for (size_t k = 0; k < n; ++k) idx[k] = (uint32_t)k;
+      std::stable_sort(idx.begin(), idx.end(),
+                       [&](uint32_t x, uint32_t y) { return O.rowkeys[x] < O.rowkeys[y]; });
+      std::vector<uint32_t> sorted(O.rowrecs.size());
+      for (size_t k = 0; k < n; ++k)
+        std::copy_n(O.rowrecs.begin() + (size_t)idx[k] * kRowRecWords, kRowRecWords,
+                    sorted.begin() + k * kRowRecWords);
+      O.rowrecs.swap(sorted);
+      std::vector<uint32_t>().swap(O.rowkeys);
     }
     O.nodemap.assign(locals.begin(), locals.end());
     release();
@@ -410,8 +482,23 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
     P->rows.insert(P->rows.end(), O.rows.begin(), O.rows.end());
     P->recs.insert(P->recs.end(), O.recs.begin(), O.recs.end());
     P->rec_elem.insert(P->rec_elem.end(), O.rec_elem.begin(), O.rec_elem.end());
-    P->parts.insert(P->parts.end(),
-                    {node_start, O.L, row_start, O.R, O.lds, rec_start, O.nelems, 0});
+    if (mode == 0) {
+      P->parts.insert(P->parts.end(),
+                      {node_start, O.L, row_start, O.R, O.lds, rec_start, O.nelems, 0});
+    } else {
+      // rows plans: [5] first row record, [6] row records, [7] first visit
+      const int32_t rr_start = (int32_t)(P->rowrecs.size() / kRowRecWords);
+      const size_t n = O.rowrecs.size() / kRowRecWords;
+      for (size_t k = 0; k < n; ++k) {
+        const uint32_t rr = O.rowrecs[k * kRowRecWords];
+        P->rowrecs.push_back((((rr >> 4) + (uint32_t)rec_start) << 4) | (rr & 15u));
+        P->rowrecs.insert(P->rowrecs.end(), O.rowrecs.begin() + k * kRowRecWords + 1,
+                          O.rowrecs.begin() + (k + 1) * kRowRecWords);
+      }
+      P->parts.insert(P->parts.end(), {node_start, O.L, row_start, O.R, O.lds, rr_start,
+                                       (int32_t)n, rec_start});
+      std::vector<uint32_t>().swap(O.rowrecs);
+    }
     max_slots = std::max(max_slots, O.lds);
     max_nodes = std::max(max_nodes, O.L);
     max_rows = std::max(max_rows, O.R);
@@ -420,7 +507,7 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
     std::vector<int32_t>().swap(O.nodemap);  // release as we go
     std::vector<uint32_t>().swap(O.recs);
   }
-  if (visits >= INT32_MAX) {
+  if (visits >= INT32_MAX || (mode == 1 && visits >= (1ll << 28))) {
     delete P;
     set_error("fused plan: more than 2^31-1 element visits");
     return FEMHIP_EOVERFLOW;
@@ -439,9 +526,43 @@ extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, i
   I.offbits = offbits;
   I.block = block;
   I.pruned = pruned ? 1 : 0;
+  I.nrowrecs = (int32_t)(P->rowrecs.size() / kRowRecWords);
   I.dup = E > 0 ? (double)visits / (double)E : 0.0;
   *info_out = I;
   *handle_out = P;
+  return FEMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int femhip_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, int64_t Nv,
+                                       const int32_t* t, const double* p,
+                                       const int32_t* indptr, const int32_t* indices,
+                                       int32_t slot_cap, int32_t node_cap, int32_t block,
+                                       int32_t order_mode, void** handle_out,
+                                       femhip_fused_plan_info_t* info_out) {
+  return build_impl(0, nverts, dim, E, Nv, t, p, indptr, indices, slot_cap, node_cap, block,
+                    order_mode, handle_out, info_out);
+}
+
+extern "C" int femhip_rows_fused_plan_build(int32_t nverts, int32_t dim, int64_t E, int64_t N,
+                                            const int32_t* t, const double* p,
+                                            const int32_t* indptr, const int32_t* indices,
+                                            int32_t slot_cap, int32_t node_cap, int32_t block,
+                                            void** handle_out,
+                                            femhip_fused_plan_info_t* info_out) {
+  return build_impl(1, nverts, dim, E, N, t, p, indptr, indices, slot_cap, node_cap, block, 1,
+                    handle_out, info_out);
+}
+
+extern "C" int femhip_rows_fused_plan_export(void* handle, int32_t* rowrecs) {
+  if (!handle) {
+    femhip::set_error("rows plan: NULL handle");
+    return FEMHIP_EINVAL;
+  }
+  Plan* P = static_cast<Plan*>(handle);
+  if (rowrecs && !P->rowrecs.empty())  // kRowRecWords words per record
+    std::memcpy(rowrecs, P->rowrecs.data(), P->rowrecs.size() * 4);
   return FEMHIP_OK;
 }
 

@@ -58,7 +58,7 @@ class FusedInfo(ctypes.Structure):
     _fields_ = [('nparts', _i32), ('max_recs', _i32), ('nrows', _i32), ('max_slots', _i32),
                 ('max_nodes', _i32), ('max_rows', _i32), ('rec_words', _i32),
                 ('idbits', _i32), ('offbits', _i32), ('block', _i32),
-                ('pruned', _i32), ('reserved1', _i32), ('nrecs', _i64), ('nlocal', _i64),
+                ('pruned', _i32), ('nrowrecs', _i32), ('nrecs', _i64), ('nlocal', _i64),
                 ('dup', _f64)]
 
 
@@ -124,6 +124,15 @@ SIGNATURES = {
                                              _i64, _p, _p, _p, ctypes.POINTER(FusedCheck),
                                              _i32, _p]),
     'femhip_fused_static_queue': (_i64, []),
+    'femhip_rows_fused_plan_build': (ctypes.c_int, [_i32, _i32, _i64, _i64, _p, _p, _p, _p,
+                                                    _i32, _i32, _i32,
+                                                    ctypes.POINTER(ctypes.c_void_p),
+                                                    ctypes.POINTER(FusedInfo)]),
+    'femhip_rows_fused_plan_export': (ctypes.c_int, [ctypes.c_void_p, _p]),
+    'femhip_rows_fused_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(FusedInfo)]),
+    'femhip_rows_fused_assemble': (ctypes.c_int, [ctypes.POINTER(FusedDev), _p, _p, _i32,
+                                                  ctypes.POINTER(Geom), ctypes.POINTER(Space),
+                                                  _p, _p, ctypes.POINTER(FusedCheck), _p]),
     'femhip_fused_adjoint_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(FusedInfo), _i32]),
     'femhip_fused_adjoint': (ctypes.c_int, [ctypes.POINTER(FusedDev), _p, _i32, _p, _i64, _p,
                                             _p, _p, _p, _p, _p, _p, _p]),

@@ -157,6 +157,8 @@ class DeviceBasis:
         self.last_verdict = None
         self.unfusable = {}      # plan serial -> why the fused plan was refused
         self.row_plan = None
+        self.rows_plans = {}     # plan serial -> RowsFusedPlan (P2 simplices)
+        self.rows_last = {}      # form -> the last plan its rows path used
 
     # -- coordinates ---------------------------------------------------------
     def set_coordinates(self, p):
@@ -359,6 +361,43 @@ class DeviceBasis:
             return None
         return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N), plan)
 
+    def rows_fused_try(self, name):
+        """The fused rows kernel (P2 simplices) on the last pattern validated
+        for ``name``, checked against it in the same pass: the CSR values
+        (DeviceCSR) when this input's exact zeros are the pattern's, else
+        None (verdict in ``last_verdict``)."""
+        plan = self.rows_last.get(name)
+        if plan is None:
+            return None
+        fp = self.rows_plans[plan.signature]
+        out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
+                          device=self.p.device)[:plan.nnz]
+        fp.assemble(name, self, out)
+        v = fp.verdict.cpu().tolist()      # synchronises
+        self.last_verdict = v
+        if v[0] != 1:
+            return None
+        return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N), plan)
+
+    def rows_plan_for(self, plan, name):
+        """Rows plan of a validated P2 pattern (built once per pattern);
+        None when the mesh cannot be encoded (the exact path stays)."""
+        from .fused import RowsFusedPlan
+        if plan.signature in self.unfusable:
+            return None
+        fp = self.rows_plans.get(plan.signature)
+        if fp is None:
+            torch.cuda.current_stream().synchronize()
+            try:
+                fp = RowsFusedPlan(self.basis, plan.indptr.cpu().numpy(),
+                                   plan.indices.cpu().numpy())
+            except (NotImplementedError, ValueError, OverflowError) as exc:
+                self.unfusable[plan.signature] = str(exc)
+                return None
+            self.rows_plans[plan.signature] = fp
+        self.rows_last[name] = plan
+        return fp
+
     def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, path='auto'):
         """CSR assembly of a built-in form.
 
@@ -386,9 +425,15 @@ class DeviceBasis:
         # comparison (same inputs, same zeros)
         scalar = (path != 'generic' and not fusable and not self.basis.elem.vector
                   and name in ('laplace', 'mass', 'kappa_laplace', 'coef_mass'))
+        rows = (path != 'generic' and kappa is None and fused.rows_eligible(self.basis, name))
         vkey = (name, lam, mu, None if kappa is None else tuple(kappa.shape), self.p_version)
         if fusable:
             A = self.fused_try(name, kappa)
+            if A is not None:
+                self.validated[vkey] = A.plan
+                return A
+        if rows:
+            A = self.rows_fused_try(name)
             if A is not None:
                 self.validated[vkey] = A.plan
                 return A
@@ -420,6 +465,8 @@ class DeviceBasis:
                     self.unfusable[plan.signature] = str(exc)
         elif mfma or scalar:
             self._validate(vkey, plan, kappa)
+            if rows:
+                self.rows_plan_for(plan, name)
         return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N), plan)
 
     def elasticity_mfma(self, lam, mu, plan, out, loc=None):
@@ -580,6 +627,43 @@ class FacetDeviceBasis(DeviceBasis):
             torch.cuda.current_stream().synchronize()
             self._normals = out
         return self._normals
+
+    def rows_fused_try(self, name):
+        """The fused rows kernel (P2 simplices) on the last pattern validated
+        for ``name``, checked against it in the same pass: the CSR values
+        (DeviceCSR) when this input's exact zeros are the pattern's, else
+        None (verdict in ``last_verdict``)."""
+        plan = self.rows_last.get(name)
+        if plan is None:
+            return None
+        fp = self.rows_plans[plan.signature]
+        out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
+                          device=self.p.device)[:plan.nnz]
+        fp.assemble(name, self, out)
+        v = fp.verdict.cpu().tolist()      # synchronises
+        self.last_verdict = v
+        if v[0] != 1:
+            return None
+        return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N), plan)
+
+    def rows_plan_for(self, plan, name):
+        """Rows plan of a validated P2 pattern (built once per pattern);
+        None when the mesh cannot be encoded (the exact path stays)."""
+        from .fused import RowsFusedPlan
+        if plan.signature in self.unfusable:
+            return None
+        fp = self.rows_plans.get(plan.signature)
+        if fp is None:
+            torch.cuda.current_stream().synchronize()
+            try:
+                fp = RowsFusedPlan(self.basis, plan.indptr.cpu().numpy(),
+                                   plan.indices.cpu().numpy())
+            except (NotImplementedError, ValueError, OverflowError) as exc:
+                self.unfusable[plan.signature] = str(exc)
+                return None
+            self.rows_plans[plan.signature] = fp
+        self.rows_last[name] = plan
+        return fp
 
     def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, path='auto'):
         """Exact path: local entries -> pattern (matched by exact-zero mask)

@@ -206,3 +206,111 @@ class FusedPlan:
             None if check is None else ctypes.byref(check), int(groups),
             _lib.stream_ptr()))
         return out
+
+
+# Rows plans (P2 simplices): one 512-thread workgroup per CU (the entry
+# arithmetic holds ~250 registers, two waves per SIMD), so a partition may
+# use most of the LDS: 16,000 slots x 9 B (accumulator, nonzero flag)
+ROWS_SLOT_CAP = int(os.environ.get('FEMHIP_ROWS_SLOT_CAP', 16000))
+ROWS_NODE_CAP = int(os.environ.get('FEMHIP_ROWS_NODE_CAP', 16384))
+_ROWS_FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS}
+ROW_RECORD_WORDS = 6   # (visit << 4) | DOF, then 2 x 10 seven-bit column offsets
+
+
+def rows_eligible(basis, name):
+    e = basis.elem
+    return (not e.vector and e.kind in ('tri', 'tet') and e.maxdeg == 2
+            and e.nbs == (6 if e.kind == 'tri' else 10) and name in _ROWS_FORM)
+
+
+def dof_positions(mesh, element_dofs, ndofs):
+    """A point per DOF for the partition (the centroid of an element holding
+    it; only the locality of the partition depends on it)."""
+    t = np.asarray(mesh.t)
+    c = np.asarray(mesh.p)[:, t].mean(axis=1)           # (dim, E)
+    pos = np.zeros((mesh.p.shape[0], ndofs))
+    ed = np.asarray(element_dofs)
+    for k in range(ed.shape[0]):
+        pos[:, ed[k]] = c
+    return pos
+
+
+class RowsFusedPlan:
+    """Device arrays of a rows plan (P2 simplices): partitions of the DOF
+    rows, element visits (local DOF ids) and one row record per (visit,
+    owned DOF); states per form and the verdict of the last call."""
+
+    def __init__(self, basis, indptr, indices, slot_cap=ROWS_SLOT_CAP, node_cap=ROWS_NODE_CAP):
+        lib = _lib.load()
+        ed = np.ascontiguousarray(basis.element_dofs, dtype=np.int32)
+        N = int(basis.N)
+        pos = np.ascontiguousarray(dof_positions(basis.mesh, ed, N), dtype=np.float64)
+        ip = np.ascontiguousarray(indptr, dtype=np.int32)
+        ix = np.ascontiguousarray(indices, dtype=np.int32)
+        handle = ctypes.c_void_p()
+        info = _lib.FusedInfo()
+        _lib.check(lib.femhip_rows_fused_plan_build(
+            ed.shape[0], pos.shape[0], ed.shape[1], N, ed.ctypes.data_as(ctypes.c_void_p),
+            pos.ctypes.data_as(ctypes.c_void_p), ip.ctypes.data_as(ctypes.c_void_p),
+            ix.ctypes.data_as(ctypes.c_void_p), int(slot_cap), int(node_cap), 512,
+            ctypes.byref(handle), ctypes.byref(info)))
+        try:
+            parts = np.empty(8 * info.nparts, np.int32)
+            nodemap = np.empty(info.nlocal, np.int32)
+            rows = np.empty(2 * info.nrows, np.int32)
+            recs = np.empty(info.rec_words * info.nrecs, np.uint32)
+            rec_elem = np.empty(info.nrecs, np.int32)
+            rowrecs = np.empty(ROW_RECORD_WORDS * info.nrowrecs, np.uint32)
+            arrs = (parts, nodemap, rows, recs, rec_elem)
+            _lib.check(lib.femhip_fused_plan_export(
+                handle, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs]))
+            _lib.check(lib.femhip_rows_fused_plan_export(
+                handle, rowrecs.ctypes.data_as(ctypes.c_void_p)))
+        finally:
+            lib.femhip_fused_plan_free(handle)
+        dev = torch.device('cuda')
+        self.info = info
+        self.nnz = int(indptr[-1])
+        self.parts = torch.from_numpy(parts).to(dev)
+        self.nodemap = torch.from_numpy(nodemap).to(dev)
+        self.rows = torch.from_numpy(rows).to(dev)
+        self.recs = torch.from_numpy(recs.view(np.int32)).to(dev)
+        self.rec_elem = torch.from_numpy(rec_elem).to(dev)
+        self.rowrecs = torch.from_numpy(rowrecs.view(np.int32)).to(dev)
+        self.dev = _lib.FusedDev(self.parts.data_ptr(), self.nodemap.data_ptr(),
+                                 self.rows.data_ptr(), self.recs.data_ptr(),
+                                 self.rec_elem.data_ptr(), self.info)
+        self.states = {}
+        self.verdict = torch.zeros(4, dtype=torch.int32, device=dev)
+        self.check = _lib.FusedCheck(None, None, None, 0, 0, 0, None, 0, None, 0,
+                                     self.verdict.data_ptr())
+
+    def state(self, name):
+        st = self.states.get(name)
+        if st is None:
+            st = torch.zeros(STATE_BYTES, dtype=torch.uint8, device=self.parts.device)
+            self.states[name] = st
+        return st
+
+    def lds_bytes(self):
+        return int(_lib.load().femhip_rows_fused_lds_bytes(ctypes.byref(self.info)))
+
+    def stats(self):
+        i = self.info
+        return dict(lds_bytes=self.lds_bytes(), parts=i.nparts, visits=int(i.nrecs),
+                    row_records=int(i.nrowrecs), dup=i.dup, max_slots=i.max_slots,
+                    max_nodes=i.max_nodes)
+
+    def assemble(self, name, d, out):
+        """out[nnz] <- CSR values of ``name`` on the basis ``d``
+        (DeviceBasis) for its current coordinates; the call is checked
+        against the plan's pattern, verdict {valid, 0, 0, flags} left in
+        ``self.verdict`` on the device."""
+        assert out.numel() == self.nnz and out.dtype == torch.float64
+        G = d.geometry()
+        _lib.check(_lib.load().femhip_rows_fused_assemble(
+            ctypes.byref(self.dev), _lib.ptr(self.rowrecs), _lib.ptr(self.state(name)),
+            _ROWS_FORM[name], ctypes.byref(d.geom), ctypes.byref(d.space), _lib.ptr(G),
+            _lib.ptr(out), ctypes.byref(self.check),
+            _lib.stream_ptr()))
+        return out

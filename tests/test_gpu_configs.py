@@ -54,6 +54,8 @@ def test_p2_tet_41_digest_first_and_later_calls():
     _check(laplace.assemble_device(b), dig['laplace'])
     print('later call', flush=True)
     _check(laplace.assemble_device(b), dig['laplace'])
+    d = b.device()
+    assert 'laplace' in d.rows_last and d.last_verdict[0] == 1   # fused rows kernel
 
 
 def test_p2_tri_257_digests_and_kappa_batch():
