@@ -592,6 +592,47 @@ def bench_p2_tet(n=41):
             "exact_path_ms": local_ms + sum_ms}
 
 
+def bench_laplace_mass(basis):
+    """Config 2a's workload on the 101^3 basis: laplace, mass and unit_load
+    in one fused pass (checked against the Laplace pattern), beside the
+    separate validated fused calls.  Algorithmic bytes: 16 B connectivity
+    per element + 24 B coordinates per node + 8 B per value of both CSR
+    matrices + 8 B per load entry."""
+    import torch
+    from femhip.fused import LaplaceMassPass
+    from femhip.models import unit_load
+    d = basis.device()
+    pair = LaplaceMassPass(basis)
+    stream = torch.cuda.current_stream()
+    L, M, f = pair.assemble()
+    ms_pair = time_stream(lambda: pair.assemble(L, M, f), 20, stream)
+    torch.cuda.synchronize()
+    verdict = pair.fp.verdict.cpu().tolist()
+    # the separate passes (fused, validated) and the load
+    pl, pm = pair.plan_l, pair.plan_m
+    fl, fm = d.fused_plans[pl.signature], d.fused_plans[pm.signature]
+    cl = fl.check_struct(d, pl.mask, pl.nzeros)
+    cm = fm.check_struct(d, pm.mask, pm.nzeros)
+    ol = torch.empty_like(L)
+    om = torch.empty_like(M)
+    ms_l = time_stream(lambda: fl.assemble('laplace', d.p, ol, check=cl), 20, stream)
+    ms_m = time_stream(lambda: fm.assemble('mass', d.p, om, check=cm), 20, stream)
+    ms_f = time_stream(lambda: unit_load.assemble_device(basis), 5, stream)
+    torch.cuda.synchronize()
+    err_l = float((L - ol).abs().max() / ol.abs().max())
+    err_m = float((M - om).abs().max() / om.abs().max())
+    E, N = d.E, d.N
+    alg = 16 * E + 24 * N + 8 * pl.nnz + 8 * pm.nnz + 8 * N
+    return {"elements": E, "nnz_laplace": pl.nnz, "nnz_mass": pm.nnz,
+            "pair_ms": ms_pair, "alg_bytes": alg, "alg_bytes_per_element": alg / E,
+            "frac_of_8TBps": alg / (ms_pair * 1e-3) / HBM_PEAK,
+            "elements_per_s": E / (ms_pair * 1e-3), "verdict": verdict,
+            "separate_laplace_ms": ms_l, "separate_mass_ms": ms_m,
+            "separate_unit_load_ms": ms_f, "separate_total_ms": ms_l + ms_m + ms_f,
+            "max_rel_diff_vs_separate": max(err_l, err_m),
+            "pair_plan": pair.fp.stats()}
+
+
 def bench_autotune(basis):
     """The fused kernel's autotuner on the 101^3 Laplace: every candidate
     launch configuration timed, the fastest installed on the basis (the
@@ -644,10 +685,6 @@ def bench_elasticity(n=41):
             "segment_sum_ms": ms_all - ms_local,
             "mfma_TFLOPs_executed": exec_flops / (ms_local * 1e-3) / 1e12,
             "mfma_util_of_78.6TF": exec_flops / (ms_local * 1e-3) / 78.6e12,
-            # the same local matrices as dense B^T D B products per point
-            # (81 x 6 strain-displacement B, 6 x 6 D, 27 points): 2,283,228
-            # flop per element
-            "BtDB_TFLOPs_equiv": E * 2283228 / (ms_local * 1e-3) / 1e12,
             "max_rel_err_vs_exact": err, "entries_differing_from_exact": ndiff}
 
 
@@ -667,7 +704,8 @@ def main():
     ap.add_argument('--extra', default=None,
                     help='comma-separated subset of the extra measurements '
                          '(changed, interpolate, facet, device_solve, batched_kappa, '
-                         'mesh_batch, adjoint, elasticity, p2_tet, autotune)')
+                         'mesh_batch, adjoint, elasticity, p2_tet, laplace_mass, '
+                         'autotune)')
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -800,6 +838,7 @@ def main():
             ('adjoint', "adjoint_kappa_tet65", bench_adjoint),
             ('elasticity', "elasticity_vhex2_40", bench_elasticity),
             ('p2_tet', "laplace_p2tet41", bench_p2_tet),
+            ('laplace_mass', "laplace_mass_load_tet101", lambda: bench_laplace_mass(basis)),
             ('autotune', "autotune_tet101", lambda: bench_autotune(basis)),
         )
         chosen = None if args.extra is None else set(args.extra.split(','))

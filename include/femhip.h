@@ -382,6 +382,22 @@ int femhip_fused_assemble(const femhip_fused_dev_t* plan, femhip_fused_state_t* 
                           const femhip_fused_check_t* check, int32_t groups,
                           femhip_stream_t s);
 
+/* Laplace and mass (and, with load != NULL, the unit load) of a P1 basis
+ * in one pass -- BilinearForm.assemble of laplace and of mass and
+ * LinearForm.assemble of unit_load (bilinear_form.py:130-148,
+ * linear_form.py:18-49) over one read of the records and coordinates.  The
+ * plan is built on the mass pattern (info.pruned == 0); lrows[nrows][2]
+ * gives, per plan row in the plan's row order, the row's start in the
+ * Laplace pattern and the mask of its slots the Laplace pattern lacks (bit
+ * k: slot k of the mass row), so laplace[] is written in the Laplace
+ * pattern and mass[] in the plan's.  check validates the Laplace zeros as
+ * for femhip_fused_assemble (the Laplace plan's zero mask). */
+int femhip_fused_laplace_mass(const femhip_fused_dev_t* plan, const int32_t* lrows,
+                              femhip_fused_state_t* state, int32_t dim, const double* p,
+                              int64_t nnodes, double* laplace, double* mass, double* load,
+                              const femhip_fused_check_t* check, int32_t groups,
+                              femhip_stream_t s);
+
 /* femhip_fused_assemble without any allocation, caller's or library's.  It
  * has no counterpart in the reference; it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving

@@ -47,6 +47,9 @@ struct FxArgs {
   const double* kappa;
   double* out;
   double* load;      // optional: unit-load vector, one entry per node
+  double* out2;      // Laplace + mass pass: the mass values (plan pattern)
+  const int32_t* lrows;  // Laplace + mass pass: per plan row, the Laplace row
+                         // start and the mask of its slots the Laplace pattern lacks
   femhip_fused_state_t* st;
   uint64_t* susp;    // suspect queue (first pass), or NULL
   int64_t susp_cap;
@@ -186,26 +189,38 @@ __device__ __forceinline__ void element_matrix(const double* X, const int (&lv)[
   }
 }
 
-// LDS layout: [max_slots accumulators | coordinates | load accumulators |
-//  2*NW reduction | row info | row starts | 4 ints]
-template <int D, bool LOAD, int NT>
+// Laplace and mass in one pass (an internal form of this kernel): the plan
+// is built on the mass pattern, the Laplace values are written compacted to
+// the Laplace pattern
+constexpr int kPairForm = 16;
+
+// LDS layout: [max_slots accumulators | (pair: max_slots mass accumulators) |
+//  coordinates | load accumulators | 2*NW reduction | row info | row starts |
+//  (pair: Laplace row starts and masks) | 4 ints]
+template <int D, bool LOAD, int NT, bool PAIR>
 struct Lds {
   unsigned long long* acc;
+  unsigned long long* accm;
   double* X;
   unsigned long long* bacc;
   double* red;
   uint32_t* rinfo;
   int32_t* rstart;
+  int32_t* lrow;
   int32_t* misc;
   __device__ Lds(double* smem, const FxArgs& a) {
     constexpr int NW = NT / 64;
+    const int slots = (a.max_slots + 1) & ~1;
     acc = reinterpret_cast<unsigned long long*>(smem);
-    X = smem + ((a.max_slots + 1) & ~1);
+    accm = acc + (PAIR ? slots : 0);
+    X = reinterpret_cast<double*>(accm + slots);
+    if (!PAIR) X = smem + slots;
     bacc = reinterpret_cast<unsigned long long*>(X + D * a.max_nodes);
     red = reinterpret_cast<double*>(bacc + (LOAD ? a.max_rows : 0));
     rinfo = reinterpret_cast<uint32_t*>(red + 2 * NW);
     rstart = reinterpret_cast<int32_t*>(rinfo + a.max_rows);
-    misc = rstart + a.max_rows;
+    lrow = rstart + a.max_rows;
+    misc = lrow + (PAIR ? 2 * a.max_rows : 0);
   }
 };
 
@@ -232,14 +247,19 @@ __global__ void __launch_bounds__(NT)
     __attribute__((amdgpu_waves_per_eu(LOAD ? 4 : (NT == 1024 ? 8 : 6))))
 p1_fused_kernel(FxArgs a) {
   constexpr int NV = D + 1;
-  constexpr bool ROWSUM = FORM != FEMHIP_MASS;
+  constexpr bool PAIR = FORM == kPairForm;
+  constexpr int KFORM = PAIR ? FEMHIP_LAPLACE : FORM;   // the form of acc
+  constexpr bool ROWSUM = KFORM != FEMHIP_MASS;
+  constexpr bool SECOND = LOAD || PAIR;   // a second exponent (load and/or mass)
   constexpr double LOADC = (D == 3) ? 1.0 / 4.0 : 1.0 / 3.0;  // int phi = vol / (d+1)
+  // P1 mass off the diagonal: |det| / (d! (d+1)(d+2)) = vol / ((d+1)(d+2))
+  constexpr double MASSC = (D == 3) ? 1.0 / 20.0 : 1.0 / 12.0;
   extern __shared__ double smem[];
-  Lds<D, LOAD, NT> L(smem, a);
+  Lds<D, LOAD, NT, PAIR> L(smem, a);
   const int tid = threadIdx.x;
   femhip_fused_state_t* st = a.st;
   const bool first = a.mode == 0;
-  const FxPass pass = fx_begin<LOAD>(st, a.mode);
+  const FxPass pass = fx_begin<SECOND>(st, a.mode);
   if (!pass.run) return;  // uniform
   const int s = pass.s, sv = pass.sv;
   const bool scatter = pass.scatter;
@@ -278,9 +298,17 @@ p1_fused_kernel(FxArgs a) {
       L.rstart[r] = rw.x;
       L.rinfo[r] = (uint32_t)rw.y;
       if constexpr (LOAD) L.bacc[r] = 0ull;
+      if constexpr (PAIR) {
+        const int2 lr = reinterpret_cast<const int2*>(a.lrows)[row_start + r];
+        L.lrow[2 * r] = lr.x;
+        L.lrow[2 * r + 1] = lr.y;
+      }
     }
     if (scatter)
-      for (int i = tid; i < nslots; i += NT) L.acc[i] = 0ull;
+      for (int i = tid; i < nslots; i += NT) {
+        L.acc[i] = 0ull;
+        if constexpr (PAIR) L.accm[i] = 0ull;
+      }
     if (nrec == 0) {
       __syncthreads();
       if (!scatter) continue;
@@ -338,10 +366,10 @@ p1_fused_kernel(FxArgs a) {
       uint32_t rinfo[NV];
 #pragma unroll
       for (int i = 0; i < NV; ++i) rinfo[i] = L.rinfo[lv[i] < R ? lv[i] : 0];
-      Elem<D, FORM> el;
-      element_matrix<D, FORM>(L.X, lv, ch.coef, suspects, el);
+      Elem<D, KFORM> el;
+      element_matrix<D, KFORM>(L.X, lv, ch.coef, suspects, el);
       uint32_t mask = 0;
-      if constexpr (FORM == FEMHIP_MASS) {
+      if constexpr (KFORM == FEMHIP_MASS) {
         mk = max_abs(mk, el.K[0][0]);
         bad = bad || !(el.tau < INFINITY);
       } else {
@@ -368,7 +396,7 @@ p1_fused_kernel(FxArgs a) {
           }
         }
       }
-      if constexpr (LOAD) mv = fmax(mv, el.vol * LOADC);
+      if constexpr (SECOND) mv = fmax(mv, el.vol * (LOAD ? LOADC : 2.0 * MASSC));
       if (!scatter) return mask;
       // each off-diagonal pair converted once; per owned row its entries
       unsigned long long F[NV][NV];
@@ -379,6 +407,12 @@ p1_fused_kernel(FxArgs a) {
           F[i][j] = F[j][i] = to_fixed(el.K[i][j], sck);
       unsigned long long fl = 0;
       if constexpr (LOAD) fl = to_fixed(el.vol * LOADC, scv);
+      unsigned long long fm = 0, fmd = 0;   // mass off / on the diagonal
+      if constexpr (PAIR) {
+        const double m = el.vol * MASSC;
+        fm = to_fixed(m, scv);
+        fmd = to_fixed(2.0 * m, scv);
+      }
       auto scatter_row = [&](auto ic) {
         constexpr int i = decltype(ic)::value;
         if (lv[i] >= R) return;  // halo vertex: its row belongs elsewhere
@@ -394,6 +428,12 @@ p1_fused_kernel(FxArgs a) {
           lds_add(row + offs[q], F[i][j]);
           ++q;
         }
+        if constexpr (PAIR) {
+          unsigned long long* mrow = L.accm + (rinfo[i] & 0xffffu);
+          lds_add(mrow + ((rinfo[i] >> 16) & 0xffu), fmd);
+#pragma unroll
+          for (int k = 0; k < NV - 1; ++k) lds_add(mrow + offs[k], fm);
+        }
         if constexpr (LOAD) lds_add(L.bacc + lv[i], fl);
       };
       scatter_row(std::integral_constant<int, 0>{});
@@ -404,7 +444,7 @@ p1_fused_kernel(FxArgs a) {
     };
     auto run = [&](const Chunk& ch, int c) {
       const uint32_t mask = element(ch, c);
-      if constexpr (FORM != FEMHIP_MASS)
+      if constexpr (KFORM != FEMHIP_MASS)
         if (mask) push_suspects<NV>(a, mask, a.rec_elem[rec_start + c * NT + tid]);
     };
     for (int c = 0; c < nchunks; c += 2) {
@@ -438,12 +478,32 @@ p1_fused_kernel(FxArgs a) {
         for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o, 16);
         dsum = -t;
       }
-      if (lane < len)
-        __builtin_nontemporal_store(from_fixed(ROWSUM && lane == dg ? dsum : v0, inv),
-                                    a.out + gstart + lane);
-      if (lane + 16 < len)
-        __builtin_nontemporal_store(from_fixed(ROWSUM && lane + 16 == dg ? dsum : v1, inv),
-                                    a.out + gstart + lane + 16);
+      if constexpr (PAIR) {
+        // Laplace: the slots the Laplace pattern holds, compacted to its row;
+        // mass: every slot of the plan's (mass) row
+        const int64_t lstart = L.lrow[2 * r];
+        const uint32_t lmask = (uint32_t)L.lrow[2 * r + 1];
+        const double invm = ldexp(1.0, -sv);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int k = lane + 16 * h;
+          if (k >= len) continue;
+          const long long vl = h ? v1 : v0;
+          if (!((lmask >> k) & 1u))
+            __builtin_nontemporal_store(
+                from_fixed(k == dg ? dsum : vl, inv),
+                a.out + lstart + (k - __popc(lmask & ((1u << k) - 1u))));
+          __builtin_nontemporal_store(from_fixed((long long)L.accm[base + k], invm),
+                                      a.out2 + gstart + k);
+        }
+      } else {
+        if (lane < len)
+          __builtin_nontemporal_store(from_fixed(ROWSUM && lane == dg ? dsum : v0, inv),
+                                      a.out + gstart + lane);
+        if (lane + 16 < len)
+          __builtin_nontemporal_store(from_fixed(ROWSUM && lane + 16 == dg ? dsum : v1, inv),
+                                      a.out + gstart + lane + 16);
+      }
     }
     if constexpr (LOAD) {
       const double invv = ldexp(1.0, -sv);
@@ -452,7 +512,7 @@ p1_fused_kernel(FxArgs a) {
     }
   }  // partitions
 
-  if (first) fx_end<NT, LOAD>(st, mk, mv, bad, suspects, L.red);
+  if (first) fx_end<NT, SECOND>(st, mk, mv, bad, suspects, L.red);
 }
 
 template <int D, int RW, int IDB, int OFB, int FORM, bool LOAD>
@@ -474,6 +534,9 @@ int launch_form(const FxArgs& a, int form, int nt, size_t lds, unsigned grid, hi
       break;
     case FEMHIP_KAPPA_LAPLACE:
       launch_one<D, RW, IDB, OFB, FEMHIP_KAPPA_LAPLACE, LOAD>(a, nt, lds, grid, s);
+      break;
+    case kPairForm:
+      launch_one<D, RW, IDB, OFB, kPairForm, LOAD>(a, nt, lds, grid, s);
       break;
     default:
       set_error("fused path: laplace, mass or kappa_laplace only");
@@ -513,12 +576,18 @@ extern "C" size_t femhip_fused_lds_bytes(const femhip_fused_plan_info_t* I, int3
          8 * 2 * (I->block / 64) + 8 * (size_t)I->max_rows + 16;
 }
 
-extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan,
-                                     femhip_fused_state_t* state, int32_t form, int32_t dim,
-                                     const double* p, int64_t nnodes, const double* kappa,
-                                     double* data, double* load,
-                                     const femhip_fused_check_t* check, int32_t groups,
-                                     femhip_stream_t s) {
+namespace {
+
+size_t fused_lds(const femhip_fused_plan_info_t& I, int dim, bool load, bool pair) {
+  return femhip_fused_lds_bytes(&I, dim, load ? 1 : 0) +
+         (pair ? 8 * (((size_t)I.max_slots + 1) & ~(size_t)1) + 8 * (size_t)I.max_rows : 0);
+}
+
+// both entry points: checks, the two launches, the validation
+int fused_run(const femhip_fused_dev_t* plan, femhip_fused_state_t* state, int form, int dim,
+              const double* p, int64_t nnodes, const double* kappa, double* data,
+              double* load, double* data2, const int32_t* lrows,
+              const femhip_fused_check_t* check, int32_t groups, femhip_stream_t s) {
   FEMHIP_REQUIRE(plan != nullptr && state != nullptr && p != nullptr && data != nullptr,
                  FEMHIP_EINVAL, "fused: NULL argument");
   FEMHIP_REQUIRE(reinterpret_cast<uintptr_t>(state) % 128 == 0, FEMHIP_EINVAL,
@@ -532,6 +601,7 @@ extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan,
   FEMHIP_REQUIRE(check == nullptr || (check->queue && check->capacity > 0 && check->verdict),
                  FEMHIP_EINVAL, "fused: check needs a queue and a verdict");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  const bool pair = form == kPairForm;
   if (I.nparts == 0) {
     if (check) return fused_check_empty(check, state, st);
     return FEMHIP_OK;
@@ -540,10 +610,10 @@ extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan,
   // `groups` workgroups walking the partitions
   const unsigned grid = (unsigned)(groups > 0 && groups < I.nparts ? groups : I.nparts);
   FxArgs a{plan->parts, plan->nodemap, plan->rows, plan->recs, plan->rec_elem, p, kappa,
-           data, load, state, check ? check->queue : nullptr,
+           data, load, data2, lrows, state, check ? check->queue : nullptr,
            check ? check->capacity : 0, nnodes, I.nparts, I.max_slots, I.max_nodes,
            I.max_rows, 0};
-  const size_t lds = femhip_fused_lds_bytes(&I, dim, load != nullptr);
+  const size_t lds = fused_lds(I, dim, load != nullptr, pair);
   FEMHIP_REQUIRE(lds <= 160 * 1024, FEMHIP_EINVAL, "fused: plan exceeds LDS");
   // first pass, then the repeat launch (a pass only when the first one
   // used other exponents)
@@ -557,8 +627,38 @@ extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan,
     if (rc != FEMHIP_OK) return rc;
   }
 
-  if (check) return fused_check_run(check, state, form, dim, p, nnodes, kappa, st);
+  if (check)
+    return fused_check_run(check, state, pair ? FEMHIP_LAPLACE : form, dim, p, nnodes, kappa,
+                           st);
   return FEMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int femhip_fused_assemble(const femhip_fused_dev_t* plan,
+                                     femhip_fused_state_t* state, int32_t form, int32_t dim,
+                                     const double* p, int64_t nnodes, const double* kappa,
+                                     double* data, double* load,
+                                     const femhip_fused_check_t* check, int32_t groups,
+                                     femhip_stream_t s) {
+  FEMHIP_REQUIRE(form == FEMHIP_LAPLACE || form == FEMHIP_MASS || form == FEMHIP_KAPPA_LAPLACE,
+                 FEMHIP_EUNSUPPORTED, "fused path: laplace, mass or kappa_laplace only");
+  return fused_run(plan, state, form, dim, p, nnodes, kappa, data, load, nullptr, nullptr,
+                   check, groups, s);
+}
+
+extern "C" int femhip_fused_laplace_mass(const femhip_fused_dev_t* plan, const int32_t* lrows,
+                                         femhip_fused_state_t* state, int32_t dim,
+                                         const double* p, int64_t nnodes, double* laplace,
+                                         double* mass, double* load,
+                                         const femhip_fused_check_t* check, int32_t groups,
+                                         femhip_stream_t s) {
+  FEMHIP_REQUIRE(lrows != nullptr && mass != nullptr, FEMHIP_EINVAL,
+                 "fused laplace + mass: NULL argument");
+  FEMHIP_REQUIRE(plan == nullptr || plan->info.pruned == 0, FEMHIP_EUNSUPPORTED,
+                 "fused laplace + mass: the plan must be built on the mass pattern");
+  return fused_run(plan, state, kPairForm, dim, p, nnodes, nullptr, laplace, load, mass, lrows,
+                   check, groups, s);
 }
 
 

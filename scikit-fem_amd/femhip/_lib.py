@@ -124,6 +124,9 @@ SIGNATURES = {
                                              _i64, _p, _p, _p, ctypes.POINTER(FusedCheck),
                                              _i32, _p]),
     'femhip_fused_static_queue': (_i64, []),
+    'femhip_fused_laplace_mass': (ctypes.c_int, [ctypes.POINTER(FusedDev), _p, _p, _i32, _p,
+                                                 _i64, _p, _p, _p, ctypes.POINTER(FusedCheck),
+                                                 _i32, _p]),
     'femhip_rows_fused_plan_build': (ctypes.c_int, [_i32, _i32, _i64, _i64, _p, _p, _p, _p,
                                                     _i32, _i32, _i32,
                                                     ctypes.POINTER(ctypes.c_void_p),

@@ -314,3 +314,81 @@ class RowsFusedPlan:
             _lib.ptr(out), ctypes.byref(self.check),
             _lib.stream_ptr()))
         return out
+
+
+# Laplace + mass (+ unit load) in one pass: two LDS accumulator sets, so a
+# smaller partition keeps two workgroups per CU
+PAIR_SLOT_CAP = int(os.environ.get('FEMHIP_PAIR_SLOT_CAP', 3072))
+PAIR_NODE_CAP = int(os.environ.get('FEMHIP_PAIR_NODE_CAP', 640))
+
+
+def laplace_rows(plan_rows, mass_indptr, mass_indices, lap_indptr, lap_indices):
+    """Per plan row (the plan's row order: its CSR row starts in the mass
+    pattern), the row's start in the Laplace pattern and the mask of the
+    mass row's slots the Laplace pattern lacks (bit k: slot k)."""
+    n = len(mass_indptr) - 1
+    mi = np.asarray(mass_indptr, dtype=np.int64)
+    li = np.asarray(lap_indptr, dtype=np.int64)
+    rows_m = np.repeat(np.arange(n, dtype=np.int64), np.diff(mi))
+    rows_l = np.repeat(np.arange(n, dtype=np.int64), np.diff(li))
+    keys_m = rows_m * n + np.asarray(mass_indices, dtype=np.int64)
+    keys_l = rows_l * n + np.asarray(lap_indices, dtype=np.int64)
+    absent = ~np.isin(keys_m, keys_l)
+    if not np.isin(keys_l, keys_m).all():
+        raise ValueError("the Laplace pattern is not contained in the mass pattern")
+    k = np.arange(len(keys_m), dtype=np.int64) - mi[rows_m]
+    bits = np.zeros(n, dtype=np.int64)
+    np.add.at(bits, rows_m[absent], np.left_shift(1, k[absent]))
+    starts = np.asarray(plan_rows, dtype=np.int64).reshape(-1, 2)[:, 0]
+    g = np.searchsorted(mi[:-1], starts, side='right') - 1
+    # an empty row shares its start with the next one; take the row whose
+    # start it is and that has slots
+    assert np.array_equal(mi[g], starts)
+    out = np.empty((len(starts), 2), dtype=np.int64)
+    out[:, 0] = li[g]
+    out[:, 1] = bits[g]
+    return out.astype(np.uint32).view(np.int32).reshape(-1)
+
+
+class LaplaceMassPass:
+    """``laplace``, ``mass`` and ``unit_load`` of a P1 basis in one fused
+    pass over the records and coordinates: the plan is built on the mass
+    pattern, the Laplace values are written in the Laplace pattern; every
+    call is checked against the Laplace pattern's zero set."""
+
+    def __init__(self, basis, slot_cap=PAIR_SLOT_CAP, node_cap=PAIR_NODE_CAP):
+        from .models import laplace, mass
+        d = basis.device()
+        if not (eligible(basis, 'laplace') and eligible(basis, 'mass')):
+            raise NotImplementedError("the fused pair needs P1 triangles or tetrahedra")
+        laplace.assemble_device(basis)       # validate both patterns (exact path)
+        mass.assemble_device(basis)
+        self.d = d
+        self.plan_l = d.fused_last['laplace']
+        self.plan_m = d.fused_last['mass']
+        mi, mx = self.plan_m.indptr.cpu().numpy(), self.plan_m.indices.cpu().numpy()
+        li, lx = self.plan_l.indptr.cpu().numpy(), self.plan_l.indices.cpu().numpy()
+        self.fp = FusedPlan(basis.mesh, mi, mx, slot_cap, node_cap)
+        if self.fp.info.pruned:
+            raise NotImplementedError("the mass pattern lacks an element pair")
+        lr = laplace_rows(self.fp.host['rows'], mi, mx, li, lx)
+        self.lrows = torch.from_numpy(lr).to(self.fp.parts.device)
+        self.check = self.fp.check_struct(d, self.plan_l.mask, self.plan_l.nzeros)
+
+    def assemble(self, laplace_out=None, mass_out=None, load_out=None, check=True):
+        """(Laplace values, mass values, load vector) on the device; with
+        ``check`` the verdict of the Laplace zeros is in ``self.fp.verdict``."""
+        d, fp = self.d, self.fp
+        dev = d.p.device
+        if laplace_out is None:
+            laplace_out = torch.empty(self.plan_l.nnz, dtype=torch.float64, device=dev)
+        if mass_out is None:
+            mass_out = torch.empty(self.plan_m.nnz, dtype=torch.float64, device=dev)
+        if load_out is None:
+            load_out = torch.empty(d.p.shape[1], dtype=torch.float64, device=dev)
+        _lib.check(_lib.load().femhip_fused_laplace_mass(
+            ctypes.byref(fp.dev), _lib.ptr(self.lrows), _lib.ptr(fp.state('pair')), d.dim,
+            _lib.ptr(d.p), d.p.shape[1], _lib.ptr(laplace_out), _lib.ptr(mass_out),
+            _lib.ptr(load_out), ctypes.byref(self.check) if check else None, fp.groups,
+            _lib.stream_ptr()))
+        return laplace_out, mass_out, load_out

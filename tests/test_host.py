@@ -165,3 +165,17 @@ def test_no_cpu_fallback_without_gpu():
     b = femhip.Basis(femhip.MeshTri(), femhip.ElementTriP1())
     with pytest.raises(RuntimeError):
         laplace.assemble(b)
+
+
+def test_laplace_rows_masks_the_slots_the_laplace_pattern_lacks():
+    """Per plan row: the Laplace row start and a bit per mass slot absent
+    from the Laplace row (the fused Laplace + mass pass compacts with it)."""
+    from femhip.fused import laplace_rows
+    mi = np.array([0, 3, 6, 8])
+    mx = np.array([0, 1, 2, 0, 1, 2, 1, 2])
+    li = np.array([0, 2, 5, 7])
+    lx = np.array([0, 2, 0, 1, 2, 1, 2])          # row 0 lacks column 1
+    plan_rows = np.array([3, 0, 0, 0, 6, 0], dtype=np.int32)   # rows 1, 0, 2
+    out = laplace_rows(plan_rows, mi, mx, li, lx).reshape(-1, 2)
+    assert out[:, 0].tolist() == [2, 0, 5]
+    assert out[:, 1].tolist() == [0, 0b010, 0]
