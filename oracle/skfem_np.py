@@ -724,13 +724,25 @@ def element_dofs(mesh, elem):
 # ---------------------------------------------------------------------------
 class CellBasis:
 
-    def __init__(self, mesh, elem, intorder=None):
+    def __init__(self, mesh, elem, intorder=None, elements=None):
         self.mesh, self.elem = mesh, elem
-        self.mapping = (MappingAffine(mesh) if mesh.affine
-                        else MappingIsoparametric(mesh))
         self.element_dofs, self.N = element_dofs(mesh, elem)
+        # elements (index array): cell_basis.py:94-99 -- the mapping and the
+        # basis are evaluated at mesh.t[:, tind] only and element_dofs is
+        # dofs.element_dofs[:, tind] (abstract_basis.py element_dofs); the
+        # affine/isoparametric mapping of an element depends only on its own
+        # vertices, so it is evaluated on the selected columns of t
+        self.tind = None if elements is None else np.asarray(elements)
+        mmesh = mesh
+        if self.tind is not None:
+            mmesh = object.__new__(type(mesh))
+            mmesh.p = mesh.p
+            mmesh.t = np.ascontiguousarray(mesh.t[:, self.tind])
+            self.element_dofs = np.ascontiguousarray(self.element_dofs[:, self.tind])
+        self.mapping = (MappingAffine(mmesh) if mesh.affine
+                        else MappingIsoparametric(mmesh))
         self.Nbfun = self.element_dofs.shape[0]
-        self.nelems = mesh.nelements
+        self.nelems = mmesh.nelements
         order = intorder if intorder is not None else 2 * elem.maxdeg
         self.X, self.W = {'tri': quadrature_tri, 'tet': quadrature_tet,
                           'hex': quadrature_hex}[mesh.kind](order)
@@ -1000,9 +1012,11 @@ def enforce(A, b=None, x=None, D=None, diag=1.):
 
 
 def interpolate(basis, w):
-    """``AbstractBasis.interpolate`` (``abstract_basis.py:271-322``) for a
-    scalar element: (value (E, Q), grad (dim, E, Q)), one einsum term per
-    local basis function after the ``0. * (...)`` start."""
+    """``AbstractBasis.interpolate`` (``abstract_basis.py:271-322``):
+    (value, grad), one einsum term per local basis function after the
+    ``0. * (...)`` start; (E, Q) and (dim, E, Q) for a scalar element,
+    (dim, E, Q) and (dim, dim, E, Q) for ElementVector (its functions are
+    zero outside their own component, ``element_vector.py:36-48``)."""
     dofs = basis.element_dofs
     fields = []
     for n in range(2):

@@ -33,7 +33,7 @@ class _Ctx:
         self.plan = d.build_plan(None)            # connectivity pattern
         indptr = self.plan.indptr.cpu().numpy()
         indices = self.plan.indices.cpu().numpy()
-        self.fp = _fused.FusedPlan(basis.mesh, indptr, indices)
+        self.fp = _fused.FusedPlan(basis.cell_mesh, indptr, indices)
         if self.fp.info.pruned:
             raise RuntimeError("the connectivity pattern lacks an element pair")
         n = len(indptr) - 1

@@ -116,7 +116,7 @@ def tune_fused(basis, form='laplace', candidates=None, steps=20, cache=None):
     for cfg in (candidates or CANDIDATES):
         sc, nc, block, order, groups = cfg
         try:
-            fp = _fused.FusedPlan(basis.mesh, ip, ix, sc, nc, block=block, order=order)
+            fp = _fused.FusedPlan(basis.cell_mesh, ip, ix, sc, nc, block=block, order=order)
         except (NotImplementedError, ValueError):   # caps the mesh cannot satisfy
             continue
         if fp.lds_bytes() > 160 * 1024:
@@ -138,5 +138,5 @@ def tune_fused(basis, form='laplace', candidates=None, steps=20, cache=None):
     rec = {'best': list(best[0]), 'ms': best[1], 'trials': trials}
     path = _cache_path(cache)
     if path:
-        _store(path, plan_key(d.kind, form, basis.mesh.t, ip, ix), rec)
+        _store(path, plan_key(d.kind, form, basis.cell_mesh.t, ip, ix), rec)
     return rec

@@ -44,6 +44,50 @@ class DiscreteField:
         return self.value[key]
 
 
+def normalize_elements(mesh, elements):
+    """Element indices from a criterion (``Mesh.normalize_elements``,
+    ``mesh.py:1331-1360``): True (all), an index, an index array (taken as
+    given), a boolean mask over the elements, a function of the element
+    centroids returning such a mask, or a list/tuple/set of these (their
+    sorted union)."""
+    if isinstance(elements, bool) and elements:
+        return np.arange(mesh.nelements, dtype=np.int32)
+    if isinstance(elements, (int, np.integer)) and not isinstance(elements, bool):
+        return np.array([elements])
+    if isinstance(elements, np.ndarray):
+        if elements.dtype == bool:
+            if elements.shape != (mesh.nelements,):
+                raise ValueError("element mask of wrong size")
+            return np.flatnonzero(elements)
+        return elements
+    if callable(elements):
+        c = np.asarray(mesh.p)[:, mesh.t].mean(axis=1)
+        return np.flatnonzero(np.asarray(elements(c), dtype=bool))
+    if isinstance(elements, (tuple, list, set)):
+        return np.unique(np.concatenate([normalize_elements(mesh, e) for e in elements]))
+    raise NotImplementedError("element criterion %r" % (type(elements),))
+
+
+class ElementSubset:
+    """The elements a ``CellBasis(elements=...)`` covers, as the device
+    layer sees a mesh: the same nodes and coordinates, the subset's
+    connectivity (other attributes are the full mesh's)."""
+
+    def __init__(self, mesh, tind):
+        self.parent = mesh
+        self.tind = np.asarray(tind)
+        self.kind, self.dim, self.nverts = mesh.kind, mesh.dim, mesh.nverts
+        self.p = mesh.p
+        self.t = np.ascontiguousarray(np.asarray(mesh.t)[:, self.tind])
+
+    @property
+    def nelements(self):
+        return self.t.shape[1]
+
+    def __getattr__(self, name):
+        return getattr(self.parent, name)
+
+
 class CellBasis:
 
     def __init__(self, mesh, elem, mapping=None, intorder=None,
@@ -53,8 +97,6 @@ class CellBasis:
             raise ValueError("Incompatible Mesh and Element.")
         if mapping is not None:
             raise NotImplementedError("custom mappings are not accelerated")
-        if elements is not None:
-            raise NotImplementedError("element subsets are not accelerated")
         self.mesh = mesh
         self.elem = elem
         self.dofs = Dofs(mesh, elem) if dofs is None else dofs
@@ -64,9 +106,18 @@ class CellBasis:
         else:
             order = intorder if intorder is not None else 2 * elem.maxdeg
             self.X, self.W = get_quadrature(mesh.kind, order)
-        self.tind = None
-        self.nelems = mesh.nelements
+        # elements= (cell_basis.py:94-99): the DOF numbering stays the full
+        # mesh's, the assembly runs over the selected elements only
+        if elements is None:
+            self.tind = None
+            self.cell_mesh = mesh
+        else:
+            self.tind = normalize_elements(mesh, elements)
+            self.cell_mesh = ElementSubset(mesh, self.tind)
+        self._ed_cache = (None, None)
+        self.nelems = self.cell_mesh.nelements
         self._device = None
+        self._scalar = None
 
     @property
     def N(self):
@@ -78,7 +129,12 @@ class CellBasis:
 
     @property
     def element_dofs(self):
-        return self.dofs.element_dofs
+        ed = self.dofs.element_dofs
+        if self.tind is None:
+            return ed
+        if self._ed_cache[0] is not ed:
+            self._ed_cache = (ed, np.ascontiguousarray(ed[:, self.tind]))
+        return self._ed_cache[1]
 
     @property
     def nodal_dofs(self):
@@ -101,10 +157,10 @@ class CellBasis:
         ``DiscreteField`` of torch tensors (value (E, Q), grad (dim, E, Q))."""
         import torch
         from . import _lib
-        if self.elem.vector:
-            raise NotImplementedError("interpolate: scalar elements only")
         if w.shape[0] != self.N:
             raise ValueError("Input array has wrong size.")
+        if self.elem.vector:
+            return self._interpolate_vector(w, grad)
         d = self.device()
         wt = torch.as_tensor(w, dtype=torch.float64, device=d.p.device).contiguous()
         E, Q, D = d.E, d.Q, d.dim
@@ -116,6 +172,23 @@ class CellBasis:
             _lib.ptr(value), _lib.ptr(gr), _lib.stream_ptr()))
         return DiscreteField(value, gr)
 
+    def _interpolate_vector(self, w, grad):
+        """ElementVector: the reference sums the interleaved functions in
+        local order, each adding to its own component only
+        (abstract_basis.py:289-316, element_vector.py:36-48), i.e. every
+        component is the scalar interpolation of the DOFs c, c + dim, ...;
+        value (dim, E, Q), grad (dim, dim, E, Q)."""
+        import torch
+        if self._scalar is None:
+            self._scalar = CellBasis(self.mesh, self.elem.elem, quadrature=(self.X, self.W),
+                                     elements=self.tind)
+        dim = self.elem.dim
+        w = np.asarray(w, dtype=np.float64) if not isinstance(w, torch.Tensor) else w
+        fs = [self._scalar.interpolate_device(w[c::dim], grad=grad) for c in range(dim)]
+        value = torch.stack([f.value for f in fs])
+        gr = torch.stack([f.grad for f in fs]) if grad else None
+        return DiscreteField(value, gr)
+
     def interpolate(self, w):
         """A DOF vector at the quadrature points (``abstract_basis.py:271-322``):
         ``DiscreteField`` with ``value`` (E, Q) and ``grad`` (dim, E, Q), equal
@@ -124,7 +197,7 @@ class CellBasis:
         return DiscreteField(f.value.cpu().numpy(), f.grad.cpu().numpy())
 
     def with_element(self, elem):
-        return type(self)(self.mesh, elem, quadrature=(self.X, self.W))
+        return type(self)(self.mesh, elem, quadrature=(self.X, self.W), elements=self.tind)
 
     def global_coordinates(self):
         """Coordinates of the quadrature points, ``DiscreteField`` with

@@ -104,7 +104,7 @@ class MeshBatch:
             plan = next(p for k, p in dev.validated.items() if k[0] == 'laplace')
             # the block-partition variant: plans of any number of meshes
             # concatenate into one launch
-            fp = _fused.FusedPlan(b.mesh, plan.indptr.cpu().numpy(),
+            fp = _fused.FusedPlan(b.cell_mesh, plan.indptr.cpu().numpy(),
                                   plan.indices.cpu().numpy(), *(caps or ()))
             plans.append((plan, fp))
             ps.append(dev.p)

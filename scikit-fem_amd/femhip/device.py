@@ -121,7 +121,7 @@ class Plan:
 class DeviceBasis:
 
     def __init__(self, basis):
-        mesh, elem = basis.mesh, basis.elem
+        mesh, elem = basis.cell_mesh, basis.elem
         check_host_inputs(mesh.p, mesh.t, basis.element_dofs, basis.N)
         lib = _lib.load()
         self.lib = lib
@@ -286,17 +286,17 @@ class DeviceBasis:
             ip, ix = plan.indptr.cpu().numpy(), plan.indices.cpu().numpy()
             cfg = None
             if autotune._cache_path(None):
-                cfg = autotune.lookup(autotune.plan_key(self.kind, form, self.basis.mesh.t,
+                cfg = autotune.lookup(autotune.plan_key(self.kind, form, self.basis.cell_mesh.t,
                                                         ip, ix))
             if cfg is not None:
                 try:
-                    fp = FusedPlan(self.basis.mesh, ip, ix, cfg[0], cfg[1], block=cfg[2],
+                    fp = FusedPlan(self.basis.cell_mesh, ip, ix, cfg[0], cfg[1], block=cfg[2],
                                    order=cfg[3])
                     fp.groups = cfg[4]
                 except (NotImplementedError, ValueError):
                     fp = None    # a cached configuration this build refuses
             if fp is None:
-                fp = FusedPlan(self.basis.mesh, ip, ix)
+                fp = FusedPlan(self.basis.cell_mesh, ip, ix)
             self.fused_plans[plan.signature] = fp
         return fp
 
@@ -627,43 +627,6 @@ class FacetDeviceBasis(DeviceBasis):
             torch.cuda.current_stream().synchronize()
             self._normals = out
         return self._normals
-
-    def rows_fused_try(self, name):
-        """The fused rows kernel (P2 simplices) on the last pattern validated
-        for ``name``, checked against it in the same pass: the CSR values
-        (DeviceCSR) when this input's exact zeros are the pattern's, else
-        None (verdict in ``last_verdict``)."""
-        plan = self.rows_last.get(name)
-        if plan is None:
-            return None
-        fp = self.rows_plans[plan.signature]
-        out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
-                          device=self.p.device)[:plan.nnz]
-        fp.assemble(name, self, out)
-        v = fp.verdict.cpu().tolist()      # synchronises
-        self.last_verdict = v
-        if v[0] != 1:
-            return None
-        return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N), plan)
-
-    def rows_plan_for(self, plan, name):
-        """Rows plan of a validated P2 pattern (built once per pattern);
-        None when the mesh cannot be encoded (the exact path stays)."""
-        from .fused import RowsFusedPlan
-        if plan.signature in self.unfusable:
-            return None
-        fp = self.rows_plans.get(plan.signature)
-        if fp is None:
-            torch.cuda.current_stream().synchronize()
-            try:
-                fp = RowsFusedPlan(self.basis, plan.indptr.cpu().numpy(),
-                                   plan.indices.cpu().numpy())
-            except (NotImplementedError, ValueError, OverflowError) as exc:
-                self.unfusable[plan.signature] = str(exc)
-                return None
-            self.rows_plans[plan.signature] = fp
-        self.rows_last[name] = plan
-        return fp
 
     def assemble_bilinear(self, name, kappa=None, lam=0.0, mu=0.0, path='auto'):
         """Exact path: local entries -> pattern (matched by exact-zero mask)

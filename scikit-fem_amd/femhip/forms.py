@@ -213,17 +213,59 @@ class LinearForm(Form):
 
 
 def asm(form, *bases, **kwargs):
-    """``skfem.assembly.asm`` for one basis (``assembly/__init__.py:69-97``)."""
-    if len(bases) != 1:
-        raise NotImplementedError("asm over several bases")
-    if isinstance(form, Form):
-        return form.assemble(bases[0], **kwargs)
-    if callable(form):
+    """``skfem.assembly.asm`` (``assembly/__init__.py:63-97``): a form, or a
+    plain function of 2 (linear) or 3 (bilinear) arguments, on one basis or
+    on a list of bases whose results are summed.
+
+    The reference sums a list by concatenating the COO entries of every
+    basis and converting once (``coo_data.py:28-36,78-90``): exact-zero
+    entries are dropped, every other slot stays, also a sum that cancels to
+    0.0.  Here each basis is assembled on the device and the CSR results are
+    concatenated as COO entries and converted once, so the pattern is the
+    reference's; values agree to rounding (a slot's contributions are added
+    per basis first).  Trial and test bases must be the same basis."""
+    if not isinstance(form, Form):
+        if not callable(form):
+            raise ValueError("unknown form type")
         import inspect
         n = len(inspect.signature(form).parameters)
-        wrapped = BilinearForm(form) if n == 3 else LinearForm(form)
-        return wrapped.assemble(bases[0], **kwargs)
-    raise ValueError("unknown form type")
+        if n not in (2, 3):
+            raise NotImplementedError("asm: linear and bilinear forms only")
+        form = BilinearForm(form) if n == 3 else LinearForm(form)
+    if not bases or len(bases) > 2:
+        raise ValueError("asm takes one basis, or a trial and a test basis")
+    lists = [list(b) if isinstance(b, (list, tuple)) else [b] for b in bases]
+    if len(lists) == 2:
+        if len(lists[0]) != len(lists[1]) or any(u is not v for u, v in zip(*lists)):
+            raise NotImplementedError("asm: different trial and test bases")
+    blist = lists[0]
+    if not blist:
+        raise ValueError("asm: empty list of bases")
+    if len(blist) == 1 and len(lists) == 1:
+        return form.assemble(blist[0], **kwargs)
+    parts = [form.assemble(b, **kwargs) for b in blist]
+    if isinstance(form, LinearForm):
+        out = parts[0].copy()
+        for v in parts[1:]:
+            out += v
+        return out
+    return _coo_sum(parts)
+
+
+def _coo_sum(mats):
+    """CSR matrices summed as one COO list converted to CSR (explicit zeros
+    of cancelled sums are kept, as the reference's conversion keeps them)."""
+    from scipy.sparse import coo_matrix
+    shape = tuple(max(m.shape[i] for m in mats) for i in range(2))
+    rows, cols, data = [], [], []
+    for m in mats:
+        c = m.tocoo()
+        rows.append(c.row)
+        cols.append(c.col)
+        data.append(c.data)
+    K = coo_matrix((np.concatenate(data), (np.concatenate(rows), np.concatenate(cols))),
+                   shape=shape)
+    return K.tocsr()
 
 
 def is_number(x):

@@ -244,7 +244,7 @@ class RowsFusedPlan:
         lib = _lib.load()
         ed = np.ascontiguousarray(basis.element_dofs, dtype=np.int32)
         N = int(basis.N)
-        pos = np.ascontiguousarray(dof_positions(basis.mesh, ed, N), dtype=np.float64)
+        pos = np.ascontiguousarray(dof_positions(basis.cell_mesh, ed, N), dtype=np.float64)
         ip = np.ascontiguousarray(indptr, dtype=np.int32)
         ix = np.ascontiguousarray(indices, dtype=np.int32)
         handle = ctypes.c_void_p()
@@ -368,7 +368,7 @@ class LaplaceMassPass:
         self.plan_m = d.fused_last['mass']
         mi, mx = self.plan_m.indptr.cpu().numpy(), self.plan_m.indices.cpu().numpy()
         li, lx = self.plan_l.indptr.cpu().numpy(), self.plan_l.indices.cpu().numpy()
-        self.fp = FusedPlan(basis.mesh, mi, mx, slot_cap, node_cap)
+        self.fp = FusedPlan(basis.cell_mesh, mi, mx, slot_cap, node_cap)
         if self.fp.info.pruned:
             raise NotImplementedError("the mass pattern lacks an element pair")
         lr = laplace_rows(self.fp.host['rows'], mi, mx, li, lx)

@@ -1,0 +1,121 @@
+"""GPU parity of element subsets (``CellBasis(elements=...)``), ``asm``
+over a list of bases, and interpolation of ElementVector fields, against
+the oracle: patterns bit-exact, values within 1e-12 of the largest entry
+(the subset's exact path, fused P1 kernel and P2 rows kernel alike),
+interpolated fields bit for bit."""
+import numpy as np
+import pytest
+
+import femhip
+from femhip.models import laplace, mass, unit_load
+from oracle import skfem_np as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+_M = {'tri': (femhip.MeshTri, O.MeshTri), 'tet': (femhip.MeshTet, O.MeshTet),
+      'hex': (femhip.MeshHex, O.MeshHex)}
+
+
+def _mesh(kind, n, a, seed=0):
+    x = np.linspace(0, 1, n)
+    M = _M[kind][0]
+    m = M.init_tensor(x, x) if kind == 'tri' else M.init_tensor(x, x, x)
+    return m.jittered(a, 1.0 / (n - 1), seed=seed) if a else m
+
+
+def _omesh(m):
+    return _M[m.kind][1](m.p, m.t)
+
+
+def _same(A, R):
+    assert A.shape == R.shape
+    assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
+    assert np.abs(A.data - R.data).max() <= TOL * np.abs(R.data).max()
+
+
+def _left(m):
+    """Elements whose centroid lies left of x = 0.45 (a ragged subset)."""
+    return lambda x: x[0] < 0.45
+
+
+@pytest.mark.parametrize('kind,el,n,a,form', [
+    ('tri', 'ElementTriP1', 17, 0.2, 'laplace'), ('tet', 'ElementTetP1', 7, 0.2, 'laplace'),
+    ('tet', 'ElementTetP1', 6, 0.2, 'mass'), ('tri', 'ElementTriP2', 9, 0.2, 'laplace'),
+    ('tet', 'ElementTetP2', 5, 0.2, 'laplace'), ('tet', 'ElementTetP2', 5, 0.2, 'mass'),
+    ('hex', 'ElementHex1', 5, 0.15, 'laplace'),
+])
+def test_subset_assembly_matches_the_oracle(kind, el, n, a, form):
+    print('case', kind, el, n, form, flush=True)
+    m = _mesh(kind, n, a)
+    b = femhip.CellBasis(m, getattr(femhip, el)(), elements=_left(m))
+    assert 0 < b.nelems < m.nelements
+    ob = O.CellBasis(_omesh(m), getattr(O, el)(), elements=b.tind)
+    f = {'laplace': laplace, 'mass': mass}[form]
+    R = O.bilinear_assemble({'laplace': O.form_laplace, 'mass': O.form_mass}[form], ob)
+    for call in range(3):      # exact path, then the fused path where there is one
+        _same(f.assemble(b), R)
+    d = b.device()
+    if kind != 'hex':          # the last call ran the fused P1 / P2 rows kernel
+        assert form in (d.fused_last if el.endswith('P1') else d.rows_last)
+        assert d.last_verdict[0] == 1
+    lv = unit_load.assemble(b)
+    lr = O.linear_assemble(O.form_unit_load, ob)
+    assert np.abs(lv - lr).max() <= TOL * np.abs(lr).max()
+    assert np.all(lv[np.setdiff1d(np.arange(b.N), b.element_dofs)] == 0.0)
+
+
+@pytest.mark.parametrize('kind,el,n', [('tri', 'ElementTriP1', 17),
+                                       ('tet', 'ElementTetP2', 5)])
+def test_asm_over_a_list_of_bases(kind, el, n):
+    """asm(form, [b0, b1]) over an element partition: the reference's
+    pattern (the COO concatenation of both), values of the full assembly
+    to rounding; linear forms add up."""
+    print('case', kind, el, n, flush=True)
+    m = _mesh(kind, n, 0.2)
+    c = m.p[:, m.t].mean(axis=1)
+    sel = np.sin(7 * c[0]) + np.cos(5 * c[1]) > 0.3
+    b0 = femhip.CellBasis(m, getattr(femhip, el)(), elements=sel)
+    b1 = femhip.CellBasis(m, getattr(femhip, el)(), elements=~sel)
+    full = femhip.CellBasis(m, getattr(femhip, el)())
+    om = _omesh(m)
+    for form, oform in ((laplace, O.form_laplace), (mass, O.form_mass)):
+        for _ in range(2):
+            A = femhip.asm(form, [b0, b1])
+            _same(A, O.bilinear_assemble(oform, O.CellBasis(om, getattr(O, el)())))
+            _same(A, form.assemble(full))
+    v = femhip.asm(unit_load, [b0, b1])
+    r = O.linear_assemble(O.form_unit_load, O.CellBasis(om, getattr(O, el)()))
+    assert np.abs(v - r).max() <= TOL * np.abs(r).max()
+    # a plain function of (u, v, w) is wrapped as the reference does
+    B = femhip.asm(lambda u, v, w: u * v, [b0, b1])
+    _same(B, O.bilinear_assemble(O.form_mass, O.CellBasis(om, getattr(O, el)())))
+
+
+@pytest.mark.parametrize('kind,el,n,a', [
+    ('tri', 'ElementTriP1', 9, 0.2), ('tri', 'ElementTriP2', 7, 0.2),
+    ('tet', 'ElementTetP1', 5, 0.2), ('tet', 'ElementTetP2', 4, 0.2),
+    ('hex', 'ElementHex1', 4, 0.15),
+])
+def test_vector_interpolation_bitexact(kind, el, n, a):
+    """ElementVector fields: value (dim, E, Q), grad (dim, dim, E, Q), equal
+    to the oracle's restatement of the reference bit for bit."""
+    print('case', kind, el, n, flush=True)
+    m = _mesh(kind, n, a)
+    b = femhip.CellBasis(m, femhip.ElementVector(getattr(femhip, el)()))
+    w = np.random.default_rng(11).standard_normal(b.N)
+    f = b.interpolate(w)
+    ob = O.CellBasis(_omesh(m), O.ElementVector(getattr(O, el)()))
+    val, grad = O.interpolate(ob, w)
+    assert f.value.shape == val.shape and f.grad.shape == grad.shape
+    assert np.array_equal(f.value, val) and np.array_equal(f.grad, grad)
+
+
+def test_subset_interpolation_bitexact():
+    m = _mesh('tet', 5, 0.2)
+    b = femhip.CellBasis(m, femhip.ElementTetP2(), elements=_left(m))
+    w = np.random.default_rng(12).standard_normal(b.N)
+    f = b.interpolate(w)
+    val, grad = O.interpolate(O.CellBasis(_omesh(m), O.ElementTetP2(), elements=b.tind), w)
+    assert f.value.shape == (b.nelems, b.W.shape[0])
+    assert np.array_equal(f.value, val) and np.array_equal(f.grad, grad)
