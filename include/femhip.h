@@ -120,6 +120,15 @@ int femhip_geometry(const femhip_geom_t* g, double* G, int32_t* status,
 int femhip_local_bilinear(const femhip_geom_t* g, const femhip_space_t* sp,
                           const femhip_form_t* f, const double* G,
                           double* contrib, femhip_stream_t s);
+/* Trial functions from u, test functions from v (BilinearForm._assemble
+ * with vbasis != ubasis, bilinear_form.py:58-128): both spaces tabulated at
+ * the same quadrature points of the same elements, both scalar or both
+ * ElementVector; the mapping and weights are the trial basis's (g, G).
+ *   contrib[(j*v->nbfun + i)*nelems + e], j trial, i test; rows are v's
+ *   DOFs, columns u's (shape (v->ndofs, u->ndofs)). */
+int femhip_local_bilinear_mixed(const femhip_geom_t* g, const femhip_space_t* u,
+                                const femhip_space_t* v, const femhip_form_t* f,
+                                const double* G, double* contrib, femhip_stream_t s);
 int femhip_local_linear(const femhip_geom_t* g, const femhip_space_t* sp,
                         const femhip_form_t* f, const double* G,
                         double* contrib, femhip_stream_t s);

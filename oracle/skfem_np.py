@@ -919,41 +919,47 @@ def form_coef_load(v, w):
 # ---------------------------------------------------------------------------
 # assembly (skfem/assembly/form/*.py)
 # ---------------------------------------------------------------------------
-def bilinear_coo(form, basis, nthreads=0, progress=None, **w):
+def bilinear_coo(form, basis, nthreads=0, progress=None, vbasis=None, **w):
     """``BilinearForm._assemble``, ``bilinear_form.py:58-128``: the data is
     laid out ``(j, i, e)`` C-order, rows from the test (i), cols from the
-    trial (j) basis.  ``nthreads > 0``: the (j, i) entries are split over
-    Python threads as the reference does (``bilinear_form.py:100-119``); the
-    values do not change.  ``progress(j, i)``, when given, is called after
-    each entry (long timing runs report their progress through it)."""
-    nt, nb = basis.nelems, basis.Nbfun
-    data = np.zeros((nb, nb, nt))
-    rows = np.zeros(nb * nb * nt, dtype=np.int32)
-    cols = np.zeros(nb * nb * nt, dtype=np.int32)
+    trial (j) basis; ``vbasis`` (default: ``basis``) is the test basis, the
+    weights ``dx`` are the trial basis's and the shape is
+    ``(vbasis.N, basis.N)``.  ``nthreads > 0``: the (j, i) entries are split
+    over Python threads as the reference does (``bilinear_form.py:100-119``);
+    the values do not change.  ``progress(j, i)``, when given, is called
+    after each entry (long timing runs report their progress through it)."""
+    ub = basis
+    vb = basis if vbasis is None else vbasis
+    if ub.X.shape[-1] != vb.X.shape[-1]:
+        raise ValueError("Quadrature mismatch: trial and test functions "
+                         "should have same number of integration points.")
+    nt, nu, nv = ub.nelems, ub.Nbfun, vb.Nbfun
+    data = np.zeros((nu, nv, nt))
+    rows = np.zeros(nu * nv * nt, dtype=np.int32)
+    cols = np.zeros(nu * nv * nt, dtype=np.int32)
 
     def entry(j, i):
-        data[j, i, :] = np.sum(form(basis.basis[j], basis.basis[i], w)
-                               * basis.dx, axis=1)
+        data[j, i, :] = np.sum(form(ub.basis[j], vb.basis[i], w) * ub.dx, axis=1)
         if progress is not None:
             progress(j, i)
 
-    for j in range(nb):
-        for i in range(nb):
-            ixs = slice(nt * (nb * j + i), nt * (nb * j + i + 1))
-            rows[ixs] = basis.element_dofs[i]
-            cols[ixs] = basis.element_dofs[j]
+    for j in range(nu):
+        for i in range(nv):
+            ixs = slice(nt * (nv * j + i), nt * (nv * j + i + 1))
+            rows[ixs] = vb.element_dofs[i]
+            cols[ixs] = ub.element_dofs[j]
             if nthreads <= 0:
                 entry(j, i)
     if nthreads > 0:
         import threading
-        pairs = np.array([[j, i] for j in range(nb) for i in range(nb)])
+        pairs = np.array([[j, i] for j in range(nu) for i in range(nv)])
         threads = [threading.Thread(target=lambda ix=ix: [entry(j, i) for j, i in ix])
                    for ix in np.array_split(pairs, nthreads, axis=0)]
         for t in threads:
             t.start()
         for t in threads:
             t.join()
-    return np.array([rows, cols]), data.flatten('C'), (basis.N, basis.N)
+    return np.array([rows, cols]), data.flatten('C'), (vb.N, ub.N)
 
 
 def coo_to_csr(indices, data, shape):
@@ -963,9 +969,10 @@ def coo_to_csr(indices, data, shape):
     return K.tocsr()
 
 
-def bilinear_assemble(form, basis, nthreads=0, progress=None, **w):
+def bilinear_assemble(form, basis, nthreads=0, progress=None, vbasis=None, **w):
     """``BilinearForm.assemble``, ``bilinear_form.py:130-148``."""
-    return coo_to_csr(*bilinear_coo(form, basis, nthreads=nthreads, progress=progress, **w))
+    return coo_to_csr(*bilinear_coo(form, basis, nthreads=nthreads, progress=progress,
+                                    vbasis=vbasis, **w))
 
 
 def linear_assemble(form, basis, **w):

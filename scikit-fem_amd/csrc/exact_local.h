@@ -22,9 +22,14 @@ struct LocalArgs {
   const double* kappa;
   double* out;
   int64_t E;
-  int nb;
+  int nb;               // trial functions (u); the only space of a one-space form
   int kappa_mode;
   double lam, two_mu;
+  // test functions (v): the same tables as u unless the form has a test
+  // basis of its own (bilinear_form.py:58-128 with vbasis != ubasis)
+  const double* phiv;
+  const double* dphiv;
+  int nbv;
 };
 
 // Inverse Jacobian and |det| at (e, q): affine values are per element,
@@ -156,9 +161,9 @@ template <int D, bool AFF, int Q, int FK, bool VEC>
 __global__ void __launch_bounds__(kBlock) local_bilinear_kernel(LocalArgs a) {
 #pragma clang fp contract(off)
   const int64_t E = a.E;
-  const unsigned nb2 = (unsigned)(a.nb * a.nb);
+  const unsigned nb2 = (unsigned)(a.nb * a.nbv);
   int64_t e;
-  int entry;  // = j * nb + i
+  int entry;  // = j * nbv + i
   if constexpr (AFF) {
     e = (int64_t)(blockIdx.x / nb2) * kBlock + threadIdx.x;
     entry = (int)(blockIdx.x % nb2);
@@ -168,8 +173,8 @@ __global__ void __launch_bounds__(kBlock) local_bilinear_kernel(LocalArgs a) {
     e = ((int64_t)(r / nb2) * kXcds + xcd) * kIsoBlock + threadIdx.x;
   }
   if (e >= E) return;
-  const int j = entry / a.nb;
-  const int i = entry - j * a.nb;
+  const int j = entry / a.nbv;
+  const int i = entry - j * a.nbv;
   const int fu = VEC ? j / D : j, cu = VEC ? j - (j / D) * D : 0;
   const int fv = VEC ? i / D : i, cv = VEC ? i - (i / D) * D : 0;
 
@@ -182,15 +187,15 @@ __global__ void __launch_bounds__(kBlock) local_bilinear_kernel(LocalArgs a) {
     const double dx = m.adet * a.W[q];
     double val;
     if constexpr (FK == FEMHIP_MASS) {
-      val = a.phi[fu * Q + q] * a.phi[fv * Q + q];
+      val = a.phi[fu * Q + q] * a.phiv[fv * Q + q];
     } else if constexpr (FK == FEMHIP_COEF_MASS) {
       // w.c * u * v, evaluated left to right
       const double k = (a.kappa_mode == 1) ? a.kappa[e] : a.kappa[e * Q + q];
-      val = (k * a.phi[fu * Q + q]) * a.phi[fv * Q + q];
+      val = (k * a.phi[fu * Q + q]) * a.phiv[fv * Q + q];
     } else {
       double gu[D], gv[D];
       global_grad<D>(m.inv, a.dphi, fu, q, Q, gu);
-      global_grad<D>(m.inv, a.dphi, fv, q, Q, gv);
+      global_grad<D>(m.inv, a.dphiv, fv, q, Q, gv);
       if constexpr (FK == FEMHIP_LAPLACE) {
         double s = 0.0;
 #pragma unroll
@@ -303,8 +308,8 @@ __global__ void __launch_bounds__(kBlock) local_load_kernel(LocalArgs a) {
 template <int D, bool AFF, int Q>
 int launch_bilinear_q(const LocalArgs& a, int fk, bool vec, hipStream_t s) {
   if constexpr (AFF) {
-    // scalar forms on simplices: one thread per (element, row)
-    if (!vec && fk != FEMHIP_ELASTICITY) {
+    // scalar forms of one space on simplices: one thread per (element, row)
+    if (!vec && fk != FEMHIP_ELASTICITY && a.phiv == a.phi && a.nbv == a.nb) {
       const dim3 rgrid((unsigned)ceil_div(a.E, kBlock), (unsigned)a.nb);
       switch (fk) {
         case FEMHIP_LAPLACE:
@@ -329,9 +334,9 @@ int launch_bilinear_q(const LocalArgs& a, int fk, bool vec, hipStream_t s) {
   }
   const int64_t nblk = AFF ? ceil_div(a.E, kBlock)
                            : ceil_div(ceil_div(a.E, kIsoBlock), kXcds) * kXcds;
-  FEMHIP_REQUIRE(nblk * a.nb * a.nb <= INT32_MAX, FEMHIP_EOVERFLOW,
+  FEMHIP_REQUIRE(nblk * a.nb * a.nbv <= INT32_MAX, FEMHIP_EOVERFLOW,
                  "local entries: grid exceeds 2^31 blocks");
-  const unsigned grid = (unsigned)(nblk * a.nb * a.nb);
+  const unsigned grid = (unsigned)(nblk * a.nb * a.nbv);
   const int threads = AFF ? kBlock : kIsoBlock;
   if (vec) {
     FEMHIP_REQUIRE(fk == FEMHIP_ELASTICITY, FEMHIP_EUNSUPPORTED,

@@ -184,9 +184,37 @@ extern "C" int femhip_local_bilinear(const femhip_geom_t* g, const femhip_space_
                  FEMHIP_EINVAL, "kappa_laplace / coef_mass need the coefficient (mode 1 or 2)");
   if (g->nelems == 0) return FEMHIP_OK;
   FEMHIP_REQUIRE(G != nullptr && contrib != nullptr, FEMHIP_EINVAL, "NULL G or output");
-  exact::LocalArgs a{G, g->W, sp->phi, sp->dphi, f->kappa, contrib, g->nelems, sp->nbfun,
-              f->kappa_mode, f->lam, f->two_mu};
+  exact::LocalArgs a{G,        g->W,          sp->phi,   sp->dphi, f->kappa,
+                     contrib,  g->nelems,     sp->nbfun, f->kappa_mode,
+                     f->lam,   f->two_mu,     sp->phi,   sp->dphi, sp->nbfun};
   return dispatch_bilinear(g->kind, g->nq, a, f->kind, sp->vector != 0,
+                           reinterpret_cast<hipStream_t>(s));
+}
+
+extern "C" int femhip_local_bilinear_mixed(const femhip_geom_t* g, const femhip_space_t* u,
+                                           const femhip_space_t* v, const femhip_form_t* f,
+                                           const double* G, double* contrib,
+                                           femhip_stream_t s) {
+  int rc = check_geom(g);
+  if (rc) return rc;
+  if ((rc = check_space(g, u))) return rc;
+  if ((rc = check_space(g, v))) return rc;
+  FEMHIP_REQUIRE(u->vector == v->vector, FEMHIP_EUNSUPPORTED,
+                 "trial and test spaces must both be scalar or both ElementVector");
+  FEMHIP_REQUIRE((int64_t)u->nbfun * v->nbfun <= 65535, FEMHIP_EUNSUPPORTED,
+                 "too many local entries");
+  FEMHIP_REQUIRE(f != nullptr, FEMHIP_EINVAL, "form is NULL");
+  FEMHIP_REQUIRE(f->kind != FEMHIP_UNIT_LOAD && f->kind != FEMHIP_COEF_LOAD, FEMHIP_EINVAL,
+                 "unit_load / coef_load are linear forms");
+  FEMHIP_REQUIRE((f->kind != FEMHIP_KAPPA_LAPLACE && f->kind != FEMHIP_COEF_MASS) ||
+                     (f->kappa != nullptr && (f->kappa_mode == 1 || f->kappa_mode == 2)),
+                 FEMHIP_EINVAL, "kappa_laplace / coef_mass need the coefficient (mode 1 or 2)");
+  if (g->nelems == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(G != nullptr && contrib != nullptr, FEMHIP_EINVAL, "NULL G or output");
+  exact::LocalArgs a{G,        g->W,          u->phi,   u->dphi, f->kappa,
+                     contrib,  g->nelems,     u->nbfun, f->kappa_mode,
+                     f->lam,   f->two_mu,     v->phi,   v->dphi, v->nbfun};
+  return dispatch_bilinear(g->kind, g->nq, a, f->kind, u->vector != 0,
                            reinterpret_cast<hipStream_t>(s));
 }
 
@@ -204,8 +232,9 @@ extern "C" int femhip_local_linear(const femhip_geom_t* g, const femhip_space_t*
                  FEMHIP_EINVAL, "coef_load needs the coefficient (mode 1 or 2)");
   if (g->nelems == 0) return FEMHIP_OK;
   FEMHIP_REQUIRE(G != nullptr && contrib != nullptr, FEMHIP_EINVAL, "NULL G or output");
-  exact::LocalArgs a{G, g->W, sp->phi, sp->dphi, coef ? f->kappa : nullptr, contrib, g->nelems,
-              sp->nbfun, coef ? f->kappa_mode : 0, 0.0, 0.0};
+  exact::LocalArgs a{G,        g->W,      sp->phi,   sp->dphi, coef ? f->kappa : nullptr,
+                     contrib,  g->nelems, sp->nbfun, coef ? f->kappa_mode : 0,
+                     0.0,      0.0,       sp->phi,   sp->dphi, sp->nbfun};
   return dispatch_load(g->kind, g->nq, a, reinterpret_cast<hipStream_t>(s));
 }
 

@@ -100,6 +100,10 @@ SIGNATURES = {
     'femhip_local_bilinear': (ctypes.c_int, [ctypes.POINTER(Geom),
                                              ctypes.POINTER(Space),
                                              ctypes.POINTER(Form), _p, _p, _p]),
+    'femhip_local_bilinear_mixed': (ctypes.c_int, [ctypes.POINTER(Geom),
+                                                   ctypes.POINTER(Space),
+                                                   ctypes.POINTER(Space),
+                                                   ctypes.POINTER(Form), _p, _p, _p]),
     'femhip_local_linear': (ctypes.c_int, [ctypes.POINTER(Geom),
                                            ctypes.POINTER(Space),
                                            ctypes.POINTER(Form), _p, _p, _p]),

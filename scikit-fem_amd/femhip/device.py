@@ -159,6 +159,7 @@ class DeviceBasis:
         self.row_plan = None
         self.rows_plans = {}     # plan serial -> RowsFusedPlan (P2 simplices)
         self.rows_last = {}      # form -> the last plan its rows path used
+        self.mixed_plans = {}    # id(test device basis) -> (it, {serial: Plan})
 
     # -- coordinates ---------------------------------------------------------
     def set_coordinates(self, p):
@@ -220,18 +221,21 @@ class DeviceBasis:
         _lib.check(self.lib.femhip_pattern_workspace(ncontrib, ctypes.byref(nb)))
         return torch.empty(int(nb.value), dtype=torch.uint8, device=self.p.device), int(nb.value)
 
-    def build_plan(self, contrib, serial=None):
-        C = self.nbfun * self.nbfun * self.E
+    def build_plan(self, contrib, serial=None, test=None):
+        """Pattern and reduction plan of ``contrib``; ``test``: the device
+        basis of a distinct test space (its DOFs index the rows)."""
+        test = self if test is None else test
+        C = self.nbfun * test.nbfun * self.E
         dev = self.p.device
         ws, wsb = self._workspace(C)
-        indptr = torch.empty(self.N + 1, dtype=torch.int32, device=dev)
+        indptr = torch.empty(test.N + 1, dtype=torch.int32, device=dev)
         indices = torch.empty(max(C, 1), dtype=torch.int32, device=dev)
         perm = torch.empty(max(C, 1), dtype=torch.int32, device=dev)
         seg = torch.empty(max(C, 1) + 1, dtype=torch.int32, device=dev)
         nnz = ctypes.c_int64(0)
         _lib.check(self.lib.femhip_pattern_build(
-            _lib.ptr(self.dofs), _lib.ptr(self.dofs), self.nbfun, self.nbfun, self.E,
-            self.N, self.N, _lib.ptr(contrib), _lib.ptr(indptr), _lib.ptr(indices),
+            _lib.ptr(test.dofs), _lib.ptr(self.dofs), test.nbfun, self.nbfun, self.E,
+            test.N, self.N, _lib.ptr(contrib), _lib.ptr(indptr), _lib.ptr(indices),
             _lib.ptr(perm), _lib.ptr(seg), ctypes.byref(nnz), _lib.ptr(ws), wsb,
             _lib.stream_ptr()))
         n = int(nnz.value)
@@ -313,6 +317,38 @@ class DeviceBasis:
             torch.cuda.current_stream().synchronize()
             self.row_plan = (perm, seg)
         return self.row_plan
+
+    def assemble_mixed(self, test, name, kappa=None, lam=0.0, mu=0.0):
+        """CSR assembly with trial functions from this basis and test
+        functions from ``test`` (another device basis on the same elements
+        and quadrature points; ``bilinear_form.py:58-128``): exact-order
+        local entries, a pattern per exact-zero set (matched bit for bit as
+        in ``plan_for``), segmented sums; shape (test.N, self.N)."""
+        G = self.geometry()
+        fs, keep = self.form_struct(name, kappa, lam, mu)
+        C = self.nbfun * test.nbfun * self.E
+        dev = self.p.device
+        if C == 0:
+            return DeviceCSR(torch.zeros(test.N + 1, dtype=torch.int32, device=dev),
+                             torch.zeros(0, dtype=torch.int32, device=dev),
+                             torch.zeros(0, dtype=torch.float64, device=dev),
+                             (test.N, self.N))
+        contrib = torch.empty(C, dtype=torch.float64, device=dev)
+        _lib.check(self.lib.femhip_local_bilinear_mixed(
+            ctypes.byref(self.geom), ctypes.byref(self.space), ctypes.byref(test.space),
+            ctypes.byref(fs), _lib.ptr(G), _lib.ptr(contrib), _lib.stream_ptr()))
+        holder = self.mixed_plans.setdefault(id(test), (test, {}))
+        plans = holder[1]
+        plan = None
+        for serial in sorted(plans, reverse=True):
+            if self.zero_mismatch(contrib, plans[serial].mask) == 0:
+                plan = plans[serial]
+                break
+        if plan is None:
+            plan = self.build_plan(contrib, len(plans), test=test)
+            plans[plan.signature] = plan
+        data = self.segment_sum(contrib, plan.perm, plan.seg, plan.nnz)
+        return DeviceCSR(plan.indptr, plan.indices, data, (test.N, self.N), plan)
 
     def segment_sum(self, contrib, perm, seg, nseg, out=None):
         if out is None:

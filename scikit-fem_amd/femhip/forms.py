@@ -118,7 +118,7 @@ class BilinearForm(Form):
         'auto' (fused kernel when available), 'generic' (exact path)."""
         self._check_dtype()
         if vbasis is not None and vbasis is not ubasis:
-            raise NotImplementedError("different trial and test bases")
+            return self._assemble_mixed(ubasis, vbasis, **kwargs)
         name, scale, params = self.spec()
         if name == 'vector_laplace':
             A = _vector_laplace(ubasis, path)
@@ -134,6 +134,44 @@ class BilinearForm(Form):
             A = dev.assemble_bilinear(name, lam=params['Lambda'], mu=params['Mu'], path=path)
         else:
             A = dev.assemble_bilinear(name, path=path)
+        if scale != 1.0:
+            A.data = A.data * scale
+        return A
+
+    def _assemble_mixed(self, ubasis, vbasis, path='auto', **kwargs):
+        """Trial functions from ``ubasis``, test functions from ``vbasis``
+        (``bilinear_form.py:58-128``): rows are the test basis's DOFs, the
+        shape is (vbasis.N, ubasis.N), the weights and coefficients are the
+        trial basis's.  Both bases are cell bases on the same elements with
+        the same quadrature points, both scalar or both vector."""
+        if ubasis.X.shape[-1] != vbasis.X.shape[-1]:
+            raise ValueError("Quadrature mismatch: trial and test functions "
+                             "should have same number of integration points.")
+        for b in (ubasis, vbasis):
+            if getattr(b, 'find', None) is not None:
+                raise NotImplementedError("distinct trial and test bases: cell bases only")
+        same_cells = (vbasis.mesh is ubasis.mesh or (
+            np.array_equal(vbasis.mesh.p, ubasis.mesh.p)
+            and np.array_equal(vbasis.mesh.t, ubasis.mesh.t)))
+        same_tind = ((ubasis.tind is None and vbasis.tind is None) or (
+            ubasis.tind is not None and vbasis.tind is not None
+            and np.array_equal(ubasis.tind, vbasis.tind)))
+        if not (same_cells and same_tind):
+            raise NotImplementedError("distinct trial and test bases on different elements")
+        if not (np.array_equal(ubasis.X, vbasis.X) and np.array_equal(ubasis.W, vbasis.W)):
+            raise NotImplementedError("distinct trial and test bases: different quadrature points")
+        if bool(ubasis.elem.vector) != bool(vbasis.elem.vector):
+            raise NotImplementedError("a vector trial basis with a scalar test basis")
+        name, scale, params = self.spec()
+        if name not in ('laplace', 'mass', 'kappa_laplace', 'coef_mass', 'linear_elasticity'):
+            raise NotImplementedError("distinct trial and test bases: form %r" % (name,))
+        kappa = None
+        if name in ('kappa_laplace', 'coef_mass'):
+            kappa = _kappa_tensor({**self.params, **kwargs}, ubasis,
+                                  params.get('coef', 'kappa'))
+        A = ubasis.device().assemble_mixed(vbasis.device(), name, kappa=kappa,
+                                           lam=params.get('Lambda', 0.0),
+                                           mu=params.get('Mu', 0.0))
         if scale != 1.0:
             A.data = A.data * scale
         return A
@@ -223,7 +261,8 @@ def asm(form, *bases, **kwargs):
     0.0.  Here each basis is assembled on the device and the CSR results are
     concatenated as COO entries and converted once, so the pattern is the
     reference's; values agree to rounding (a slot's contributions are added
-    per basis first).  Trial and test bases must be the same basis."""
+    per basis first).  A trial and a test argument, each a basis or a list,
+    assemble every (trial, test) combination, as the reference does."""
     if not isinstance(form, Form):
         if not callable(form):
             raise ValueError("unknown form type")
@@ -235,15 +274,16 @@ def asm(form, *bases, **kwargs):
     if not bases or len(bases) > 2:
         raise ValueError("asm takes one basis, or a trial and a test basis")
     lists = [list(b) if isinstance(b, (list, tuple)) else [b] for b in bases]
-    if len(lists) == 2:
-        if len(lists[0]) != len(lists[1]) or any(u is not v for u, v in zip(*lists)):
-            raise NotImplementedError("asm: different trial and test bases")
-    blist = lists[0]
-    if not blist:
+    if any(not b for b in lists):
         raise ValueError("asm: empty list of bases")
-    if len(blist) == 1 and len(lists) == 1:
-        return form.assemble(blist[0], **kwargs)
-    parts = [form.assemble(b, **kwargs) for b in blist]
+    if isinstance(form, LinearForm) and len(lists) != 1:
+        raise ValueError("LinearForm takes one basis")
+    # every combination of the arguments, summed (assembly/__init__.py:91-94)
+    import itertools
+    combos = list(itertools.product(*lists))
+    if len(combos) == 1:
+        return form.assemble(*combos[0], **kwargs)
+    parts = [form.assemble(*c, **kwargs) for c in combos]
     if isinstance(form, LinearForm):
         out = parts[0].copy()
         for v in parts[1:]:

@@ -1,8 +1,8 @@
 """GPU parity of element subsets (``CellBasis(elements=...)``), ``asm``
-over a list of bases, and interpolation of ElementVector fields, against
-the oracle: patterns bit-exact, values within 1e-12 of the largest entry
-(the subset's exact path, fused P1 kernel and P2 rows kernel alike),
-interpolated fields bit for bit."""
+over lists of bases, distinct trial and test bases, and interpolation of
+ElementVector fields, against the oracle: patterns bit-exact, values within
+1e-12 of the largest entry (the subset's exact path, fused P1 kernel and P2
+rows kernel alike), interpolated fields bit for bit."""
 import numpy as np
 import pytest
 
@@ -119,3 +119,69 @@ def test_subset_interpolation_bitexact():
     val, grad = O.interpolate(O.CellBasis(_omesh(m), O.ElementTetP2(), elements=b.tind), w)
     assert f.value.shape == (b.nelems, b.W.shape[0])
     assert np.array_equal(f.value, val) and np.array_equal(f.grad, grad)
+
+
+@pytest.mark.parametrize('kind,eu,ev,n,form', [
+    ('tri', 'ElementTriP2', 'ElementTriP1', 9, 'laplace'),
+    ('tri', 'ElementTriP1', 'ElementTriP2', 9, 'mass'),
+    ('tet', 'ElementTetP2', 'ElementTetP1', 4, 'laplace'),
+    ('tet', 'ElementTetP1', 'ElementTetP2', 4, 'coef_mass'),
+    ('hex', 'ElementHex2', 'ElementHex1', 3, 'laplace'),
+    ('tet', 'vP2', 'vP1', 3, 'elasticity'),
+])
+def test_distinct_trial_and_test_bases(kind, eu, ev, n, form):
+    """BilinearForm.assemble(ubasis, vbasis): rows from the test basis,
+    shape (vbasis.N, ubasis.N), against the oracle's mixed assembly; a
+    second call reuses the pattern and gives the same result."""
+    print('case', kind, eu, ev, n, form, flush=True)
+    from femhip.models import linear_elasticity
+    m = _mesh(kind, n, 0.15 if kind == 'hex' else 0.2)
+    om = _omesh(m)
+
+    def el(name, mod):
+        if name.startswith('v'):
+            return mod.ElementVector(getattr(mod, 'ElementTet' + name[1:])())
+        return getattr(mod, name)()
+    order = {'tri': 4, 'tet': 4, 'hex': 4}[kind]
+    ub = femhip.CellBasis(m, el(eu, femhip), intorder=order)
+    vb = femhip.CellBasis(m, el(ev, femhip), intorder=order)
+    oub = O.CellBasis(om, el(eu, O), intorder=order)
+    ovb = O.CellBasis(om, el(ev, O), intorder=order)
+    kw, okw = {}, {}
+    if form == 'laplace':
+        F, OF = laplace, O.form_laplace
+    elif form == 'mass':
+        F, OF = mass, O.form_mass
+    elif form == 'coef_mass':
+        c = 1.0 + np.random.default_rng(2).random((ub.nelems, ub.W.shape[0]))
+        F = femhip.BilinearForm(lambda u, v, w: w.c * u * v)
+        OF, kw, okw = O.form_coef_mass, {'c': c}, {'c': c}
+    else:
+        lam, mu = O.lame_parameters(1.0, 0.3)
+        F, OF = linear_elasticity(lam, mu), O.form_elasticity(lam, mu)
+    R = O.bilinear_assemble(OF, oub, vbasis=ovb, **okw)
+    A = F.assemble(ub, vb, **kw)
+    assert A.shape == (vb.N, ub.N) == R.shape
+    _same(A, R)
+    A2 = F.assemble(ub, vb, **kw)
+    assert np.array_equal(A2.data, A.data) and np.array_equal(A2.indices, A.indices)
+
+
+def test_asm_over_trial_and_test_lists():
+    """asm(form, [u0, u1], [v0, v1]) sums every (trial, test) combination
+    (assembly/__init__.py:91-94); here the two halves of an element
+    partition with a P2 trial and a P1 test space on each."""
+    m = _mesh('tri', 9, 0.2)
+    c = m.p[:, m.t].mean(axis=1)
+    sel = c[0] < 0.5
+    ub = [femhip.CellBasis(m, femhip.ElementTriP2(), elements=s) for s in (sel, ~sel)]
+    vb = [femhip.CellBasis(m, femhip.ElementTriP1(), intorder=4, elements=s)
+          for s in (sel, ~sel)]
+    with pytest.raises(NotImplementedError):   # (u0, v1): different elements
+        femhip.asm(mass, ub, vb)
+    A = femhip.asm(mass, ub[0], vb[0])
+    om = _omesh(m)
+    R = O.bilinear_assemble(O.form_mass, O.CellBasis(om, O.ElementTriP2(), elements=ub[0].tind),
+                            vbasis=O.CellBasis(om, O.ElementTriP1(), intorder=4,
+                                               elements=vb[0].tind))
+    _same(A, R)
