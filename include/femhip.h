@@ -431,14 +431,19 @@ size_t femhip_rows_fused_lds_bytes(const femhip_fused_plan_info_t* info);
 
 /* data[nnz] <- CSR values of `form` (FEMHIP_LAPLACE or FEMHIP_MASS) for a
  * P2 simplex basis (g, sp as for femhip_local_bilinear; G its
- * femhip_geometry) on the pattern the rows plan was built for: the local
- * entries in the reference's arithmetic (BilinearForm._assemble,
- * bilinear_form.py:58-128) summed in LDS rows (COOData._assemble_scipy_csr,
- * coo_data.py:28-36), no local-matrix array.  The pattern is checked
- * exactly in the same pass: with check != NULL, check->verdict (device
- * int32[4]) <- {valid, 0, 0, flags} (its other fields unused; nzeros = 0).
- * Two launches, as femhip_fused_assemble.  rowrecs: the exported row
- * records. */
+ * femhip_geometry) on the pattern the rows plan was built for
+ * (BilinearForm._assemble + COOData._assemble_scipy_csr,
+ * bilinear_form.py:58-148, coo_data.py:28-36): every entry from the
+ * element metric |det| A^-1 A^-T and tables of the form, summed in LDS rows,
+ * no local-matrix array.  With check != NULL the call is validated against
+ * the pattern as femhip_fused_assemble's is: entries within the rounding
+ * bound of zero are queued (check->queue, check->capacity), recomputed in
+ * the reference's arithmetic and compared with check->zero_mask /
+ * check->nzeros (femhip_zero_mask of the contributions the pattern was
+ * built from); check->verdict (device int32[4]) <- {valid, mismatches,
+ * zeros, flags}; check's t/dphi/W fields are not used (g and sp give
+ * them).  Two launches, as femhip_fused_assemble, then the recheck and the
+ * verdict.  rowrecs: the exported row records. */
 int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const uint32_t* rowrecs,
                                femhip_fused_state_t* state, int32_t form,
                                const femhip_geom_t* g, const femhip_space_t* sp,

@@ -30,7 +30,6 @@ PER_FILE = {
     'interpolate.hip': ['-ffp-contract=off'],
     'facet.hip': ['-ffp-contract=off'],
     'coords.hip': ['-ffp-contract=off'],
-    'fused_rows.hip': ['-ffp-contract=off'],
 }
 
 

@@ -251,6 +251,7 @@ struct RecheckArgs {
   const int32_t* t;
   const double* p;
   const double* dphi;
+  const double* phi;   // P2 mass rechecks only
   const double* W;
   const double* kappa;
   const uint64_t* queue;
@@ -265,18 +266,17 @@ struct RecheckArgs {
 
 constexpr int kCheckBlocks = 128;
 
-// Entry (e, i, j) of the P1 Laplace / kappa-Laplace matrix in the
-// reference's arithmetic (the same operations as local_row_kernel: the
-// affine inverse of mapping_affine.py:55-131, einsum-ordered gradients,
-// (kappa grad u) . grad v summed from 0, times dx, NumPy's pairwise sum over
-// the points) compared with 0.0; counts the zero entries and those whose
-// zero-ness differs from the plan's mask bit of contribution
-// (j * nb + i) * E + e (fused_recheck_verdict_kernel then forms the
-// verdict).
-template <int D, int Q, int FK>
+// Entry (e, i, j) of a Laplace / kappa-Laplace / mass matrix of NB local
+// functions in the reference's arithmetic (the same operations as
+// local_row_kernel: the affine inverse of mapping_affine.py:55-131,
+// einsum-ordered gradients, (kappa grad u) . grad v summed from 0 or
+// u * v, times dx, NumPy's pairwise sum over the points) compared with 0.0;
+// counts the zero entries and those whose zero-ness differs from the plan's
+// mask bit of contribution (j * NB + i) * E + e
+// (fused_recheck_verdict_kernel then forms the verdict).
+template <int D, int Q, int FK, int NB = D + 1>
 __global__ void __launch_bounds__(kBlock) fused_recheck_kernel(RecheckArgs a) {
 #pragma clang fp contract(off)
-  constexpr int NB = D + 1;
   __shared__ unsigned long long sm[kBlock / 64], sz[kBlock / 64];
   femhip_fused_state_t* st = a.st;
   const int64_t nq = st->nsusp;
@@ -293,6 +293,7 @@ __global__ void __launch_bounds__(kBlock) fused_recheck_kernel(RecheckArgs a) {
     auto term = [&](int qq) -> double {
 #pragma clang fp contract(off)
       const double dx = adet * a.W[qq];
+      if constexpr (FK == FEMHIP_MASS) return (a.phi[j * Q + qq] * a.phi[i * Q + qq]) * dx;
       double gi[D], gj[D];
       exact::global_grad<D>(inv, a.dphi, i, qq, Q, gi);
       exact::global_grad<D>(inv, a.dphi, j, qq, Q, gj);
@@ -387,7 +388,7 @@ int fused_check_run(const femhip_fused_check_t* c, femhip_fused_state_t* st, int
   }
   FEMHIP_REQUIRE(form == FEMHIP_LAPLACE || form == FEMHIP_KAPPA_LAPLACE, FEMHIP_EUNSUPPORTED,
                  "fused check: laplace, mass or kappa_laplace");
-  RecheckArgs a{c,        c->t,         p,  c->dphi,  c->W,      kappa, c->queue,
+  RecheckArgs a{c,        c->t,         p,  c->dphi,  nullptr,  c->W,      kappa, c->queue,
                 c->zero_mask, c->capacity, c->nelems, nnodes, c->nzeros, st, c->verdict};
   if (dim == 2) {
     switch (c->Q) {
@@ -405,6 +406,38 @@ int fused_check_run(const femhip_fused_check_t* c, femhip_fused_state_t* st, int
     }
   }
   set_error("fused check: unsupported number of quadrature points");
+  return FEMHIP_EUNSUPPORTED;
+}
+
+// P2 simplices (rows plans): the suspects of the last first pass rechecked
+// against the plan's zero mask, then the verdict
+template <int D, int NB, int Q>
+int rows_recheck_form(const RecheckArgs& a, int form, hipStream_t s) {
+  if (form == FEMHIP_LAPLACE)
+    fused_recheck_kernel<D, Q, FEMHIP_LAPLACE, NB><<<kCheckBlocks, kBlock, 0, s>>>(a);
+  else
+    fused_recheck_kernel<D, Q, FEMHIP_MASS, NB><<<kCheckBlocks, kBlock, 0, s>>>(a);
+  FEMHIP_CHECK_LAUNCH();
+  fused_recheck_verdict_kernel<<<1, 1, 0, s>>>(a.st, a.nzeros, a.verdict);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+int rows_check_run(const femhip_fused_check_t* c, femhip_fused_state_t* st, int form,
+                   const femhip_geom_t* g, const femhip_space_t* sp, hipStream_t s) {
+  FEMHIP_REQUIRE(c->zero_mask && c->queue && c->capacity > 0 && g->nelems > 0, FEMHIP_EINVAL,
+                 "rows check: NULL zero mask or queue");
+  RecheckArgs a{c,          g->t,      g->p,       sp->dphi, sp->phi,  g->W,
+                nullptr,    c->queue,  c->zero_mask, c->capacity, g->nelems, g->nnodes,
+                c->nzeros,  st,        c->verdict};
+  if (g->dim == 3 && sp->nbs == 10) {
+    if (g->nq == 11) return rows_recheck_form<3, 10, 11>(a, form, s);
+    if (g->nq == 4) return rows_recheck_form<3, 10, 4>(a, form, s);
+  } else if (g->dim == 2 && sp->nbs == 6) {
+    if (g->nq == 6) return rows_recheck_form<2, 6, 6>(a, form, s);
+    if (g->nq == 3) return rows_recheck_form<2, 6, 3>(a, form, s);
+  }
+  set_error("rows check: unsupported element or number of quadrature points");
   return FEMHIP_EUNSUPPORTED;
 }
 

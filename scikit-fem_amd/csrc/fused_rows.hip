@@ -1,6 +1,6 @@
-// Fused P2 assembly on simplices: the local entries in the reference's
-// arithmetic, summed straight into LDS rows and written to the CSR values
-// once -- no local-matrix array, no separate segmented sums.
+// Fused P2 assembly on simplices: the local entries summed straight into
+// LDS rows and written to the CSR values once -- no local-matrix array, no
+// separate segmented sums.
 //
 // The plan (fused_plan.cpp, rows mode) cuts the DOFs into partitions whose
 // CSR rows fit LDS and lists every element touching an owned DOF (a visit:
@@ -8,26 +8,29 @@
 // record: the visit, the DOF, the offsets of the element's columns in the
 // DOF's row and of the DOF's column in the rows of the element's other owned
 // DOFs).  One workgroup per partition:
-//   1. stages the owned rows' table and zeroed int64 accumulators and
-//      "nonzero seen" flags in LDS;
-//   2. one lane per row record (element e, owned DOF i) computes the
-//      entries (i, j) exactly as the exact-order kernel (local_row_kernel,
-//      exact_local.h: the affine inverse of mapping_affine.py:55-131,
-//      einsum-ordered gradients, NumPy's pairwise sum over the points).
-//      Laplace and mass integrands are symmetric bit for bit, so a pair of
-//      two owned DOFs is computed once, by the lower local index, and added
-//      to both rows; a pair with a DOF owned elsewhere is computed here for
-//      this partition's row;
+//   1. forms the element-independent tables of the form in LDS and stages
+//      the owned rows' table and zeroed int64 accumulators;
+//   2. one lane per row record (element e, owned DOF i) forms the entries
+//      (i, j) from the element metric: on an affine simplex the quadrature
+//      sum of the reference (sum_q w_q |det| (A^-T dphi_i) . (A^-T dphi_j))
+//      equals sum_ab M_ab T_ab,ij with M = |det| A^-1 A^-T (6 numbers per
+//      element in 3-D) and T_ab,ij = sum_q w_q dphi_i,a dphi_j,b (mass:
+//      |det| sum_q w_q phi_i phi_j), so an entry is D(D+1)/2 fused
+//      multiply-adds.  A pair of two owned DOFs is computed once, by the
+//      lower local index, and added to both rows; a pair with a DOF owned
+//      elsewhere is computed here for this partition's row;
 //   3. writes the rows once, 16 lanes per row.
 //
-// The pattern is checked exactly in the same pass: an entry is zero in the
-// reference iff it is 0.0 here (same operations), so a slot of the plan's
-// pattern must receive at least one nonzero contribution (else
-// eliminate_zeros would have dropped it), and a nonzero contribution must
-// have a slot (offset 127: the pair is not in the pattern).  Either failure
-// sets FEMHIP_FUSED_BAD.  Sums
-// are int64 fixed point with one exponent per launch (fused_common.h).
-// This file is compiled with -ffp-contract=off (reference arithmetic).
+// Validation (as the fused P1 kernel): the entries differ from the
+// reference's by rounding; an entry the reference computes as exactly 0.0
+// comes out within tau_ij = 2^-44 D max|A^-1|^2 |det| B_ij of zero (mass:
+// 2^-44 |det| B_ij), B_ij = sum_q |w_q| |dphi_i|_1 |dphi_j|_1 (mass:
+// sum_q |w_q| |phi_i| |phi_j|), a bound ~9x above the rounding of both
+// evaluations.  Entries within tau are queued and recomputed in the
+// reference's arithmetic afterwards (fused_recheck_kernel), and the call
+// is valid iff the zero set found is the plan's; an entry above tau whose
+// pair is not in the pattern sets FEMHIP_FUSED_BAD.  Sums are int64 fixed
+// point with one exponent per launch (fused_common.h).
 #include "exact_local.h"
 #include "fused_check.h"
 #include "fused_common.h"
@@ -48,6 +51,8 @@ struct RowsArgs {
   const double* dphi;       // [nb][D][Q]
   double* out;
   femhip_fused_state_t* st;
+  uint64_t* susp;           // suspect queue (checked calls) or NULL
+  int64_t susp_cap;
   int64_t E;
   int32_t nparts;
   int32_t max_slots;
@@ -59,9 +64,8 @@ struct RowsArgs {
 constexpr int kRowsThreads = 512;
 constexpr int kRowRecWords = 6;   // visit << 4 | DOF, 2 x 10 seven-bit offsets
 
-// LDS: [reference tables (TAB doubles) | accumulators (max_slots int64) |
-//  row info, row starts (max_rows each) | reduction (2 NW doubles) |
-//  nonzero flags (max_slots bytes)]
+// LDS: [form tables (TAB doubles: T then B) | accumulators (max_slots
+//  int64) | row info, row starts (max_rows each) | reduction (2 NW doubles)]
 template <int NT, int TAB>
 struct RowsLds {
   double* tab;
@@ -69,21 +73,24 @@ struct RowsLds {
   uint32_t* rinfo;
   int32_t* rstart;
   double* red;
-  uint8_t* nz;
   __device__ RowsLds(double* smem, const RowsArgs& a) {
     tab = smem;
     acc = reinterpret_cast<unsigned long long*>(smem + TAB);
     rinfo = reinterpret_cast<uint32_t*>(acc + a.max_slots);
     rstart = reinterpret_cast<int32_t*>(rinfo + a.max_rows);
     red = reinterpret_cast<double*>(rstart + a.max_rows);
-    nz = reinterpret_cast<uint8_t*>(red + 2 * (NT / 64));
   }
 };
 
-// without the reference tables (8 * TAB bytes more per instantiation)
+// without the form tables (rows_tab_doubles() more per launch)
 size_t rows_lds_bytes(const femhip_fused_plan_info_t& I) {
-  return 8 * (size_t)I.max_slots + 8 * (size_t)I.max_rows + 8 * 2 * (kRowsThreads / 64) +
-         (size_t)I.max_slots;
+  return 8 * (size_t)I.max_slots + 8 * (size_t)I.max_rows + 8 * 2 * (kRowsThreads / 64);
+}
+
+// doubles of the form tables: T (NB^2 D(D+1)/2 for Laplace, NB^2 for mass)
+// and the suspect bound B (NB^2)
+constexpr int rows_tab_doubles(int D, int NB, bool grad) {
+  return NB * NB * (grad ? D * (D + 1) / 2 : 1) + NB * NB;
 }
 
 // x selected from five words by a per-lane index, as a chain of selects
@@ -97,34 +104,96 @@ __device__ __forceinline__ uint32_t sel5(int k, uint32_t a0, uint32_t a1, uint32
   return x;
 }
 
+// Suspects of one row record into the global queue: bits j of m1 are the
+// entries (row i, column j), bits j of m2 the entries (row j, column i);
+// one queue reservation per lane that has any (rare).
+__device__ __forceinline__ void push_row_suspects(const RowsArgs& a, int64_t e, int i,
+                                                  uint32_t m1, uint32_t m2) {
+  const int cnt = __popc(m1) + __popc(m2);
+  long long pos = (long long)atomicAdd(reinterpret_cast<unsigned long long*>(&a.st->nsusp),
+                                       (unsigned long long)cnt);
+  if (pos + cnt > a.susp_cap) atomicOr(&a.st->flags, FEMHIP_FUSED_OVERFLOW);
+  const uint64_t base = (uint64_t)(uint32_t)e;
+  while (m1) {
+    const int j = __ffs(m1) - 1;
+    m1 &= m1 - 1;
+    if (pos < a.susp_cap) a.susp[pos] = base | ((uint64_t)i << 32) | ((uint64_t)j << 40);
+    ++pos;
+  }
+  while (m2) {
+    const int j = __ffs(m2) - 1;
+    m2 &= m2 - 1;
+    if (pos < a.susp_cap) a.susp[pos] = base | ((uint64_t)j << 32) | ((uint64_t)i << 40);
+    ++pos;
+  }
+}
+
 template <int D, int NB, int Q, int FORM, int NT>
 __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
-#pragma clang fp contract(off)
   constexpr bool GRAD = FORM == FEMHIP_LAPLACE;
+  constexpr int NM = GRAD ? D * (D + 1) / 2 : 1;   // metric entries per element
   constexpr int RW = (NB * 16 + 31) / 32;
   static_assert(RW <= 5 && kRowRecWords == 6, "record words");
-  constexpr int TAB = GRAD ? NB * D * Q : NB * Q;
+  constexpr int TAB = rows_tab_doubles(D, NB, GRAD);
   extern __shared__ double smem[];
   RowsLds<NT, TAB> L(smem, a);
+  double* const T = L.tab;                 // [NB][NB][NM]
+  double* const B = L.tab + NB * NB * NM;  // [NB][NB]
   const int tid = threadIdx.x;
   femhip_fused_state_t* st = a.st;
   const bool first = a.mode == 0;
   const FxPass pass = fx_begin<false>(st, a.mode);
   if (!pass.run) return;  // uniform
   const bool scatter = pass.scatter;
+  const bool suspects = first && a.susp != nullptr;
   const double sc = ldexp(1.0, pass.s);
   double mk = 0.0;
   bool bad = false;
-  // reference gradients [nb][D][Q] (values [nb][Q] for mass) in LDS: each
-  // lane reads the table rows of its own trial functions
-  for (int k = threadIdx.x; k < TAB; k += NT) L.tab[k] = GRAD ? a.dphi[k] : a.phi[k];
+  // the form tables from the reference tables and weights: metric pair m =
+  // (a, a) for m < D, then (0,1), (0,2), (1,2) (the two orders summed)
+  for (int k = tid; k < NB * NB * NM + NB * NB; k += NT) {
+    double acc = 0.0;
+    if (k < NB * NB * NM) {
+      const int pr = k / NM, m = k - pr * NM;
+      const int i = pr / NB, j = pr - i * NB;
+      if constexpr (GRAD) {
+        const int ca = m < D ? m : (m == D ? 0 : (m == D + 1 ? 0 : 1));
+        const int cb = m < D ? m : (m == D ? 1 : 2);
+        for (int q = 0; q < Q; ++q) {
+          const double wi = a.W[q] * a.dphi[(i * D + ca) * Q + q];
+          acc += wi * a.dphi[(j * D + cb) * Q + q];
+          if (m >= D) acc += a.W[q] * a.dphi[(i * D + cb) * Q + q] * a.dphi[(j * D + ca) * Q + q];
+        }
+      } else {
+        for (int q = 0; q < Q; ++q) acc += a.W[q] * a.phi[i * Q + q] * a.phi[j * Q + q];
+      }
+      T[k] = acc;
+    } else {
+      const int pr = k - NB * NB * NM;
+      const int i = pr / NB, j = pr - i * NB;
+      for (int q = 0; q < Q; ++q) {
+        double ni = 0.0, nj = 0.0;
+        if constexpr (GRAD) {
+          for (int d = 0; d < D; ++d) {
+            ni += fabs(a.dphi[(i * D + d) * Q + q]);
+            nj += fabs(a.dphi[(j * D + d) * Q + q]);
+          }
+        } else {
+          ni = fabs(a.phi[i * Q + q]);
+          nj = fabs(a.phi[j * Q + q]);
+        }
+        acc += fabs(a.W[q]) * ni * nj;
+      }
+      B[pr] = acc;
+    }
+  }
 
   for (int k = 0;; ++k) {
     const int part = next_partition(a.nparts, k);
     if (part < 0) break;
-    if (k > 0) __syncthreads();  // LDS reuse
+    __syncthreads();  // the tables (k = 0), LDS reuse (k > 0)
     const int32_t* hdr = a.parts + 8 * (int64_t)part;
-    const int node_start = hdr[0], Ln = hdr[1], row_start = hdr[2], R = hdr[3];
+    const int row_start = hdr[2], R = hdr[3];
     const int nslots = hdr[4], rr_start = hdr[5], nrr = hdr[6];
 
     for (int r = tid; r < R; r += NT) {
@@ -132,10 +201,7 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
       L.rstart[r] = rw.x;
       L.rinfo[r] = (uint32_t)rw.y;
     }
-    for (int i = tid; i < nslots; i += NT) {
-      L.acc[i] = 0ull;
-      L.nz[i] = 0;
-    }
+    for (int i = tid; i < nslots; i += NT) L.acc[i] = 0ull;
     __syncthreads();
 
     for (int c0 = 0; c0 < nrr; c0 += NT) {
@@ -153,76 +219,81 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
         auto lid_of = [&](int j) -> int {
           return (int)((sel5(j >> 1, w0, w1, w2, w3, w4) >> ((j & 1) * 16)) & 0xffffu);
         };
-        auto field = [&](int k) -> int {
-          return (int)((sel5(k >> 2, o0, o1, o2, o3, o4) >> (7 * (k & 3))) & 127u);
+        auto field = [&](int kk) -> int {
+          return (int)((sel5(kk >> 2, o0, o1, o2, o3, o4) >> (7 * (kk & 3))) & 127u);
         };
         const int64_t e = a.rec_elem[v];
         exact::MapAt<D, true, Q> m;
         m.load(a.G, a.E, e, 0);
-        double vi[Q][GRAD ? D : 1];
+        // the metric and the suspect scale of the element
+        double M[NM];
+        double tsc;
+        if constexpr (GRAD) {
+          double mi = 0.0;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          if constexpr (GRAD)
-            exact::global_grad<D>(m.inv, L.tab, i, q, Q, vi[q]);
-          else
-            vi[q][0] = L.tab[i * Q + q];
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < D; ++c) mi = max_abs(mi, m.inv[r][c]);
+#pragma unroll
+          for (int mm = 0; mm < NM; ++mm) {
+            const int ca = mm < D ? mm : (mm == D ? 0 : (mm == D + 1 ? 0 : 1));
+            const int cb = mm < D ? mm : (mm == D ? 1 : 2);
+            double s = m.inv[ca][0] * m.inv[cb][0];
+#pragma unroll
+            for (int c = 1; c < D; ++c) s = __fma_rn(m.inv[ca][c], m.inv[cb][c], s);
+            M[mm] = m.adet * s;
+          }
+          tsc = (0x1p-44 * D) * (mi * mi) * m.adet;
+        } else {
+          M[0] = m.adet;
+          tsc = 0x1p-44 * m.adet;
         }
+        bad = bad || !(tsc < INFINITY);
         // the local row of DOF i, from the top nibbles of offset words 0-3
         const int ri = (int)((o0 >> 28) | ((o1 >> 28) << 4) | ((o2 >> 28) << 8) |
                              ((o3 >> 28) << 12));
         const int basei = (int)(L.rinfo[ri] & 0xffffu);   // an owned row
         // the trial functions this lane computes: j >= i when both DOFs are
         // owned here (the pair is added to both rows), every j whose DOF is
-        // owned elsewhere.  The records are sorted by (i, this set), so a
-        // wave's lanes mostly share it and the loop index stays uniform
-        // (the reference gradients of j are read as scalars).
+        // owned elsewhere
         uint32_t need = 0;
 #pragma unroll
         for (int j = 0; j < NB; ++j)
           if (lid_of(j) >= R || j >= i) need |= 1u << j;
+        uint32_t s1 = 0, s2 = 0;   // suspects (row i, col j) and (row j, col i)
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
           if (!((need >> j) & 1u)) continue;
           const int rj = lid_of(j);
           const bool ownj = rj < R;
-          auto term = [&](int q) -> double {
-#pragma clang fp contract(off)
-            const double dx = m.adet * a.W[q];
-            double val;
-            if constexpr (GRAD) {
-              double gj[D];
-              exact::global_grad<D>(m.inv, a.dphi, j, q, Q, gj);
-              double s = 0.0;
+          const double* tij = T + (i * NB + j) * NM;
+          double val = M[0] * tij[0];
 #pragma unroll
-              for (int d = 0; d < D; ++d) s = s + gj[d] * vi[q][d];
-              val = s;
-            } else {
-              val = a.phi[j * Q + q] * vi[q][0];
-            }
-            return val * dx;
-          };
-          const double r = 0.0 + np_pairwise<0, Q>(term);
-          mk = max_abs(mk, r);
-          if (r == 0.0) continue;   // dropped by eliminate_zeros
-          const unsigned long long fx = to_fixed(r, sc);
+          for (int mm = 1; mm < NM; ++mm) val = __fma_rn(M[mm], tij[mm], val);
+          mk = max_abs(mk, val);
+          const bool sus = !(fabs(val) > tsc * B[i * NB + j]);
+          const bool two = ownj && j != i;
+          if (sus) {
+            s1 |= 1u << j;
+            if (two) s2 |= 1u << j;
+          }
+          const unsigned long long fx = to_fixed(val, sc);
           const int oij = field(j);
           if (oij == 127) {
-            bad = true;
+            bad = bad || !sus;   // a nonzero entry outside the pattern
           } else if (scatter) {
             lds_add(L.acc + basei + oij, fx);
-            L.nz[basei + oij] = 1;
           }
-          if (ownj && j != i) {
+          if (two) {
             const int oji = field(NB + j);
             if (oji == 127) {
-              bad = true;
+              bad = bad || !sus;
             } else if (scatter) {
-              const int sji = (int)(L.rinfo[rj] & 0xffffu) + oji;
-              lds_add(L.acc + sji, fx);
-              L.nz[sji] = 1;
+              lds_add(L.acc + (int)(L.rinfo[rj] & 0xffffu) + oji, fx);
             }
           }
         }
+        if (suspects && (s1 | s2)) push_row_suspects(a, e, i, s1, s2);
       }
     }
     if (!scatter) continue;
@@ -233,10 +304,8 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
       const uint32_t info = L.rinfo[r];
       const int base = (int)(info & 0xffffu), len = (int)((info >> 24) & 0x7fu);
       const int64_t gs = L.rstart[r];
-      for (int q = lane; q < len; q += 16) {
-        bad = bad || L.nz[base + q] == 0;   // a slot the reference would prune
+      for (int q = lane; q < len; q += 16)
         __builtin_nontemporal_store(from_fixed((long long)L.acc[base + q], inv), a.out + gs + q);
-      }
     }
   }  // partitions
 
@@ -248,10 +317,10 @@ template <int D, int NB, int Q>
 int launch_rows(const RowsArgs& a, int form, size_t lds, unsigned grid, hipStream_t s) {
   if (form == FEMHIP_LAPLACE)
     p2_fused_rows_kernel<D, NB, Q, FEMHIP_LAPLACE, kRowsThreads>
-        <<<grid, kRowsThreads, lds + 8 * NB * D * Q, s>>>(a);
+        <<<grid, kRowsThreads, lds + 8 * rows_tab_doubles(D, NB, true), s>>>(a);
   else
     p2_fused_rows_kernel<D, NB, Q, FEMHIP_MASS, kRowsThreads>
-        <<<grid, kRowsThreads, lds + 8 * NB * Q, s>>>(a);
+        <<<grid, kRowsThreads, lds + 8 * rows_tab_doubles(D, NB, false), s>>>(a);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }
@@ -276,8 +345,9 @@ extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const 
                  "rows fused: state must be 128-byte aligned");
   FEMHIP_REQUIRE(form == FEMHIP_LAPLACE || form == FEMHIP_MASS, FEMHIP_EUNSUPPORTED,
                  "rows fused: laplace or mass only");
-  FEMHIP_REQUIRE(check == nullptr || check->verdict != nullptr, FEMHIP_EINVAL,
-                 "rows fused: the check needs a verdict output");
+  FEMHIP_REQUIRE(check == nullptr || (check->verdict != nullptr && check->zero_mask != nullptr &&
+                                        check->queue != nullptr && check->capacity > 0),
+                 FEMHIP_EINVAL, "rows fused: the check needs a verdict, a zero mask and a queue");
   const femhip_fused_plan_info_t& I = plan->info;
   FEMHIP_REQUIRE(I.idbits == 16 && I.offbits == 0, FEMHIP_EINVAL,
                  "rows fused: not a rows plan");
@@ -286,12 +356,13 @@ extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const 
   FEMHIP_REQUIRE(tet || tri, FEMHIP_EUNSUPPORTED, "rows fused: P2 triangles or tetrahedra");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
   if (I.nparts > 0) {
-    RowsArgs a{plan->parts, plan->nodemap, plan->rows,  plan->recs,  plan->rec_elem,
-               rowrecs,     G,             g->W,        sp->phi,     sp->dphi,
-               data,        state,         g->nelems,   I.nparts,    I.max_slots,
-               I.max_nodes, I.max_rows,    0};
-    const size_t lds = rows_lds_bytes(I);   // + the reference tables (<= 2,640 B)
-    FEMHIP_REQUIRE(lds + 8 * 10 * 3 * 11 <= 160 * 1024, FEMHIP_EINVAL,
+    RowsArgs a{plan->parts,  plan->nodemap, plan->rows,  plan->recs,  plan->rec_elem,
+               rowrecs,      G,             g->W,        sp->phi,     sp->dphi,
+               data,         state,
+               check ? check->queue : nullptr, check ? check->capacity : 0,
+               g->nelems,    I.nparts,      I.max_slots, I.max_nodes, I.max_rows,    0};
+    const size_t lds = rows_lds_bytes(I);   // + the form tables (<= 5,600 B)
+    FEMHIP_REQUIRE(lds + 8 * rows_tab_doubles(3, 10, true) <= 160 * 1024, FEMHIP_EINVAL,
                    "rows fused: plan exceeds LDS");
     for (int mode = 0; mode < 2; ++mode) {
       a.mode = mode;
@@ -306,6 +377,9 @@ extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const 
       if (rc != FEMHIP_OK) return rc;
     }
   }
-  if (check) return fused_check_empty(check, state, st);
+  if (check) {
+    if (I.nparts == 0) return fused_check_empty(check, state, st);
+    return rows_check_run(check, state, form, g, sp, st);
+  }
   return FEMHIP_OK;
 }

@@ -410,6 +410,10 @@ class DeviceBasis:
                           device=self.p.device)[:plan.nnz]
         fp.assemble(name, self, out)
         v = fp.verdict.cpu().tolist()      # synchronises
+        if v[0] != 1 and (v[3] & _lib.FUSED_OVERFLOW):
+            fp.grow_queue()                # more entries near zero than queued
+            fp.assemble(name, self, out)
+            v = fp.verdict.cpu().tolist()
         self.last_verdict = v
         if v[0] != 1:
             return None
@@ -430,6 +434,7 @@ class DeviceBasis:
             except (NotImplementedError, ValueError, OverflowError) as exc:
                 self.unfusable[plan.signature] = str(exc)
                 return None
+            fp.set_pattern_zeros(plan.mask, plan.nzeros)
             self.rows_plans[plan.signature] = fp
         self.rows_last[name] = plan
         return fp

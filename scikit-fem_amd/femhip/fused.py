@@ -208,10 +208,10 @@ class FusedPlan:
         return out
 
 
-# Rows plans (P2 simplices): one 512-thread workgroup per CU (the entry
-# arithmetic holds ~250 registers, two waves per SIMD), so a partition may
-# use most of the LDS: 16,000 slots x 9 B (accumulator, nonzero flag)
-ROWS_SLOT_CAP = int(os.environ.get('FEMHIP_ROWS_SLOT_CAP', 16000))
+# Rows plans (P2 simplices): 9,000 slots x 8 B of accumulators (72 KB of
+# LDS, two 512-thread workgroups per CU) measured faster than 16,000 (one
+# per CU, less halo) and 6,000 (three per CU, more halo) on 41^3 P2 tets
+ROWS_SLOT_CAP = int(os.environ.get('FEMHIP_ROWS_SLOT_CAP', 9000))
 ROWS_NODE_CAP = int(os.environ.get('FEMHIP_ROWS_NODE_CAP', 16384))
 _ROWS_FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS}
 ROW_RECORD_WORDS = 6   # (visit << 4) | DOF, then 2 x 10 seven-bit column offsets
@@ -282,8 +282,25 @@ class RowsFusedPlan:
                                  self.rec_elem.data_ptr(), self.info)
         self.states = {}
         self.verdict = torch.zeros(4, dtype=torch.int32, device=dev)
-        self.check = _lib.FusedCheck(None, None, None, 0, 0, 0, None, 0, None, 0,
+        self.nelems = int(ed.shape[1])
+        self.mask, self.nzeros, self.queue, self.check = None, 0, None, None
+
+    def set_pattern_zeros(self, mask, nzeros, capacity=None):
+        """The validated pattern's exact-zero mask (one bit per contribution,
+        ``femhip_zero_mask``) and count, and a suspect queue of ``capacity``
+        entries (default: 12 per element -- the structural zeros of P2
+        triangles' mass, both orders -- or twice the pattern's zeros)."""
+        if capacity is None:
+            capacity = max(1 << 17, 12 * self.nelems, 2 * int(nzeros))
+        self.mask, self.nzeros = mask, int(nzeros)
+        self.queue = torch.empty(int(capacity), dtype=torch.int64, device=self.parts.device)
+        self.check = _lib.FusedCheck(None, None, None, self.nelems, 0, 0, mask.data_ptr(),
+                                     self.nzeros, self.queue.data_ptr(), self.queue.numel(),
                                      self.verdict.data_ptr())
+
+    def grow_queue(self):
+        """A queue four times as long (after a call that overflowed it)."""
+        self.set_pattern_zeros(self.mask, self.nzeros, 4 * self.queue.numel())
 
     def state(self, name):
         st = self.states.get(name)
@@ -304,9 +321,10 @@ class RowsFusedPlan:
     def assemble(self, name, d, out):
         """out[nnz] <- CSR values of ``name`` on the basis ``d``
         (DeviceBasis) for its current coordinates; the call is checked
-        against the plan's pattern, verdict {valid, 0, 0, flags} left in
-        ``self.verdict`` on the device."""
+        against the plan's pattern (``set_pattern_zeros``), verdict {valid,
+        mismatches, zeros, flags} left in ``self.verdict`` on the device."""
         assert out.numel() == self.nnz and out.dtype == torch.float64
+        assert self.check is not None, "set_pattern_zeros first"
         G = d.geometry()
         _lib.check(_lib.load().femhip_rows_fused_assemble(
             ctypes.byref(self.dev), _lib.ptr(self.rowrecs), _lib.ptr(self.state(name)),
