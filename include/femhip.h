@@ -300,7 +300,8 @@ void femhip_fused_plan_free(void* handle);
 /* Rows plan of a P2 simplex mesh (nverts 6 / 10 DOFs per element, `p` the
  * DOF locations used only for the partition): the same partitions, node
  * maps and row tables as a fused plan of the DOF graph, records holding the
- * element's local DOF ids (16 bits each) and one row record per (visit,
+ * element's local DOF ids (16 bits each) followed by the element id (one
+ * word; info.rec_words counts it) and one row record per (visit,
  * owned DOF) of six words -- (visit << 4) | local DOF, then 7-bit column
  * offsets four per word: of the element's DOFs in the DOF's row, and of the
  * DOF in the rows of the element's other owned DOFs (127: not in the
@@ -429,9 +430,19 @@ int femhip_fused_assemble_static(const femhip_fused_dev_t* plan, int32_t form, i
 /* LDS bytes per workgroup that femhip_rows_fused_assemble asks for. */
 size_t femhip_rows_fused_lds_bytes(const femhip_fused_plan_info_t* info);
 
+/* Element metrics for femhip_rows_fused_assemble: metric[8 e + k] for the
+ * affine simplex geometry g and its femhip_geometry G -- |det| A^-1 A^-T
+ * (entries (0,0), (1,1), (2,2), (0,1), (0,2), (1,2); 2-D: (0,0), (1,1),
+ * (0,1)) in k < D(D+1)/2, the Laplace suspect scale 2^-44 D max|A^-1|^2
+ * |det| in k = 6 and |det| in k = 7.  metric: 8 * nelems doubles, 16-byte
+ * aligned.  Part of MappingAffine (mapping_affine.py:55-131,205-232) as the
+ * fused P2 kernel uses it. */
+int femhip_rows_metric(const femhip_geom_t* g, const double* G, double* metric,
+                       femhip_stream_t s);
+
 /* data[nnz] <- CSR values of `form` (FEMHIP_LAPLACE or FEMHIP_MASS) for a
- * P2 simplex basis (g, sp as for femhip_local_bilinear; G its
- * femhip_geometry) on the pattern the rows plan was built for
+ * P2 simplex basis (g, sp as for femhip_local_bilinear; metric its
+ * femhip_rows_metric) on the pattern the rows plan was built for
  * (BilinearForm._assemble + COOData._assemble_scipy_csr,
  * bilinear_form.py:58-148, coo_data.py:28-36): every entry from the
  * element metric |det| A^-1 A^-T and tables of the form, summed in LDS rows,
@@ -447,7 +458,7 @@ size_t femhip_rows_fused_lds_bytes(const femhip_fused_plan_info_t* info);
 int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const uint32_t* rowrecs,
                                femhip_fused_state_t* state, int32_t form,
                                const femhip_geom_t* g, const femhip_space_t* sp,
-                               const double* G, double* data,
+                               const double* metric, double* data,
                                const femhip_fused_check_t* check, femhip_stream_t s);
 
 /* out[S][nseg] <- kappa-weighted segmented sums  sum_k kappa[s][c_k % E] *

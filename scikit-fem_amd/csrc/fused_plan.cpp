@@ -399,7 +399,9 @@ int build_impl(int mode, int32_t nverts, int32_t dim, int64_t E, int64_t Nv, con
           O.rowkeys.push_back(((uint32_t)a << 16) | need);
           O.rowrecs.insert(O.rowrecs.end(), rw, rw + kRowRecWords);
         }
+        // rows plans: the element id follows the local ids in the record
         for (int k = 0; k < rec_words; ++k) O.recs.push_back(w[k]);
+        O.recs.push_back((uint32_t)e);
         O.rec_elem.push_back(e);
         ++visit;
         continue;
@@ -521,7 +523,7 @@ int build_impl(int mode, int32_t nverts, int32_t dim, int64_t E, int64_t Nv, con
   I.max_slots = max_slots;
   I.max_nodes = max_nodes;
   I.max_rows = max_rows;
-  I.rec_words = rec_words;
+  I.rec_words = mode == 1 ? rec_words + 1 : rec_words;
   I.idbits = idbits;
   I.offbits = offbits;
   I.block = block;

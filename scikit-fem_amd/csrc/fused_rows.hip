@@ -45,7 +45,7 @@ struct RowsArgs {
   const uint32_t* recs;
   const int32_t* rec_elem;
   const uint32_t* rowrecs;  // kRowRecWords words per row record
-  const double* G;          // affine mapping per element (exact order)
+  const double* metric;     // [E][kMetric] (femhip_rows_metric)
   const double* W;
   const double* phi;        // [nb][Q]
   const double* dphi;       // [nb][D][Q]
@@ -62,6 +62,10 @@ struct RowsArgs {
 };
 
 constexpr int kRowsThreads = 512;
+// per element: the metric |det| A^-1 A^-T (pairs (a, a), then (0,1), (0,2),
+// (1,2)) in [0, D(D+1)/2), the Laplace suspect scale 2^-44 D max|A^-1|^2
+// |det| in [6] and |det| in [7]
+constexpr int kMetric = 8;
 constexpr int kRowRecWords = 6;   // visit << 4 | DOF, 2 x 10 seven-bit offsets
 
 // LDS: [form tables (TAB doubles: T then B) | accumulators (max_slots
@@ -133,7 +137,8 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
   constexpr bool GRAD = FORM == FEMHIP_LAPLACE;
   constexpr int NM = GRAD ? D * (D + 1) / 2 : 1;   // metric entries per element
   constexpr int RW = (NB * 16 + 31) / 32;
-  static_assert(RW <= 5 && kRowRecWords == 6, "record words");
+  constexpr int RV = RW + 1;   // visit record: local ids, then the element id
+  static_assert((RV == 4 || RV == 6) && kRowRecWords == 6, "record words");
   constexpr int TAB = rows_tab_doubles(D, NB, GRAD);
   extern __shared__ double smem[];
   RowsLds<NT, TAB> L(smem, a);
@@ -204,16 +209,72 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
     for (int i = tid; i < nslots; i += NT) L.acc[i] = 0ull;
     __syncthreads();
 
+    // row records in chunks of NT, software-pipelined: the next chunk's
+    // records and their visits load while this chunk computes (the index
+    // of a lane past the end is clamped to the last record, so every lane
+    // issues the same loads)
+    struct RecIn {
+      uint32_t rr, o0, o1, o2, o3, o4;   // row record
+      uint32_t w0, w1, w2, w3, w4;       // its visit's local DOF ids
+      int64_t e;                         // and element
+    };
+    auto load_rec = [&](int c0, RecIn& x) {
+      const int r = c0 + tid < nrr ? c0 + tid : nrr - 1;
+      const uint2* rp =
+          reinterpret_cast<const uint2*>(a.rowrecs + (int64_t)(rr_start + r) * kRowRecWords);
+      const uint2 p0 = rp[0], p1 = rp[1], p2 = rp[2];
+      x.rr = p0.x;
+      x.o0 = p0.y;
+      x.o1 = p1.x;
+      x.o2 = p1.y;
+      x.o3 = p2.x;
+      x.o4 = p2.y;
+    };
+    auto load_visit = [&](RecIn& x) {
+      const int64_t v = x.rr >> 4;
+      if constexpr (RV == 4) {
+        const uint4 q = reinterpret_cast<const uint4*>(a.recs)[v];
+        x.w0 = q.x;
+        x.w1 = q.y;
+        x.w2 = q.z;
+        x.w3 = 0u;
+        x.w4 = 0u;
+        x.e = q.w;
+      } else {
+        const uint2* vp = reinterpret_cast<const uint2*>(a.recs + v * RV);
+        const uint2 q0 = vp[0], q1 = vp[1], q2 = vp[2];
+        x.w0 = q0.x;
+        x.w1 = q0.y;
+        x.w2 = q1.x;
+        x.w3 = q1.y;
+        x.w4 = q2.x;
+        x.e = q2.y;
+      }
+    };
+    RecIn cur{}, nxt{};
+    if (nrr > 0) {
+      load_rec(0, cur);
+      load_visit(cur);
+    }
     for (int c0 = 0; c0 < nrr; c0 += NT) {
+      // this chunk's element metrics (4 x 16 B per lane)
+      const double2* mp = reinterpret_cast<const double2*>(a.metric + kMetric * cur.e);
+      double2 q0 = make_double2(0.0, 0.0), q1 = q0, q2 = q0;
+      if constexpr (GRAD) {
+        q0 = mp[0];
+        q1 = mp[1];
+        if constexpr (D == 3) q2 = mp[2];
+      }
+      const double2 q3 = mp[3];
+      if (c0 + NT < nrr) {   // uniform
+        load_rec(c0 + NT, nxt);
+        load_visit(nxt);
+      }
       if (c0 + tid < nrr) {
-        const uint32_t* rp = a.rowrecs + (int64_t)(rr_start + c0 + tid) * kRowRecWords;
-        const uint32_t rr = rp[0];
-        const uint32_t o0 = rp[1], o1 = rp[2], o2 = rp[3], o3 = rp[4], o4 = rp[5];
-        const int64_t v = rr >> 4;
+        const uint32_t rr = cur.rr;
+        const uint32_t o0 = cur.o0, o1 = cur.o1, o2 = cur.o2, o3 = cur.o3, o4 = cur.o4;
         const int i = (int)(rr & 15u);
-        const uint32_t* vp = a.recs + v * RW;
-        const uint32_t w0 = vp[0], w1 = vp[1], w2 = RW > 2 ? vp[2] : 0u,
-                       w3 = RW > 3 ? vp[3] : 0u, w4 = RW > 4 ? vp[4] : 0u;
+        const uint32_t w0 = cur.w0, w1 = cur.w1, w2 = cur.w2, w3 = cur.w3, w4 = cur.w4;
         // the element's DOF j: local id (16-bit fields) and the 7-bit offset
         // fields of the row record (four per word)
         auto lid_of = [&](int j) -> int {
@@ -222,31 +283,23 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
         auto field = [&](int kk) -> int {
           return (int)((sel5(kk >> 2, o0, o1, o2, o3, o4) >> (7 * (kk & 3))) & 127u);
         };
-        const int64_t e = a.rec_elem[v];
-        exact::MapAt<D, true, Q> m;
-        m.load(a.G, a.E, e, 0);
+        const int64_t e = cur.e;
         // the metric and the suspect scale of the element
         double M[NM];
         double tsc;
         if constexpr (GRAD) {
-          double mi = 0.0;
-#pragma unroll
-          for (int r = 0; r < D; ++r)
-#pragma unroll
-            for (int c = 0; c < D; ++c) mi = max_abs(mi, m.inv[r][c]);
-#pragma unroll
-          for (int mm = 0; mm < NM; ++mm) {
-            const int ca = mm < D ? mm : (mm == D ? 0 : (mm == D + 1 ? 0 : 1));
-            const int cb = mm < D ? mm : (mm == D ? 1 : 2);
-            double s = m.inv[ca][0] * m.inv[cb][0];
-#pragma unroll
-            for (int c = 1; c < D; ++c) s = __fma_rn(m.inv[ca][c], m.inv[cb][c], s);
-            M[mm] = m.adet * s;
+          M[0] = q0.x;
+          M[1] = q0.y;
+          M[2] = q1.x;
+          if constexpr (D == 3) {
+            M[3] = q1.y;
+            M[4] = q2.x;
+            M[5] = q2.y;
           }
-          tsc = (0x1p-44 * D) * (mi * mi) * m.adet;
+          tsc = q3.x;
         } else {
-          M[0] = m.adet;
-          tsc = 0x1p-44 * m.adet;
+          M[0] = q3.y;
+          tsc = 0x1p-44 * q3.y;
         }
         bad = bad || !(tsc < INFINITY);
         // the local row of DOF i, from the top nibbles of offset words 0-3
@@ -295,6 +348,7 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
         }
         if (suspects && (s1 | s2)) push_row_suspects(a, e, i, s1, s2);
       }
+      cur = nxt;
     }
     if (!scatter) continue;
     __syncthreads();  // every contribution has landed
@@ -311,6 +365,38 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
 
   if (__any(bad) && (tid & 63) == 0) atomicOr(&st->flags, FEMHIP_FUSED_BAD);
   if (first) fx_end<NT, false>(st, mk, 0.0, false, false, L.red);
+}
+
+// Per element e of an affine simplex basis: kMetric doubles (see kMetric)
+// from the mapping G (femhip_geometry: A^-1 row-major, then |det|).
+template <int D>
+__global__ void __launch_bounds__(256) rows_metric_kernel(const double* __restrict__ G,
+                                                          int64_t E, double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= E) return;
+  exact::MapAt<D, true, 1> m;
+  m.load(G, E, e, 0);
+  double o[kMetric] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double mi = 0.0;
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int c = 0; c < D; ++c) mi = max_abs(mi, m.inv[r][c]);
+  constexpr int NM = D * (D + 1) / 2;
+#pragma unroll
+  for (int mm = 0; mm < NM; ++mm) {
+    const int ca = mm < D ? mm : (mm == D ? 0 : (mm == D + 1 ? 0 : 1));
+    const int cb = mm < D ? mm : (mm == D ? 1 : 2);
+    double s = m.inv[ca][0] * m.inv[cb][0];
+#pragma unroll
+    for (int c = 1; c < D; ++c) s = __fma_rn(m.inv[ca][c], m.inv[cb][c], s);
+    o[mm] = m.adet * s;
+  }
+  o[6] = (0x1p-44 * D) * (mi * mi) * m.adet;
+  o[7] = m.adet;
+  double2* dst = reinterpret_cast<double2*>(out + kMetric * e);
+#pragma unroll
+  for (int k = 0; k < kMetric / 2; ++k) dst[k] = make_double2(o[2 * k], o[2 * k + 1]);
 }
 
 template <int D, int NB, int Q>
@@ -334,13 +420,32 @@ extern "C" size_t femhip_rows_fused_lds_bytes(const femhip_fused_plan_info_t* I)
   return I ? rows_lds_bytes(*I) : 0;
 }
 
+extern "C" int femhip_rows_metric(const femhip_geom_t* g, const double* G, double* metric,
+                                  femhip_stream_t s) {
+  FEMHIP_REQUIRE(g && (g->kind == FEMHIP_TRI || g->kind == FEMHIP_TET) && g->nelems >= 0,
+                 FEMHIP_EINVAL, "rows metric: affine simplex geometry required");
+  if (g->nelems == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(G && metric && reinterpret_cast<uintptr_t>(metric) % 16 == 0, FEMHIP_EINVAL,
+                 "rows metric: NULL or misaligned buffer");
+  const unsigned grid = (unsigned)ceil_div(g->nelems, 256);
+  hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  if (g->dim == 3)
+    rows_metric_kernel<3><<<grid, 256, 0, st>>>(G, g->nelems, metric);
+  else
+    rows_metric_kernel<2><<<grid, 256, 0, st>>>(G, g->nelems, metric);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
 extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const uint32_t* rowrecs,
                                           femhip_fused_state_t* state, int32_t form,
                                           const femhip_geom_t* g, const femhip_space_t* sp,
-                                          const double* G, double* data,
+                                          const double* metric, double* data,
                                           const femhip_fused_check_t* check, femhip_stream_t s) {
-  FEMHIP_REQUIRE(plan && rowrecs && state && g && sp && G && data, FEMHIP_EINVAL,
+  FEMHIP_REQUIRE(plan && rowrecs && state && g && sp && metric && data, FEMHIP_EINVAL,
                  "rows fused: NULL argument");
+  FEMHIP_REQUIRE(reinterpret_cast<uintptr_t>(metric) % 16 == 0, FEMHIP_EINVAL,
+                 "rows fused: metric must be 16-byte aligned");
   FEMHIP_REQUIRE(reinterpret_cast<uintptr_t>(state) % 128 == 0, FEMHIP_EINVAL,
                  "rows fused: state must be 128-byte aligned");
   FEMHIP_REQUIRE(form == FEMHIP_LAPLACE || form == FEMHIP_MASS, FEMHIP_EUNSUPPORTED,
@@ -349,15 +454,16 @@ extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const 
                                         check->queue != nullptr && check->capacity > 0),
                  FEMHIP_EINVAL, "rows fused: the check needs a verdict, a zero mask and a queue");
   const femhip_fused_plan_info_t& I = plan->info;
-  FEMHIP_REQUIRE(I.idbits == 16 && I.offbits == 0, FEMHIP_EINVAL,
-                 "rows fused: not a rows plan");
+  FEMHIP_REQUIRE(I.idbits == 16 && I.offbits == 0 &&
+                     I.rec_words == (sp->nbs * 16 + 31) / 32 + 1,
+                 FEMHIP_EINVAL, "rows fused: not a rows plan of this element");
   const bool tet = g->kind == FEMHIP_TET && g->dim == 3 && sp->nbs == 10 && !sp->vector;
   const bool tri = g->kind == FEMHIP_TRI && g->dim == 2 && sp->nbs == 6 && !sp->vector;
   FEMHIP_REQUIRE(tet || tri, FEMHIP_EUNSUPPORTED, "rows fused: P2 triangles or tetrahedra");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
   if (I.nparts > 0) {
     RowsArgs a{plan->parts,  plan->nodemap, plan->rows,  plan->recs,  plan->rec_elem,
-               rowrecs,      G,             g->W,        sp->phi,     sp->dphi,
+               rowrecs,      metric,        g->W,        sp->phi,     sp->dphi,
                data,         state,
                check ? check->queue : nullptr, check ? check->capacity : 0,
                g->nelems,    I.nparts,      I.max_slots, I.max_nodes, I.max_rows,    0};

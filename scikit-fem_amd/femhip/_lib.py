@@ -137,6 +137,7 @@ SIGNATURES = {
                                                     ctypes.POINTER(FusedInfo)]),
     'femhip_rows_fused_plan_export': (ctypes.c_int, [ctypes.c_void_p, _p]),
     'femhip_rows_fused_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(FusedInfo)]),
+    'femhip_rows_metric': (ctypes.c_int, [ctypes.POINTER(Geom), _p, _p, _p]),
     'femhip_rows_fused_assemble': (ctypes.c_int, [ctypes.POINTER(FusedDev), _p, _p, _i32,
                                                   ctypes.POINTER(Geom), ctypes.POINTER(Space),
                                                   _p, _p, ctypes.POINTER(FusedCheck), _p]),

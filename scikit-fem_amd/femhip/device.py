@@ -148,6 +148,7 @@ class DeviceBasis:
                                 self.N, self.dofs.data_ptr(), self.phi.data_ptr(),
                                 self.dphi.data_ptr())
         self._G = None
+        self._metric = None
         self.p_version = 0
         self.plans = {}          # serial number -> Plan
         self.validated = {}      # (form, params, coefficient shape, p version) -> Plan
@@ -169,7 +170,19 @@ class DeviceBasis:
         self.p = p.contiguous()
         self.geom.p = self.p.data_ptr()
         self._G = None
+        self._metric = None
         self.p_version += 1
+
+    def rows_metric(self):
+        """Element metrics of the fused P2 kernel (``femhip_rows_metric``),
+        recomputed when the coordinates change."""
+        if self._metric is None:
+            G = self.geometry()
+            m = torch.empty(max(8 * self.E, 2), dtype=torch.float64, device=self.p.device)
+            _lib.check(self.lib.femhip_rows_metric(ctypes.byref(self.geom), _lib.ptr(G),
+                                                   _lib.ptr(m), _lib.stream_ptr()))
+            self._metric = m
+        return self._metric
 
     def geometry(self):
         if self._G is None:

@@ -325,10 +325,10 @@ class RowsFusedPlan:
         mismatches, zeros, flags} left in ``self.verdict`` on the device."""
         assert out.numel() == self.nnz and out.dtype == torch.float64
         assert self.check is not None, "set_pattern_zeros first"
-        G = d.geometry()
+        metric = d.rows_metric()
         _lib.check(_lib.load().femhip_rows_fused_assemble(
             ctypes.byref(self.dev), _lib.ptr(self.rowrecs), _lib.ptr(self.state(name)),
-            _ROWS_FORM[name], ctypes.byref(d.geom), ctypes.byref(d.space), _lib.ptr(G),
+            _ROWS_FORM[name], ctypes.byref(d.geom), ctypes.byref(d.space), _lib.ptr(metric),
             _lib.ptr(out), ctypes.byref(self.check),
             _lib.stream_ptr()))
         return out

@@ -102,6 +102,7 @@ def test_rows_plan_replay_gives_the_oracle_matrix(kind, n, a, form, caps):
             w = P['recs'][v * RW:(v + 1) * RW]
             lid = [int((w[j >> 1] >> ((j & 1) * 16)) & 0xffff) for j in range(nb)]
             e = P['rec_elem'][v]
+            assert w[RW - 1] == e                 # the element id closes the record
             assert all(lid[j] < Ln and gid[lid[j]] == ed[j, e] for j in range(nb))
             assert lid[i] < R and ri == lid[i]
             seen[i, e] += 1
