@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
       int64_t e;                         // and element
     };
     auto load_rec = [&](int c0, RecIn& x) {
-      const int r = c0 + tid < nrr ? c0 + tid : nrr - 1;
+      const int r = min(c0 + tid, nrr - 1);
       const uint2* rp =
           reinterpret_cast<const uint2*>(a.rowrecs + (int64_t)(rr_start + r) * kRowRecWords);
       const uint2 p0 = rp[0], p1 = rp[1], p2 = rp[2];
@@ -266,10 +266,10 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
         if constexpr (D == 3) q2 = mp[2];
       }
       const double2 q3 = mp[3];
-      if (c0 + NT < nrr) {   // uniform
-        load_rec(c0 + NT, nxt);
-        load_visit(nxt);
-      }
+      // the next chunk's loads are issued on every path (past the last
+      // chunk they reload the last record)
+      load_rec(c0 + NT, nxt);
+      load_visit(nxt);
       if (c0 + tid < nrr) {
         const uint32_t rr = cur.rr;
         const uint32_t o0 = cur.o0, o1 = cur.o1, o2 = cur.o2, o3 = cur.o3, o4 = cur.o4;
@@ -358,8 +358,10 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
       const uint32_t info = L.rinfo[r];
       const int base = (int)(info & 0xffffu), len = (int)((info >> 24) & 0x7fu);
       const int64_t gs = L.rstart[r];
+      // plain stores: the L2 merges the partial lines at row ends (P2 rows
+      // are not contiguous in the CSR; streaming stores wrote 1.5x the bytes)
       for (int q = lane; q < len; q += 16)
-        __builtin_nontemporal_store(from_fixed((long long)L.acc[base + q], inv), a.out + gs + q);
+        a.out[gs + q] = from_fixed((long long)L.acc[base + q], inv);
     }
   }  // partitions
 
