@@ -35,8 +35,10 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/femhip.h"
@@ -129,23 +131,62 @@ int build_impl(int mode, int32_t nverts, int32_t dim, int64_t E, int64_t Nv, con
   const int rec_words = (total_bits + 31) / 32;
   node_cap = std::min(node_cap, 1 << idbits);
 
-  // node -> element incidence
-  std::vector<int64_t> inc_ptr(Nv + 1, 0);
-  for (int64_t e = 0; e < E; ++e)
-    for (int k = 0; k < NV; ++k) {
-      const int32_t v = t[(int64_t)k * E + e];
-      if (v < 0 || v >= Nv) {
-        set_error("fused plan: connectivity index out of range");
-        return FEMHIP_EINVAL;
+  // host threads of every phase below, and fn(k, thread) for k in [0, n)
+  // on up to that many threads
+  const int nth = plan_threads(std::max<int64_t>(1, std::max<int64_t>(Nv, E) / 4096));
+  auto parallel_for = [&](size_t n, auto&& fn) {
+    std::atomic<size_t> next{0};
+    auto work = [&](int th) {
+      for (;;) {
+        const size_t k = next.fetch_add(1);
+        if (k >= n) break;
+        fn(k, th);
       }
-      inc_ptr[v + 1]++;
-    }
+    };
+    std::vector<std::thread> pool;
+    const int m = (int)std::min<size_t>((size_t)nth, n);
+    for (int th = 1; th < m; ++th) pool.emplace_back(work, th);
+    work(0);
+    for (auto& th : pool) th.join();
+  };
+  // element ranges of the parallel passes over the connectivity
+  const int64_t echunk = 1 << 16;
+  const size_t nechunks = (size_t)((E + echunk - 1) / echunk);
+
+  // node -> element incidence (counted and filled with atomic increments;
+  // the order of a node's elements is not used: every consumer sorts or
+  // only counts them)
+  std::vector<int64_t> inc_ptr(Nv + 1, 0);
+  std::atomic<bool> out_of_range{false};
+  parallel_for(nechunks, [&](size_t c, int) {
+    const int64_t e1 = std::min<int64_t>(E, (int64_t)(c + 1) * echunk);
+    for (int64_t e = (int64_t)c * echunk; e < e1; ++e)
+      for (int k = 0; k < NV; ++k) {
+        const int32_t v = t[(int64_t)k * E + e];
+        if (v < 0 || v >= Nv) {
+          out_of_range = true;
+          continue;
+        }
+        __atomic_fetch_add(&inc_ptr[v + 1], (int64_t)1, __ATOMIC_RELAXED);
+      }
+  });
+  if (out_of_range) {
+    set_error("fused plan: connectivity index out of range");
+    return FEMHIP_EINVAL;
+  }
   for (int64_t n = 0; n < Nv; ++n) inc_ptr[n + 1] += inc_ptr[n];
   std::vector<int32_t> inc(inc_ptr[Nv]);
   {
     std::vector<int64_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
-    for (int64_t e = 0; e < E; ++e)
-      for (int k = 0; k < NV; ++k) inc[fill[t[(int64_t)k * E + e]]++] = (int32_t)e;
+    parallel_for(nechunks, [&](size_t c, int) {
+      const int64_t e1 = std::min<int64_t>(E, (int64_t)(c + 1) * echunk);
+      for (int64_t e = (int64_t)c * echunk; e < e1; ++e)
+        for (int k = 0; k < NV; ++k) {
+          const int64_t pos =
+              __atomic_fetch_add(&fill[t[(int64_t)k * E + e]], (int64_t)1, __ATOMIC_RELAXED);
+          inc[pos] = (int32_t)e;
+        }
+    });
   }
 
   // 1. recursive coordinate bisection of the nodes into blocks whose CSR
@@ -155,69 +196,135 @@ int build_impl(int mode, int32_t nverts, int32_t dim, int64_t E, int64_t Nv, con
   std::vector<int32_t> order(Nv);
   for (int64_t n = 0; n < Nv; ++n) order[n] = (int32_t)n;
   std::vector<std::pair<int64_t, int64_t>> leaves;  // [begin, end) in `order`
-  std::vector<int64_t> cstamp(Nv, -1);
-  int64_t check_id = 0;
-  auto local_nodes = [&](int64_t b, int64_t e) {
-    const int64_t id = check_id++;
-    int64_t L = 0;
-    for (int64_t k = b; k < e; ++k) {
-      const int32_t n = order[k];
-      if (cstamp[n] != id) {
-        cstamp[n] = id;
-        ++L;
-      }
-      for (int64_t q = inc_ptr[n]; q < inc_ptr[n + 1]; ++q)
-        for (int j = 0; j < NV; ++j) {
-          const int32_t v = t[(int64_t)j * E + inc[q]];
-          if (cstamp[v] != id) {
-            cstamp[v] = id;
-            ++L;
-          }
+  // one split step of the range [b, e): a leaf when its rows fit the caps,
+  // else partitioned in `order` at the returned midpoint (-1: a leaf; -2: a
+  // single row that does not fit).  `stamp` / `check_id` are the caller's.
+  auto split = [&](int64_t b, int64_t e, std::vector<int64_t>& stamp,
+                   int64_t& check_id) -> int64_t {
+    auto local_nodes = [&]() {
+      const int64_t id = check_id++;
+      int64_t L = 0;
+      for (int64_t k = b; k < e; ++k) {
+        const int32_t n = order[k];
+        if (stamp[n] != id) {
+          stamp[n] = id;
+          ++L;
         }
+        for (int64_t q = inc_ptr[n]; q < inc_ptr[n + 1]; ++q)
+          for (int j = 0; j < NV; ++j) {
+            const int32_t v = t[(int64_t)j * E + inc[q]];
+            if (stamp[v] != id) {
+              stamp[v] = id;
+              ++L;
+            }
+          }
+      }
+      return L;
+    };
+    int64_t slots = 0;
+    for (int64_t k = b; k < e; ++k) slots += indptr[order[k] + 1] - indptr[order[k]] + 1;
+    int64_t L = -1;
+    if (slots <= slot_cap) {
+      L = local_nodes();
+      if (L <= node_cap) return -1;
     }
-    return L;
+    if (e - b <= 1) return -2;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t k = b; k < e; ++k)
+      for (int d = 0; d < dim; ++d) {
+        const double v = p[d * Nv + order[k]];
+        lo[d] = std::min(lo[d], v);
+        hi[d] = std::max(hi[d], v);
+      }
+    int ax = 0;
+    for (int d = 1; d < dim; ++d)
+      if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+    const int64_t nleaf = std::max<int64_t>(
+        2, std::max<int64_t>((slots + slot_cap - 1) / slot_cap,
+                             L > 0 ? (L + node_cap - 1) / node_cap : 2));
+    const int64_t mid = b + (int64_t)((double)(e - b) * (double)(nleaf / 2) / (double)nleaf);
+    std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e,
+                     [&](int32_t x, int32_t y) {
+                       const double px = p[ax * Nv + x], py = p[ax * Nv + y];
+                       return px < py || (px == py && x < y);
+                     });
+    return mid;
   };
-  {
-    std::vector<std::pair<int64_t, int64_t>> stack{{0, Nv}};
+  // depth-first, left part first, within [b, e): leaves come out spatially
+  // ordered
+  auto bisect = [&](int64_t b0, int64_t e0, std::vector<int64_t>& stamp, int64_t& check_id,
+                    std::vector<std::pair<int64_t, int64_t>>& out) -> bool {
+    std::vector<std::pair<int64_t, int64_t>> stack{{b0, e0}};
     while (!stack.empty()) {
       auto [b, e] = stack.back();
       stack.pop_back();
-      int64_t slots = 0;
-      for (int64_t k = b; k < e; ++k) slots += indptr[order[k] + 1] - indptr[order[k]] + 1;
-      int64_t L = -1;
-      if (slots <= slot_cap) {
-        L = local_nodes(b, e);
-        if (L <= node_cap) {
-          leaves.push_back({b, e});
-          continue;
+      const int64_t mid = split(b, e, stamp, check_id);
+      if (mid == -1) {
+        out.push_back({b, e});
+      } else if (mid == -2) {
+        return false;
+      } else {
+        stack.push_back({mid, e});
+        stack.push_back({b, mid});
+      }
+    }
+    return true;
+  };
+  {
+    // the top levels breadth first (the ranges of a level split in
+    // parallel) until there are enough independent ranges, then each range
+    // bisected depth first on a pool thread; the leaves are concatenated in
+    // range order, which is the depth-first order of one thread, so the
+    // plan does not depend on the threads
+    // per-thread stamp arrays (allocated on first use)
+    std::vector<std::vector<int64_t>> stamps(nth);
+    std::vector<int64_t> ids(nth, 0);
+    auto stamp_of = [&](int th) -> std::vector<int64_t>& {
+      if (stamps[th].empty()) stamps[th].assign(Nv, -1);
+      return stamps[th];
+    };
+    // each entry: a range and whether it is already a leaf
+    std::vector<std::tuple<int64_t, int64_t, bool>> ranges{{0, Nv, false}};
+    while (nth > 1 && (int64_t)ranges.size() < 8 * (int64_t)nth) {
+      std::vector<int64_t> mids(ranges.size(), -1);
+      parallel_for(ranges.size(), [&](size_t k, int th) {
+        const auto& [b, e, leaf] = ranges[k];
+        if (!leaf) mids[k] = split(b, e, stamp_of(th), ids[th]);
+      });
+      std::vector<std::tuple<int64_t, int64_t, bool>> next;
+      bool split_any = false;
+      for (size_t k = 0; k < ranges.size(); ++k) {
+        const auto& [b, e, leaf] = ranges[k];
+        if (mids[k] == -2) {
+          set_error("fused plan: one row does not fit the LDS budget");
+          return FEMHIP_EUNSUPPORTED;
+        }
+        if (leaf || mids[k] == -1) {
+          next.push_back({b, e, true});
+        } else {
+          next.push_back({b, mids[k], false});
+          next.push_back({mids[k], e, false});
+          split_any = true;
         }
       }
-      if (e - b <= 1) {
+      ranges.swap(next);
+      if (!split_any) break;
+    }
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> parts_out(ranges.size());
+    std::vector<char> ok(ranges.size(), 1);
+    parallel_for(ranges.size(), [&](size_t k, int th) {
+      const auto& [b, e, leaf] = ranges[k];
+      if (leaf)
+        parts_out[k].push_back({b, e});
+      else
+        ok[k] = bisect(b, e, stamp_of(th), ids[th], parts_out[k]) ? 1 : 0;
+    });
+    for (size_t k = 0; k < ranges.size(); ++k) {
+      if (!ok[k]) {
         set_error("fused plan: one row does not fit the LDS budget");
         return FEMHIP_EUNSUPPORTED;
       }
-      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-      for (int64_t k = b; k < e; ++k)
-        for (int d = 0; d < dim; ++d) {
-          const double v = p[d * Nv + order[k]];
-          lo[d] = std::min(lo[d], v);
-          hi[d] = std::max(hi[d], v);
-        }
-      int ax = 0;
-      for (int d = 1; d < dim; ++d)
-        if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
-      const int64_t nleaf = std::max<int64_t>(
-          2, std::max<int64_t>((slots + slot_cap - 1) / slot_cap,
-                               L > 0 ? (L + node_cap - 1) / node_cap : 2));
-      const int64_t mid = b + (int64_t)((double)(e - b) * (double)(nleaf / 2) / (double)nleaf);
-      std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e,
-                       [&](int32_t x, int32_t y) {
-                         const double px = p[ax * Nv + x], py = p[ax * Nv + y];
-                         return px < py || (px == py && x < y);
-                       });
-      // depth-first, left part first: leaves come out spatially ordered
-      stack.push_back({mid, e});
-      stack.push_back({b, mid});
+      leaves.insert(leaves.end(), parts_out[k].begin(), parts_out[k].end());
     }
   }
 
@@ -448,20 +555,11 @@ int build_impl(int mode, int32_t nverts, int32_t dim, int64_t E, int64_t Nv, con
     O.nelems = (int32_t)elems.size();
   };
   {
-    std::atomic<size_t> next{0};
-    auto worker = [&]() {
-      Scratch S(Nv, E);
-      for (;;) {
-        const size_t k = next.fetch_add(1);
-        if (k >= leaves.size()) break;
-        build_leaf(k, S, outs[k]);
-      }
-    };
-    const int nth = plan_threads((int64_t)leaves.size());
-    std::vector<std::thread> pool;
-    for (int k = 1; k < nth; ++k) pool.emplace_back(worker);
-    worker();
-    for (auto& th : pool) th.join();
+    std::vector<std::unique_ptr<Scratch>> scratch(nth);
+    parallel_for(leaves.size(), [&](size_t k, int th) {
+      if (!scratch[th]) scratch[th] = std::make_unique<Scratch>(Nv, E);
+      build_leaf(k, *scratch[th], outs[k]);
+    });
   }
   for (const LeafOut& O : outs)
     if (O.err) {
@@ -474,41 +572,73 @@ int build_impl(int mode, int32_t nverts, int32_t dim, int64_t E, int64_t Nv, con
   int max_slots = 0, max_nodes = 0, max_rows = 0, max_recs = 0;
   int64_t visits = 0;
   bool pruned = false;
-  for (size_t leaf = 0; leaf < outs.size(); ++leaf) {
-    LeafOut& O = outs[leaf];
+  // every block's offsets in the concatenated arrays, then the copies in
+  // parallel
+  const size_t nl = outs.size();
+  std::vector<int64_t> o_node(nl + 1, 0), o_rows(nl + 1, 0), o_recs(nl + 1, 0),
+      o_elem(nl + 1, 0), o_rr(nl + 1, 0);
+  for (size_t leaf = 0; leaf < nl; ++leaf) {
+    const LeafOut& O = outs[leaf];
+    o_node[leaf + 1] = o_node[leaf] + (int64_t)O.nodemap.size();
+    o_rows[leaf + 1] = o_rows[leaf] + (int64_t)O.rows.size();
+    o_recs[leaf + 1] = o_recs[leaf] + (int64_t)O.recs.size();
+    o_elem[leaf + 1] = o_elem[leaf] + (int64_t)O.rec_elem.size();
+    o_rr[leaf + 1] = o_rr[leaf] + (int64_t)O.rowrecs.size();
     pruned = pruned || O.pruned;
-    const int32_t node_start = (int32_t)P->nodemap.size();
-    const int32_t row_start = (int32_t)(P->rows.size() / 2);
-    const int32_t rec_start = (int32_t)P->rec_elem.size();
-    P->nodemap.insert(P->nodemap.end(), O.nodemap.begin(), O.nodemap.end());
-    P->rows.insert(P->rows.end(), O.rows.begin(), O.rows.end());
-    P->recs.insert(P->recs.end(), O.recs.begin(), O.recs.end());
-    P->rec_elem.insert(P->rec_elem.end(), O.rec_elem.begin(), O.rec_elem.end());
-    if (mode == 0) {
-      P->parts.insert(P->parts.end(),
-                      {node_start, O.L, row_start, O.R, O.lds, rec_start, O.nelems, 0});
-    } else {
-      // rows plans: [5] first row record, [6] row records, [7] first visit
-      const int32_t rr_start = (int32_t)(P->rowrecs.size() / kRowRecWords);
-      const size_t n = O.rowrecs.size() / kRowRecWords;
-      for (size_t k = 0; k < n; ++k) {
-        const uint32_t rr = O.rowrecs[k * kRowRecWords];
-        P->rowrecs.push_back((((rr >> 4) + (uint32_t)rec_start) << 4) | (rr & 15u));
-        P->rowrecs.insert(P->rowrecs.end(), O.rowrecs.begin() + k * kRowRecWords + 1,
-                          O.rowrecs.begin() + (k + 1) * kRowRecWords);
-      }
-      P->parts.insert(P->parts.end(), {node_start, O.L, row_start, O.R, O.lds, rr_start,
-                                       (int32_t)n, rec_start});
-      std::vector<uint32_t>().swap(O.rowrecs);
-    }
     max_slots = std::max(max_slots, O.lds);
     max_nodes = std::max(max_nodes, O.L);
     max_rows = std::max(max_rows, O.R);
     max_recs = std::max(max_recs, O.nelems);
     visits += O.nelems;
-    std::vector<int32_t>().swap(O.nodemap);  // release as we go
-    std::vector<uint32_t>().swap(O.recs);
   }
+  if (o_node[nl] >= INT32_MAX || o_rows[nl] / 2 >= INT32_MAX ||
+      o_rr[nl] / kRowRecWords >= INT32_MAX) {
+    delete P;
+    set_error("fused plan: more than 2^31-1 local nodes, rows or row records");
+    return FEMHIP_EOVERFLOW;
+  }
+  P->nodemap.resize(o_node[nl]);
+  P->rows.resize(o_rows[nl]);
+  P->recs.resize(o_recs[nl]);
+  P->rec_elem.resize(o_elem[nl]);
+  P->rowrecs.resize(o_rr[nl]);
+  P->parts.resize(8 * nl);
+  parallel_for(nl, [&](size_t leaf, int) {
+    LeafOut& O = outs[leaf];
+    const int32_t node_start = (int32_t)o_node[leaf];
+    const int32_t row_start = (int32_t)(o_rows[leaf] / 2);
+    const int32_t rec_start = (int32_t)o_elem[leaf];
+    std::copy(O.nodemap.begin(), O.nodemap.end(), P->nodemap.begin() + o_node[leaf]);
+    std::copy(O.rows.begin(), O.rows.end(), P->rows.begin() + o_rows[leaf]);
+    std::copy(O.recs.begin(), O.recs.end(), P->recs.begin() + o_recs[leaf]);
+    std::copy(O.rec_elem.begin(), O.rec_elem.end(), P->rec_elem.begin() + o_elem[leaf]);
+    int32_t* hdr = P->parts.data() + 8 * leaf;
+    if (mode == 0) {
+      const int32_t h[8] = {node_start, O.L, row_start, O.R, O.lds, rec_start, O.nelems, 0};
+      std::copy(h, h + 8, hdr);
+    } else {
+      // rows plans: [5] first row record, [6] row records, [7] first visit;
+      // the records' visit indices rebased to the concatenated visits
+      const int32_t rr_start = (int32_t)(o_rr[leaf] / kRowRecWords);
+      const size_t n = O.rowrecs.size() / kRowRecWords;
+      uint32_t* dst = P->rowrecs.data() + o_rr[leaf];
+      for (size_t k = 0; k < n; ++k) {
+        const uint32_t rr = O.rowrecs[k * kRowRecWords];
+        dst[k * kRowRecWords] = (((rr >> 4) + (uint32_t)rec_start) << 4) | (rr & 15u);
+        std::copy(O.rowrecs.begin() + k * kRowRecWords + 1,
+                  O.rowrecs.begin() + (k + 1) * kRowRecWords, dst + k * kRowRecWords + 1);
+      }
+      const int32_t h[8] = {node_start, O.L, row_start, O.R, O.lds, rr_start, (int32_t)n,
+                            rec_start};
+      std::copy(h, h + 8, hdr);
+    }
+    // release as we go
+    std::vector<int32_t>().swap(O.nodemap);
+    std::vector<int32_t>().swap(O.rows);
+    std::vector<uint32_t>().swap(O.recs);
+    std::vector<int32_t>().swap(O.rec_elem);
+    std::vector<uint32_t>().swap(O.rowrecs);
+  });
   if (visits >= INT32_MAX || (mode == 1 && visits >= (1ll << 28))) {
     delete P;
     set_error("fused plan: more than 2^31-1 element visits");
