@@ -440,9 +440,10 @@ size_t femhip_rows_fused_lds_bytes(const femhip_fused_plan_info_t* info);
 int femhip_rows_metric(const femhip_geom_t* g, const double* G, double* metric,
                        femhip_stream_t s);
 
-/* data[nnz] <- CSR values of `form` (FEMHIP_LAPLACE or FEMHIP_MASS) for a
- * P2 simplex basis (g, sp as for femhip_local_bilinear; metric its
- * femhip_rows_metric) on the pattern the rows plan was built for
+/* data[nnz] <- CSR values of `form` (FEMHIP_LAPLACE, FEMHIP_MASS, or
+ * FEMHIP_KAPPA_LAPLACE with kappa[nelems] per element; kappa NULL for the
+ * others) for a P2 simplex basis (g, sp as for femhip_local_bilinear;
+ * metric its femhip_rows_metric) on the pattern the rows plan was built for
  * (BilinearForm._assemble + COOData._assemble_scipy_csr,
  * bilinear_form.py:58-148, coo_data.py:28-36): every entry from the
  * element metric |det| A^-1 A^-T and tables of the form, summed in LDS rows,
@@ -458,7 +459,7 @@ int femhip_rows_metric(const femhip_geom_t* g, const double* G, double* metric,
 int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const uint32_t* rowrecs,
                                femhip_fused_state_t* state, int32_t form,
                                const femhip_geom_t* g, const femhip_space_t* sp,
-                               const double* metric, double* data,
+                               const double* metric, const double* kappa, double* data,
                                const femhip_fused_check_t* check, femhip_stream_t s);
 
 /* out[S][nseg] <- kappa-weighted segmented sums  sum_k kappa[s][c_k % E] *

@@ -415,6 +415,8 @@ template <int D, int NB, int Q>
 int rows_recheck_form(const RecheckArgs& a, int form, hipStream_t s) {
   if (form == FEMHIP_LAPLACE)
     fused_recheck_kernel<D, Q, FEMHIP_LAPLACE, NB><<<kCheckBlocks, kBlock, 0, s>>>(a);
+  else if (form == FEMHIP_KAPPA_LAPLACE)
+    fused_recheck_kernel<D, Q, FEMHIP_KAPPA_LAPLACE, NB><<<kCheckBlocks, kBlock, 0, s>>>(a);
   else
     fused_recheck_kernel<D, Q, FEMHIP_MASS, NB><<<kCheckBlocks, kBlock, 0, s>>>(a);
   FEMHIP_CHECK_LAUNCH();
@@ -424,11 +426,12 @@ int rows_recheck_form(const RecheckArgs& a, int form, hipStream_t s) {
 }
 
 int rows_check_run(const femhip_fused_check_t* c, femhip_fused_state_t* st, int form,
-                   const femhip_geom_t* g, const femhip_space_t* sp, hipStream_t s) {
+                   const femhip_geom_t* g, const femhip_space_t* sp, const double* kappa,
+                   hipStream_t s) {
   FEMHIP_REQUIRE(c->zero_mask && c->queue && c->capacity > 0 && g->nelems > 0, FEMHIP_EINVAL,
                  "rows check: NULL zero mask or queue");
   RecheckArgs a{c,          g->t,      g->p,       sp->dphi, sp->phi,  g->W,
-                nullptr,    c->queue,  c->zero_mask, c->capacity, g->nelems, g->nnodes,
+                kappa,      c->queue,  c->zero_mask, c->capacity, g->nelems, g->nnodes,
                 c->nzeros,  st,        c->verdict};
   if (g->dim == 3 && sp->nbs == 10) {
     if (g->nq == 11) return rows_recheck_form<3, 10, 11>(a, form, s);

@@ -16,9 +16,11 @@ int fused_check_run(const femhip_fused_check_t* check, femhip_fused_state_t* sta
                     int dim, const double* p, int64_t nnodes, const double* kappa,
                     hipStream_t s);
 // the same for the suspects of a P2 rows pass (P2 triangles/tetrahedra,
-// laplace or mass; tables and coordinates from g and sp)
+// laplace, mass or kappa_laplace with kappa per element; tables and
+// coordinates from g and sp)
 int rows_check_run(const femhip_fused_check_t* check, femhip_fused_state_t* state, int form,
-                   const femhip_geom_t* g, const femhip_space_t* sp, hipStream_t s);
+                   const femhip_geom_t* g, const femhip_space_t* sp, const double* kappa,
+                   hipStream_t s);
 // verdict of an empty plan (no partitions): valid iff the plan has no zeros
 int fused_check_empty(const femhip_fused_check_t* check, femhip_fused_state_t* state,
                       hipStream_t s);

@@ -46,6 +46,7 @@ struct RowsArgs {
   const int32_t* rec_elem;
   const uint32_t* rowrecs;  // kRowRecWords words per row record
   const double* metric;     // [E][kMetric] (femhip_rows_metric)
+  const double* kappa;      // kappa-Laplace: per element
   const double* W;
   const double* phi;        // [nb][Q]
   const double* dphi;       // [nb][D][Q]
@@ -134,7 +135,8 @@ __device__ __forceinline__ void push_row_suspects(const RowsArgs& a, int64_t e, 
 
 template <int D, int NB, int Q, int FORM, int NT>
 __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
-  constexpr bool GRAD = FORM == FEMHIP_LAPLACE;
+  constexpr bool GRAD = FORM != FEMHIP_MASS;   // laplace, kappa_laplace
+  constexpr bool KAPPA = FORM == FEMHIP_KAPPA_LAPLACE;
   constexpr int NM = GRAD ? D * (D + 1) / 2 : 1;   // metric entries per element
   constexpr int RW = (NB * 16 + 31) / 32;
   constexpr int RV = RW + 1;   // visit record: local ids, then the element id
@@ -266,6 +268,7 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
         if constexpr (D == 3) q2 = mp[2];
       }
       const double2 q3 = mp[3];
+      const double kap = KAPPA ? a.kappa[cur.e] : 1.0;
       // the next chunk's loads are issued on every path (past the last
       // chunk they reload the last record)
       load_rec(c0 + NT, nxt);
@@ -297,6 +300,11 @@ __global__ void __launch_bounds__(NT) p2_fused_rows_kernel(RowsArgs a) {
             M[5] = q2.y;
           }
           tsc = q3.x;
+          if constexpr (KAPPA) {   // kappa |det| A^-1 A^-T, the bound times |kappa|
+#pragma unroll
+            for (int mm = 0; mm < NM; ++mm) M[mm] *= kap;
+            tsc *= fabs(kap);
+          }
         } else {
           M[0] = q3.y;
           tsc = 0x1p-44 * q3.y;
@@ -403,7 +411,10 @@ __global__ void __launch_bounds__(256) rows_metric_kernel(const double* __restri
 
 template <int D, int NB, int Q>
 int launch_rows(const RowsArgs& a, int form, size_t lds, unsigned grid, hipStream_t s) {
-  if (form == FEMHIP_LAPLACE)
+  if (form == FEMHIP_KAPPA_LAPLACE)
+    p2_fused_rows_kernel<D, NB, Q, FEMHIP_KAPPA_LAPLACE, kRowsThreads>
+        <<<grid, kRowsThreads, lds + 8 * rows_tab_doubles(D, NB, true), s>>>(a);
+  else if (form == FEMHIP_LAPLACE)
     p2_fused_rows_kernel<D, NB, Q, FEMHIP_LAPLACE, kRowsThreads>
         <<<grid, kRowsThreads, lds + 8 * rows_tab_doubles(D, NB, true), s>>>(a);
   else
@@ -442,16 +453,20 @@ extern "C" int femhip_rows_metric(const femhip_geom_t* g, const double* G, doubl
 extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const uint32_t* rowrecs,
                                           femhip_fused_state_t* state, int32_t form,
                                           const femhip_geom_t* g, const femhip_space_t* sp,
-                                          const double* metric, double* data,
-                                          const femhip_fused_check_t* check, femhip_stream_t s) {
+                                          const double* metric, const double* kappa,
+                                          double* data, const femhip_fused_check_t* check,
+                                          femhip_stream_t s) {
   FEMHIP_REQUIRE(plan && rowrecs && state && g && sp && metric && data, FEMHIP_EINVAL,
                  "rows fused: NULL argument");
   FEMHIP_REQUIRE(reinterpret_cast<uintptr_t>(metric) % 16 == 0, FEMHIP_EINVAL,
                  "rows fused: metric must be 16-byte aligned");
   FEMHIP_REQUIRE(reinterpret_cast<uintptr_t>(state) % 128 == 0, FEMHIP_EINVAL,
                  "rows fused: state must be 128-byte aligned");
-  FEMHIP_REQUIRE(form == FEMHIP_LAPLACE || form == FEMHIP_MASS, FEMHIP_EUNSUPPORTED,
-                 "rows fused: laplace or mass only");
+  FEMHIP_REQUIRE(form == FEMHIP_LAPLACE || form == FEMHIP_MASS ||
+                     form == FEMHIP_KAPPA_LAPLACE,
+                 FEMHIP_EUNSUPPORTED, "rows fused: laplace, mass or kappa_laplace only");
+  FEMHIP_REQUIRE((form == FEMHIP_KAPPA_LAPLACE) == (kappa != nullptr), FEMHIP_EINVAL,
+                 "rows fused: kappa_laplace takes kappa (per element), the others none");
   FEMHIP_REQUIRE(check == nullptr || (check->verdict != nullptr && check->zero_mask != nullptr &&
                                         check->queue != nullptr && check->capacity > 0),
                  FEMHIP_EINVAL, "rows fused: the check needs a verdict, a zero mask and a queue");
@@ -465,7 +480,8 @@ extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const 
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
   if (I.nparts > 0) {
     RowsArgs a{plan->parts,  plan->nodemap, plan->rows,  plan->recs,  plan->rec_elem,
-               rowrecs,      metric,        g->W,        sp->phi,     sp->dphi,
+               rowrecs,      metric,        kappa,       g->W,        sp->phi,
+               sp->dphi,
                data,         state,
                check ? check->queue : nullptr, check ? check->capacity : 0,
                g->nelems,    I.nparts,      I.max_slots, I.max_nodes, I.max_rows,    0};
@@ -487,7 +503,7 @@ extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const 
   }
   if (check) {
     if (I.nparts == 0) return fused_check_empty(check, state, st);
-    return rows_check_run(check, state, form, g, sp, st);
+    return rows_check_run(check, state, form, g, sp, kappa, st);
   }
   return FEMHIP_OK;
 }

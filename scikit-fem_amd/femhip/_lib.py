@@ -140,7 +140,7 @@ SIGNATURES = {
     'femhip_rows_metric': (ctypes.c_int, [ctypes.POINTER(Geom), _p, _p, _p]),
     'femhip_rows_fused_assemble': (ctypes.c_int, [ctypes.POINTER(FusedDev), _p, _p, _i32,
                                                   ctypes.POINTER(Geom), ctypes.POINTER(Space),
-                                                  _p, _p, ctypes.POINTER(FusedCheck), _p]),
+                                                  _p, _p, _p, ctypes.POINTER(FusedCheck), _p]),
     'femhip_fused_adjoint_lds_bytes': (ctypes.c_size_t, [ctypes.POINTER(FusedInfo), _i32]),
     'femhip_fused_adjoint': (ctypes.c_int, [ctypes.POINTER(FusedDev), _p, _i32, _p, _i64, _p,
                                             _p, _p, _p, _p, _p, _p, _p]),

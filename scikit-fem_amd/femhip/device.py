@@ -410,7 +410,7 @@ class DeviceBasis:
             return None
         return DeviceCSR(plan.indptr, plan.indices, out, (self.N, self.N), plan)
 
-    def rows_fused_try(self, name):
+    def rows_fused_try(self, name, kappa=None):
         """The fused rows kernel (P2 simplices) on the last pattern validated
         for ``name``, checked against it in the same pass: the CSR values
         (DeviceCSR) when this input's exact zeros are the pattern's, else
@@ -421,11 +421,11 @@ class DeviceBasis:
         fp = self.rows_plans[plan.signature]
         out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
                           device=self.p.device)[:plan.nnz]
-        fp.assemble(name, self, out)
+        fp.assemble(name, self, out, kappa)
         v = fp.verdict.cpu().tolist()      # synchronises
         if v[0] != 1 and (v[3] & _lib.FUSED_OVERFLOW):
             fp.grow_queue()                # more entries near zero than queued
-            fp.assemble(name, self, out)
+            fp.assemble(name, self, out, kappa)
             v = fp.verdict.cpu().tolist()
         self.last_verdict = v
         if v[0] != 1:
@@ -479,7 +479,8 @@ class DeviceBasis:
         # comparison (same inputs, same zeros)
         scalar = (path != 'generic' and not fusable and not self.basis.elem.vector
                   and name in ('laplace', 'mass', 'kappa_laplace', 'coef_mass'))
-        rows = (path != 'generic' and kappa is None and fused.rows_eligible(self.basis, name))
+        rows = (path != 'generic' and (kappa is None or kappa.dim() == 1)
+                and fused.rows_eligible(self.basis, name))
         vkey = (name, lam, mu, None if kappa is None else tuple(kappa.shape), self.p_version)
         if fusable:
             A = self.fused_try(name, kappa)
@@ -487,7 +488,7 @@ class DeviceBasis:
                 self.validated[vkey] = A.plan
                 return A
         if rows:
-            A = self.rows_fused_try(name)
+            A = self.rows_fused_try(name, kappa)
             if A is not None:
                 self.validated[vkey] = A.plan
                 return A

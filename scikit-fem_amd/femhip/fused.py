@@ -213,7 +213,7 @@ class FusedPlan:
 # per CU, less halo) and 6,000 (three per CU, more halo) on 41^3 P2 tets
 ROWS_SLOT_CAP = int(os.environ.get('FEMHIP_ROWS_SLOT_CAP', 9000))
 ROWS_NODE_CAP = int(os.environ.get('FEMHIP_ROWS_NODE_CAP', 16384))
-_ROWS_FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS}
+_ROWS_FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS, 'kappa_laplace': _lib.KAPPA_LAPLACE}
 ROW_RECORD_WORDS = 6   # (visit << 4) | DOF, then 2 x 10 seven-bit column offsets
 
 
@@ -318,18 +318,21 @@ class RowsFusedPlan:
                     row_records=int(i.nrowrecs), dup=i.dup, max_slots=i.max_slots,
                     max_nodes=i.max_nodes)
 
-    def assemble(self, name, d, out):
+    def assemble(self, name, d, out, kappa=None):
         """out[nnz] <- CSR values of ``name`` on the basis ``d``
-        (DeviceBasis) for its current coordinates; the call is checked
+        (DeviceBasis) for its current coordinates (``kappa``: per element,
+        for kappa_laplace); the call is checked
         against the plan's pattern (``set_pattern_zeros``), verdict {valid,
         mismatches, zeros, flags} left in ``self.verdict`` on the device."""
         assert out.numel() == self.nnz and out.dtype == torch.float64
         assert self.check is not None, "set_pattern_zeros first"
+        assert (name == 'kappa_laplace') == (kappa is not None)
+        assert kappa is None or (kappa.dim() == 1 and kappa.numel() == d.E)
         metric = d.rows_metric()
         _lib.check(_lib.load().femhip_rows_fused_assemble(
             ctypes.byref(self.dev), _lib.ptr(self.rowrecs), _lib.ptr(self.state(name)),
             _ROWS_FORM[name], ctypes.byref(d.geom), ctypes.byref(d.space), _lib.ptr(metric),
-            _lib.ptr(out), ctypes.byref(self.check),
+            _lib.ptr(kappa), _lib.ptr(out), ctypes.byref(self.check),
             _lib.stream_ptr()))
         return out
 

@@ -88,3 +88,36 @@ def test_rows_kernel_falls_back_when_the_zeros_change():
     A = laplace.assemble(b)
     assert d.last_verdict[0] == 0
     _same(A, R1)
+
+
+@pytest.mark.parametrize('kind,n', [('tri', 17), ('tet', 6)])
+def test_rows_kernel_kappa_laplace(kind, n):
+    """kappa-Laplace with kappa per element: later calls run the rows
+    kernel (the metric times kappa_e); a new kappa that keeps every exact
+    zero (twice the first: every rounding scales exactly) stays on it, an
+    unrelated new kappa may change the reference's zeros on the mesh's
+    right-angled boundary elements (either verdict, the result is the
+    reference's), and a kappa with zeros falls back to the exact path;
+    every result against the oracle."""
+    from femhip.models import kappa_laplace
+    m = _mesh(kind, n, 0.2)
+    b = femhip.Basis(m, femhip.ElementTriP2() if kind == 'tri' else femhip.ElementTetP2())
+    om = (O.MeshTri if kind == 'tri' else O.MeshTet)(m.p, m.t)
+    ob = O.CellBasis(om, O.ElementTriP2() if kind == 'tri' else O.ElementTetP2())
+    rng = np.random.default_rng(4)
+    E = m.nelements
+    k1 = rng.lognormal(0, 1, (E, 1))
+    k2 = rng.lognormal(0, 1, (E, 1))
+    k3 = k2.copy()
+    k3[rng.random(E) < 0.1] = 0.0
+    d = b.device()
+    steps = ((k1, False), (k1, True), (2.0 * k1, True), (k2, None), (k3, False))
+    for step, (k, fused) in enumerate(steps):
+        print('step', step, flush=True)
+        A = kappa_laplace.assemble(b, kappa=k)
+        R = O.bilinear_assemble(O.form_kappa_laplace, ob, kappa=k)
+        _same(A, R)
+        if step > 0:
+            assert 'kappa_laplace' in d.rows_last
+            if fused is not None:
+                assert (d.last_verdict[0] == 1) == fused, d.last_verdict

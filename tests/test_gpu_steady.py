@@ -1,10 +1,10 @@
-"""Later calls of scalar forms that have no fused kernel (P2 with a
-coefficient, P1 with a per-point coefficient or ``w.c * u * v``, Q1/Q2
+"""Later calls of scalar forms that have no fused kernel (P2 with
+``w.c * u * v``, P1 with a per-point coefficient or ``w.c * u * v``, Q1/Q2
 hexahedra): the validated pattern is reused without the exact-zero
 signature pass.  The values and the summation order are the first call's, so
-the CSR must be that call's bit for bit.  P2 laplace and mass later run the
-fused rows kernel (fixed-point sums): the same pattern, values within 1e-12,
-bitwise repeatable."""
+the CSR must be that call's bit for bit.  P2 laplace, mass and kappa-Laplace
+with kappa per element later run the fused rows kernel (fixed-point sums):
+the same pattern, values within 1e-12, bitwise repeatable."""
 import numpy as np
 import pytest
 
@@ -51,7 +51,7 @@ def test_second_call_bit_identical(kind, elem, n):
         A1 = form.assemble(b, **kw)
         A2 = form.assemble(b, **kw)
         A3 = form.assemble(b, **kw)
-        rows = elem.endswith('2') and form in (laplace, mass) and kind != 'hex'
+        rows = elem.endswith('2') and form in (laplace, mass, kappa_laplace) and kind != 'hex'
         for A in (A2, A3):
             assert np.array_equal(A.indptr, A1.indptr)
             assert np.array_equal(A.indices, A1.indices)
