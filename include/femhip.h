@@ -462,12 +462,13 @@ int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const uint32_t* r
                                const double* metric, const double* kappa, double* data,
                                const femhip_fused_check_t* check, femhip_stream_t s);
 
-/* out[S][nseg] <- kappa-weighted segmented sums  sum_k kappa[s][c_k % E] *
- * K1[c_k]  over the plan (perm, seg): many per-element coefficient samples
+/* out[S][nseg] <- kappa-weighted segmented sums  sum_k kappa[s][elem[k]] *
+ * K1p[k]  over the plan's slots (seg): many per-element coefficient samples
  * (kappa[S][E]) assembled on one shared pattern from the kappa = 1 local
- * entries K1.  Batched counterpart of BilinearForm.assemble
- * (bilinear_form.py:130-148) called once per sample. */
-int femhip_batched_kappa_values(const double* K1, const int32_t* perm, const int32_t* seg,
+ * entries in plan order, K1p[k] = K1[perm[k]], elem[k] = perm[k] % E
+ * (perm, seg: a femhip_pattern_build plan).  Batched counterpart of
+ * BilinearForm.assemble (bilinear_form.py:130-148) called once per sample. */
+int femhip_batched_kappa_values(const double* K1p, const int32_t* elem, const int32_t* seg,
                                 int64_t nseg, int64_t nelems, const double* kappa,
                                 int64_t nsamples, double* out, femhip_stream_t s);
 

@@ -1,14 +1,15 @@
 // Batched assembly of many per-element coefficient samples on one mesh.
 //
 // For a form linear in a per-element coefficient (kappa-Laplace), the CSR
-// values of sample s are  out[s][slot] = sum_k kappa[s][e_k] * K1[c_k]
-// over the contributions c_k of the slot (in the symbolic plan's fixed
-// order), with K1 the local entries at kappa = 1 and e_k = c_k mod E.  All
-// samples share one pattern.  A workgroup owns a tile of 64 consecutive
-// slots and TS samples at a time: lanes run over slots (coalesced stores of
-// out[s][slot..]), the contribution list of a slot is read once per sample
-// tile, the TS accumulators stay in registers, and each slot is summed in
-// plan order (deterministic).
+// values of sample s are  out[s][slot] = sum_k kappa[s][elem[k]] * K1p[k]
+// over the contributions k of the slot (seg[slot] <= k < seg[slot+1], the
+// symbolic plan's fixed order), with K1p the local entries at kappa = 1
+// listed in plan order and elem[k] their elements.  All samples share one
+// pattern.  A workgroup owns a tile of 64 consecutive slots and kGroups
+// sample groups: lanes run over slots (the contribution lists of
+// neighbouring slots are neighbouring runs of K1p / elem, and the stores of
+// out[s][slot..] are coalesced), each thread keeps TS sample accumulators in
+// registers, and each slot is summed in plan order (deterministic).
 #include "common.h"
 
 namespace femhip {
@@ -19,7 +20,7 @@ constexpr int kGroups = 4;   // sample groups per workgroup
 constexpr int kTS = 8;       // samples per thread and pass
 
 __global__ void __launch_bounds__(kSlots * kGroups)
-batched_kappa_kernel(const double* __restrict__ K1, const int32_t* __restrict__ perm,
+batched_kappa_kernel(const double* __restrict__ K1p, const int32_t* __restrict__ elem,
                      const int32_t* __restrict__ seg, int64_t nseg, int64_t E,
                      const double* __restrict__ kappa, int64_t S,
                      double* __restrict__ out) {
@@ -31,13 +32,13 @@ batched_kappa_kernel(const double* __restrict__ K1, const int32_t* __restrict__ 
     double acc[kTS];
 #pragma unroll
     for (int j = 0; j < kTS; ++j) acc[j] = 0.0;
+    const double* ks = kappa + s0 * E;
     for (int32_t k = b; k < en; ++k) {
-      const int64_t c = perm[k];
-      const double w = K1[c];
-      const int64_t e = c % E;
+      const double w = K1p[k];
+      const int64_t e = elem[k];
 #pragma unroll
       for (int j = 0; j < kTS; ++j)
-        if (s0 + j < S) acc[j] += kappa[(s0 + j) * E + e] * w;
+        if (s0 + j < S) acc[j] += ks[j * E + e] * w;
     }
 #pragma unroll
     for (int j = 0; j < kTS; ++j)
@@ -50,15 +51,15 @@ batched_kappa_kernel(const double* __restrict__ K1, const int32_t* __restrict__ 
 
 using namespace femhip;
 
-extern "C" int femhip_batched_kappa_values(const double* K1, const int32_t* perm,
+extern "C" int femhip_batched_kappa_values(const double* K1p, const int32_t* elem,
                                            const int32_t* seg, int64_t nseg, int64_t nelems,
                                            const double* kappa, int64_t nsamples, double* out,
                                            femhip_stream_t s) {
   FEMHIP_REQUIRE(nseg >= 0 && nelems > 0 && nsamples >= 0, FEMHIP_EINVAL, "batched: bad sizes");
   if (nseg == 0 || nsamples == 0) return FEMHIP_OK;
-  FEMHIP_REQUIRE(K1 && perm && seg && kappa && out, FEMHIP_EINVAL, "batched: NULL argument");
+  FEMHIP_REQUIRE(K1p && elem && seg && kappa && out, FEMHIP_EINVAL, "batched: NULL argument");
   batched_kappa_kernel<<<(unsigned)ceil_div(nseg, kSlots), kSlots * kGroups, 0,
-                         reinterpret_cast<hipStream_t>(s)>>>(K1, perm, seg, nseg, nelems,
+                         reinterpret_cast<hipStream_t>(s)>>>(K1p, elem, seg, nseg, nelems,
                                                              kappa, nsamples, out);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;

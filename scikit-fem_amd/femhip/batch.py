@@ -30,6 +30,11 @@ class KappaSamples:
         self.plan = d.build_plan(None)          # connectivity pattern
         self.K1 = d.local_bilinear('laplace')   # unit-coefficient local entries
         self.nnz = self.plan.nnz
+        # the local entries in plan order and their elements (contiguous runs
+        # per slot for the batched kernel)
+        perm = self.plan.perm[:self.K1.numel()].long()
+        self.K1p = self.K1.index_select(0, perm).contiguous()
+        self.elem = torch.remainder(perm, d.E).to(torch.int32).contiguous()
         self.indptr, self.indices = self.plan.indptr, self.plan.indices
 
     def export(self, s, kappa_s, values):
@@ -62,7 +67,7 @@ class KappaSamples:
         if out is None:
             out = torch.empty((S, self.nnz), dtype=torch.float64, device=kappa.device)
         _lib.check(self.d.lib.femhip_batched_kappa_values(
-            _lib.ptr(self.K1), _lib.ptr(self.plan.perm), _lib.ptr(self.plan.seg),
+            _lib.ptr(self.K1p), _lib.ptr(self.elem), _lib.ptr(self.plan.seg),
             self.nnz, E, _lib.ptr(kappa), S, _lib.ptr(out), _lib.stream_ptr()))
         return out
 

@@ -118,6 +118,10 @@ class Plan:
         self.seen = 0       # exact-path calls that matched this pattern
 
 
+# patterns (exact-zero sets) kept per basis
+PLAN_CACHE = 8
+
+
 class DeviceBasis:
 
     def __init__(self, basis):
@@ -160,6 +164,7 @@ class DeviceBasis:
         self.row_plan = None
         self.rows_plans = {}     # plan serial -> RowsFusedPlan (P2 simplices)
         self.rows_last = {}      # form -> the last plan its rows path used
+        self.rows_skip = {}      # form -> its last rows call failed the check
         self.mixed_plans = {}    # id(test device basis) -> (it, {serial: Plan})
 
     # -- coordinates ---------------------------------------------------------
@@ -279,16 +284,38 @@ class DeviceBasis:
 
     def plan_for(self, contrib):
         """Plan whose exact-zero positions equal those of ``contrib``,
-        compared bit for bit with each known plan's mask (most recent first;
-        patterns are shared by every form with the same zero set on this
-        basis), else a new one."""
-        for serial in sorted(self.plans, reverse=True):
-            plan = self.plans[serial]
-            if self.zero_mismatch(contrib, plan.mask) == 0:
-                return plan
-        serial = len(self.plans)
+        compared bit for bit with each known plan's mask (most recently
+        matched first; patterns are shared by every form with the same zero
+        set on this basis), else a new one.  At most ``PLAN_CACHE`` patterns
+        are kept per basis: a loop over inputs whose zero sets all differ
+        (a coefficient with zeros, say) does not accumulate them."""
+        serials = list(reversed(list(self.plans)))
+        if serials:
+            # every cached mask compared on the device, one read-back
+            counts = torch.empty(len(serials), dtype=torch.int64, device=self.p.device)
+            for k, serial in enumerate(serials):
+                _lib.check(self.lib.femhip_zero_mask_mismatch(
+                    _lib.ptr(contrib), contrib.numel(), _lib.ptr(self.plans[serial].mask),
+                    _lib.ptr(counts[k:]), _lib.stream_ptr()))
+            for k, c in enumerate(counts.cpu().tolist()):
+                if c == 0:
+                    serial = serials[k]
+                    self.plans[serial] = self.plans.pop(serial)   # most recent last
+                    return self.plans[serial]
+        serial = getattr(self, '_serial', len(self.plans))
+        self._serial = serial + 1
         plan = self.build_plan(contrib, serial)
         self.plans[serial] = plan
+        if len(self.plans) > PLAN_CACHE:
+            live = {p.signature for p in list(self.fused_last.values()) +
+                    list(getattr(self, 'rows_last', {}).values())}
+            for old in list(self.plans):
+                if old != serial and old not in live:
+                    del self.plans[old]
+                    self.fused_plans.pop(old, None)
+                    getattr(self, 'rows_plans', {}).pop(old, None)
+                    self.unfusable.pop(old, None)
+                    break
         return plan
 
     def fused_plan(self, plan, form='laplace'):
@@ -487,11 +514,14 @@ class DeviceBasis:
             if A is not None:
                 self.validated[vkey] = A.plan
                 return A
-        if rows:
+        if rows and not self.rows_skip.get(name):
             A = self.rows_fused_try(name, kappa)
             if A is not None:
                 self.validated[vkey] = A.plan
                 return A
+            # the input's zeros differ from the pattern's: the next calls go
+            # straight to the exact path until one matches a rows pattern
+            self.rows_skip[name] = True
         plan = self._validated_plan(vkey, kappa) if (mfma or scalar) else None
         if plan is not None:
             out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
@@ -520,8 +550,15 @@ class DeviceBasis:
                     self.unfusable[plan.signature] = str(exc)
         elif mfma or scalar:
             self._validate(vkey, plan, kappa)
-            if rows:
-                self.rows_plan_for(plan, name)
+            if rows and plan.signature not in self.unfusable:
+                # a rows plan (host build) when the form has none yet, or
+                # when this pattern comes back: inputs whose zero sets keep
+                # changing stay on the exact path without a build per call
+                plan.seen += 1
+                if plan.signature in self.rows_plans or name not in self.rows_last \
+                        or plan.seen >= 2:
+                    if self.rows_plan_for(plan, name) is not None:
+                        self.rows_skip[name] = False
         return DeviceCSR(plan.indptr, plan.indices, data, (self.N, self.N), plan)
 
     def elasticity_mfma(self, lam, mu, plan, out, loc=None):
