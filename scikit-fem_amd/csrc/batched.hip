@@ -17,7 +17,7 @@ namespace {
 
 constexpr int kSlots = 64;   // slots per workgroup (one wave wide)
 constexpr int kGroups = 4;   // sample groups per workgroup
-constexpr int kTS = 8;       // samples per thread and pass
+constexpr int kTS = 16;      // samples per thread and pass
 
 __global__ void __launch_bounds__(kSlots * kGroups)
 batched_kappa_kernel(const double* __restrict__ K1p, const int32_t* __restrict__ elem,
@@ -30,15 +30,23 @@ batched_kappa_kernel(const double* __restrict__ K1p, const int32_t* __restrict__
   const int32_t b = seg[slot], en = seg[slot + 1];
   for (int64_t s0 = (int64_t)group * kTS; s0 < S; s0 += (int64_t)kGroups * kTS) {
     double acc[kTS];
+    // the sample rows of this pass; past the last sample the last one is
+    // read again (and not stored), so every lane issues all kTS loads
+    // together instead of one conditional load at a time
+    const double* ks[kTS];
 #pragma unroll
-    for (int j = 0; j < kTS; ++j) acc[j] = 0.0;
-    const double* ks = kappa + s0 * E;
+    for (int j = 0; j < kTS; ++j) {
+      acc[j] = 0.0;
+      ks[j] = kappa + (s0 + j < S ? s0 + j : S - 1) * E;
+    }
     for (int32_t k = b; k < en; ++k) {
       const double w = K1p[k];
       const int64_t e = elem[k];
+      double kv[kTS];
 #pragma unroll
-      for (int j = 0; j < kTS; ++j)
-        if (s0 + j < S) acc[j] += ks[j * E + e] * w;
+      for (int j = 0; j < kTS; ++j) kv[j] = ks[j][e];
+#pragma unroll
+      for (int j = 0; j < kTS; ++j) acc[j] += kv[j] * w;
     }
 #pragma unroll
     for (int j = 0; j < kTS; ++j)
