@@ -19,6 +19,7 @@ from .basis import CellBasis, Basis, FacetBasis, DiscreteField
 from .forms import BilinearForm, LinearForm, asm
 from .utils import enforce, enforce_device, condense_device, solve_device
 from .device import DeviceCSR
+from .mapping import MappingAffine, MappingIsoparametric
 
 __version__ = "0.1.0"
 
@@ -26,5 +27,6 @@ __all__ = [
     'MeshTri', 'MeshTet', 'MeshHex', 'MeshTri1', 'MeshTet1', 'MeshHex1',
     'ElementTriP1', 'ElementTriP2', 'ElementTetP1', 'ElementTetP2',
     'ElementHex1', 'ElementHex2', 'ElementVector', 'CellBasis', 'Basis', 'FacetBasis',
-    'BilinearForm', 'LinearForm', 'asm', 'enforce', 'DeviceCSR',
+    'BilinearForm', 'LinearForm', 'asm', 'enforce', 'DeviceCSR', 'MappingAffine',
+    'MappingIsoparametric',
 ]

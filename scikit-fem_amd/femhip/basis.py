@@ -12,6 +12,7 @@ import ctypes
 import numpy as np
 
 from .dofs import Dofs
+from .mapping import check_mapping
 from .quadrature import get_quadrature
 from .element import ElementHex1
 
@@ -95,10 +96,10 @@ class CellBasis:
                  disable_doflocs=False):
         if mesh.kind != elem.kind:
             raise ValueError("Incompatible Mesh and Element.")
-        if mapping is not None:
-            raise NotImplementedError("custom mappings are not accelerated")
+        check_mapping(mapping, mesh)
         self.mesh = mesh
         self.elem = elem
+        self.mapping = mesh.mapping() if mapping is None else mapping
         self.dofs = Dofs(mesh, elem) if dofs is None else dofs
         if quadrature is not None:
             self.X, self.W = (np.asarray(quadrature[0], dtype=np.float64),
@@ -248,8 +249,7 @@ class FacetBasis:
                  facets=None, dofs=None, side=0, disable_doflocs=False):
         if mesh.kind != elem.kind:
             raise ValueError("Incompatible Mesh and Element.")
-        if mapping is not None:
-            raise NotImplementedError("custom mappings are not accelerated")
+        check_mapping(mapping, mesh)
         if mesh.kind not in ('tri', 'tet') or elem.vector or elem.maxdeg not in (1, 2):
             raise NotImplementedError("facet bases: P1/P2 triangles and tetrahedra")
         if side not in (0, 1):

@@ -209,6 +209,20 @@ class Mesh:
         mask[self.boundary_nodes()] = False
         return np.nonzero(mask)[0]
 
+    def mapping(self):
+        """The mesh's default reference mapping (``mesh.py:513-528``):
+        ``MappingAffine`` on simplices, ``MappingIsoparametric`` with the
+        trilinear element on hexahedra (a descriptor; the kernels evaluate
+        it)."""
+        from .mapping import MappingAffine, MappingIsoparametric
+        if not hasattr(self, '_cached_mapping'):
+            if self.kind == 'hex':
+                from .element import ElementHex1
+                self._cached_mapping = MappingIsoparametric(self, ElementHex1())
+            else:
+                self._cached_mapping = MappingAffine(self)
+        return self._cached_mapping
+
     def with_p(self, p):
         """Same topology (and cached entities), new node coordinates."""
         out = type(self).__new__(type(self))

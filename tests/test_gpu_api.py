@@ -185,3 +185,13 @@ def test_asm_over_trial_and_test_lists():
                             vbasis=O.CellBasis(om, O.ElementTriP1(), intorder=4,
                                                elements=vb[0].tind))
     _same(A, R)
+
+
+def test_default_mapping_argument_changes_nothing():
+    """``CellBasis(mapping=mesh.mapping())`` assembles exactly what the
+    default basis does (the kernels evaluate that mapping)."""
+    for kind, el, n in (('tet', 'ElementTetP1', 5), ('hex', 'ElementHex1', 4)):
+        m = _mesh(kind, n, 0.15)
+        A = laplace.assemble(femhip.CellBasis(m, getattr(femhip, el)()))
+        B = laplace.assemble(femhip.CellBasis(m, getattr(femhip, el)(), mapping=m.mapping()))
+        assert np.array_equal(A.indices, B.indices) and np.array_equal(A.data, B.data)

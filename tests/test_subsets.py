@@ -142,3 +142,29 @@ def test_vector_dofs_interleave_components(el):
     om = O.MeshTri(m.p, m.t)
     ovd, _ = O.element_dofs(om, O.ElementVector(getattr(O, el)()))
     assert np.array_equal(ovd, vb.element_dofs)
+
+
+def test_default_mapping_is_accepted_others_refused():
+    """``mapping=``: the mesh's default mapping (``Mesh.mapping()``,
+    ``mesh.py:513-528``) is what the kernels evaluate, so it is accepted;
+    any other mapping raises NotImplementedError (no CPU fallback)."""
+    m = _tri(5)
+    b = femhip.CellBasis(m, femhip.ElementTriP1(), mapping=m.mapping())
+    assert isinstance(b.mapping, femhip.MappingAffine) and b.mapping.mesh is m
+    femhip.CellBasis(m, femhip.ElementTriP2(), mapping=femhip.MappingAffine(m))
+    twin = femhip.MeshTri(m.p.copy(), m.t.copy())
+    femhip.CellBasis(m, femhip.ElementTriP1(), mapping=femhip.MappingAffine(twin))
+    with pytest.raises(NotImplementedError):
+        femhip.CellBasis(m, femhip.ElementTriP1(),
+                         mapping=femhip.MappingIsoparametric(m, femhip.ElementTriP2()))
+    other = _tri(6)
+    with pytest.raises(NotImplementedError):
+        femhip.CellBasis(m, femhip.ElementTriP1(), mapping=femhip.MappingAffine(other))
+    x = np.linspace(0, 1, 3)
+    h = femhip.MeshHex.init_tensor(x, x, x)
+    assert isinstance(h.mapping(), femhip.MappingIsoparametric)
+    femhip.CellBasis(h, femhip.ElementHex1(), mapping=h.mapping())
+    with pytest.raises(NotImplementedError):
+        femhip.CellBasis(h, femhip.ElementHex1(), mapping=femhip.MappingAffine(h))
+    with pytest.raises(NotImplementedError):
+        femhip.FacetBasis(m, femhip.ElementTriP1(), mapping=object())
