@@ -561,6 +561,20 @@ int femhip_hex2_elasticity_mfma(const femhip_geom_t* g, const femhip_space_t* sp
 int femhip_segment_sum_em(const double* loc, const int32_t* perm, const int32_t* seg,
                           int64_t nseg, int64_t nelems, int64_t nb, double* out,
                           femhip_stream_t s);
+/* The same sums row by row (COOData._assemble_scipy_csr, coo_data.py:28-36,
+ * on element-major row-major local matrices): femhip_row_sum_em_plan writes
+ * pos[(e*nb + i)*nb + j] = offset of entry (i, j) of element e in its CSR
+ * row (the caller fills pos with -1 first: pruned contributions keep it);
+ * femhip_row_sum_em sums, for every row r, the local rows
+ * rowe[rowe_ptr[r] .. rowe_ptr[r+1]) (lr = e*nb + i with element_dofs[i][e]
+ * == r, increasing e) into out[indptr[r] ..] in that order (rows of at most
+ * 384 slots: max_row_len). */
+int femhip_row_sum_em_plan(const int32_t* perm, const int32_t* seg, const int32_t* indptr,
+                           int64_t nrows, int64_t nelems, int64_t nb, int16_t* pos,
+                           femhip_stream_t s);
+int femhip_row_sum_em(const double* loc, const int16_t* pos, const int32_t* rowe_ptr,
+                      const int32_t* rowe, const int32_t* indptr, int64_t nrows, int64_t nb,
+                      int32_t max_row_len, double* out, femhip_stream_t s);
 
 #ifdef __cplusplus
 }

@@ -210,10 +210,114 @@ segment_sum_em_kernel(const double* __restrict__ loc, const int32_t* __restrict_
   out[k] = acc;
 }
 
+// Row sums of element-major, row-major local matrices.  pos[lr*nb + j]
+// (lr = e*nb + i, the local row of element e whose DOF is global row r) is
+// the offset of entry (i, j) in row r of the CSR pattern, -1 for a
+// contribution the pattern prunes.  One thread per row writes the offsets
+// of its slots' contributions (c = (j*nb + i)*E + e in the plan).
+__global__ void __launch_bounds__(256)
+row_sum_pos_kernel(const int32_t* __restrict__ perm, const int32_t* __restrict__ seg,
+                   const int32_t* __restrict__ indptr, int64_t nrows, uint32_t E, uint32_t nb,
+                   Div31 byE, Div31 byNB, int16_t* __restrict__ pos) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= nrows) return;
+  const int32_t k0 = indptr[r], k1 = indptr[r + 1];
+  for (int32_t k = k0; k < k1; ++k)
+    for (int32_t q = seg[k]; q < seg[k + 1]; ++q) {
+      const uint32_t cu = (uint32_t)perm[q], ent = byE.div(cu), e = cu - ent * E;
+      const uint32_t tj = byNB.div(ent), ti = ent - tj * nb;
+      pos[((int64_t)e * nb + ti) * nb + tj] = (int16_t)(k - k0);
+    }
+}
+
+// One wave per global row r: the row's slots accumulate in LDS while the
+// wave walks the local rows of r (rowe[rowe_ptr[r] .. rowe_ptr[r+1]), in
+// increasing element order), each read as nb contiguous values with their
+// offsets; the entries of one local row go to distinct slots, so the lanes
+// add without atomics, and every slot is summed in the same order on every
+// call.  The row is then written once, coalesced.
+constexpr int kRowWaves = 4;
+constexpr int kMaxRowLen = 384;
+__global__ void __launch_bounds__(64 * kRowWaves)
+row_sum_em_kernel(const double* __restrict__ loc, const int16_t* __restrict__ pos,
+                  const int32_t* __restrict__ rowe_ptr, const int32_t* __restrict__ rowe,
+                  const int32_t* __restrict__ indptr, int64_t nrows, int nb,
+                  double* __restrict__ out) {
+  __shared__ double acc[kRowWaves][kMaxRowLen];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * kRowWaves + w;
+  if (r >= nrows) return;   // a whole wave: no barrier below
+  double* a = acc[w];
+  const int32_t k0 = indptr[r], len = indptr[r + 1] - k0;
+  for (int q = lane; q < len; q += 64) a[q] = 0.0;
+  const int32_t t0 = rowe_ptr[r], nt = rowe_ptr[r + 1] - t0;
+  // kLr local rows at a time, their loads issued together (an index past
+  // the row's last local row reloads that one; its values are not added)
+  constexpr int kLr = 8;
+  for (int c = 0; c < nt; c += kLr) {
+    double v0[kLr], v1[kLr];
+    int p0[kLr], p1[kLr];
+#pragma unroll
+    for (int u = 0; u < kLr; ++u) {
+      const int64_t base = (int64_t)rowe[t0 + min(c + u, nt - 1)] * nb;
+      const int j1 = min(lane + 64, nb - 1);
+      v0[u] = loc[base + min(lane, nb - 1)];
+      p0[u] = pos[base + min(lane, nb - 1)];
+      v1[u] = loc[base + j1];
+      p1[u] = pos[base + j1];
+    }
+    // LDS adds without a return value: no wait between the local rows; one
+    // wave's adds reach a slot in instruction order (u), and no two lanes of
+    // one instruction share a slot, so every slot is summed in the same
+    // order on every call
+#pragma unroll
+    for (int u = 0; u < kLr; ++u) {
+      if (c + u >= nt) break;
+      if (lane < nb && p0[u] >= 0)
+        __hip_atomic_fetch_add(a + p0[u], v0[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (lane + 64 < nb && p1[u] >= 0)
+        __hip_atomic_fetch_add(a + p1[u], v1[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  for (int q = lane; q < len; q += 64) out[k0 + q] = a[q];
+}
+
 }  // namespace
 }  // namespace femhip
 
 using namespace femhip;
+
+extern "C" int femhip_row_sum_em_plan(const int32_t* perm, const int32_t* seg,
+                                      const int32_t* indptr, int64_t nrows, int64_t nelems,
+                                      int64_t nb, int16_t* pos, femhip_stream_t s) {
+  FEMHIP_REQUIRE(nrows >= 0 && nelems > 0 && nb > 0, FEMHIP_EINVAL, "row sums: bad sizes");
+  FEMHIP_REQUIRE(nelems * nb * nb <= INT32_MAX, FEMHIP_EOVERFLOW,
+                 "row sums: contributions exceed int32 indices");
+  if (nrows == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(perm && seg && indptr && pos, FEMHIP_EINVAL, "row sums: NULL argument");
+  row_sum_pos_kernel<<<(unsigned)ceil_div(nrows, 256), 256, 0,
+                       reinterpret_cast<hipStream_t>(s)>>>(
+      perm, seg, indptr, nrows, (uint32_t)nelems, (uint32_t)nb, Div31::make((uint32_t)nelems),
+      Div31::make((uint32_t)nb), pos);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
+
+extern "C" int femhip_row_sum_em(const double* loc, const int16_t* pos, const int32_t* rowe_ptr,
+                                 const int32_t* rowe, const int32_t* indptr, int64_t nrows,
+                                 int64_t nb, int32_t max_row_len, double* out, femhip_stream_t s) {
+  FEMHIP_REQUIRE(nrows >= 0 && nb > 0, FEMHIP_EINVAL, "row sums: bad sizes");
+  FEMHIP_REQUIRE(max_row_len <= kMaxRowLen, FEMHIP_EUNSUPPORTED,
+                 "row sums: a CSR row longer than 384 slots");
+  if (nrows == 0) return FEMHIP_OK;
+  FEMHIP_REQUIRE(loc && pos && rowe_ptr && rowe && indptr && out, FEMHIP_EINVAL,
+                 "row sums: NULL argument");
+  row_sum_em_kernel<<<(unsigned)ceil_div(nrows, kRowWaves), 64 * kRowWaves, 0,
+                      reinterpret_cast<hipStream_t>(s)>>>(loc, pos, rowe_ptr, rowe, indptr,
+                                                          nrows, (int)nb, out);
+  FEMHIP_CHECK_LAUNCH();
+  return FEMHIP_OK;
+}
 
 extern "C" int femhip_hex2_elasticity_mfma(const femhip_geom_t* g, const femhip_space_t* sp,
                                            double lam, double mu, const double* G,

@@ -152,6 +152,8 @@ SIGNATURES = {
     'femhip_hex2_elasticity_mfma': (ctypes.c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Space),
                                                    _f64, _f64, _p, _p, _i32, _p]),
     'femhip_segment_sum_em': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _p]),
+    'femhip_row_sum_em_plan': (ctypes.c_int, [_p, _p, _p, _i64, _i64, _i64, _p, _p]),
+    'femhip_row_sum_em': (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i64, _i32, _p, _p]),
     'femhip_enforce_rows': (ctypes.c_int, [_p, _p, _p, _p, _i64, _f64, _p, _p, _p, _p]),
     'femhip_facet_local': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _i32, _p, _i32, _p, _p]),
     'femhip_facet_interpolate': (ctypes.c_int, [ctypes.POINTER(FacetGeom), _p, _p, _p, _p, _p]),

@@ -563,18 +563,54 @@ class DeviceBasis:
 
     def elasticity_mfma(self, lam, mu, plan, out, loc=None):
         """Numeric vector-Q2-hex elasticity: MFMA local matrices (element
-        major, row-major) + segmented sums over the plan."""
+        major, row-major) + row sums over the plan (each local row read once,
+        contiguous; ``femhip_row_sum_em``), or the slot-wise segmented sums
+        when a row is longer than the row kernel takes."""
         G = self.geometry()
-        nb2 = self.nbfun * self.nbfun
+        nb = self.nbfun
         if loc is None:
-            loc = torch.empty(nb2 * self.E, dtype=torch.float64, device=self.p.device)
+            loc = torch.empty(nb * nb * self.E, dtype=torch.float64, device=self.p.device)
         _lib.check(self.lib.femhip_hex2_elasticity_mfma(
             ctypes.byref(self.geom), ctypes.byref(self.space), float(lam), float(mu),
             _lib.ptr(G), _lib.ptr(loc), 0, _lib.stream_ptr()))
-        _lib.check(self.lib.femhip_segment_sum_em(
-            _lib.ptr(loc), _lib.ptr(plan.perm), _lib.ptr(plan.seg), plan.nnz, self.E, self.nbfun,
-            _lib.ptr(out), _lib.stream_ptr()))
+        rs = self._row_sum_plan(plan)
+        if rs is None:
+            _lib.check(self.lib.femhip_segment_sum_em(
+                _lib.ptr(loc), _lib.ptr(plan.perm), _lib.ptr(plan.seg), plan.nnz, self.E, nb,
+                _lib.ptr(out), _lib.stream_ptr()))
+        else:
+            pos, rowe_ptr, rowe, max_len = rs
+            _lib.check(self.lib.femhip_row_sum_em(
+                _lib.ptr(loc), _lib.ptr(pos), _lib.ptr(rowe_ptr), _lib.ptr(rowe),
+                _lib.ptr(plan.indptr), self.N, nb, max_len, _lib.ptr(out), _lib.stream_ptr()))
         return out
+
+    def _row_sum_plan(self, plan):
+        """(pos, rowe_ptr, rowe, max row length) of a pattern for the row
+        sums, built once per plan; None for rows over 384 slots."""
+        rs = getattr(plan, 'row_sums', 0)
+        if rs != 0:
+            return rs
+        dev = self.p.device
+        lens = plan.indptr[1:] - plan.indptr[:-1]
+        max_len = int(lens.max().item()) if self.N > 0 else 0
+        if max_len > 384:
+            plan.row_sums = None
+            return None
+        nb = self.nbfun
+        # local rows lr = e*nb + i grouped by their global row, increasing e
+        rows_of_lr = self.dofs.t().contiguous().reshape(-1).long()   # [E][nb] -> lr
+        order = torch.argsort(rows_of_lr, stable=True)
+        counts = torch.bincount(rows_of_lr, minlength=self.N)
+        rowe_ptr = torch.zeros(self.N + 1, dtype=torch.int32, device=dev)
+        rowe_ptr[1:] = torch.cumsum(counts, 0).to(torch.int32)
+        rowe = order.to(torch.int32).contiguous()
+        pos = torch.full((self.E * nb * nb,), -1, dtype=torch.int16, device=dev)
+        _lib.check(self.lib.femhip_row_sum_em_plan(
+            _lib.ptr(plan.perm), _lib.ptr(plan.seg), _lib.ptr(plan.indptr), self.N, self.E, nb,
+            _lib.ptr(pos), _lib.stream_ptr()))
+        plan.row_sums = (pos, rowe_ptr, rowe, max_len)
+        return plan.row_sums
 
     def assemble_linear(self, name='unit_load', kappa=None):
         contrib = self.local_linear(name, kappa)
