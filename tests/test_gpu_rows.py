@@ -1,8 +1,10 @@
 """GPU parity of the fused rows kernel (P2 triangles and tetrahedra,
-laplace and mass): later calls run it on the validated pattern and must give
-the oracle's pattern and values within 1e-12, also for new coordinates
-(checked in the same pass), and fall back to the exact path when the new
-input's exact zeros differ from the pattern's."""
+laplace, mass and kappa-Laplace): later calls run it on the validated
+pattern and must give the oracle's pattern and values within 1e-12, also
+for new coordinates and on unjittered meshes whose entries include
+structural zeros (every element's near-zero entries rechecked), and fall
+back to the exact path when the new input's exact zeros differ from the
+pattern's."""
 import numpy as np
 import pytest
 import torch
@@ -35,7 +37,8 @@ def _same(A, R):
 
 @pytest.mark.parametrize('kind,n,a,form', [
     ('tri', 17, 0.2, 'laplace'), ('tri', 17, 0.0, 'laplace'), ('tri', 9, 0.2, 'mass'),
-    ('tet', 6, 0.2, 'laplace'), ('tet', 5, 0.0, 'laplace'), ('tet', 5, 0.2, 'mass'),
+    ('tri', 9, 0.0, 'mass'), ('tet', 6, 0.2, 'laplace'), ('tet', 5, 0.0, 'laplace'),
+    ('tet', 5, 0.2, 'mass'), ('tet', 5, 0.0, 'mass'),
 ])
 def test_rows_kernel_matches_the_oracle(kind, n, a, form):
     print('case', kind, n, a, form, flush=True)
