@@ -1,6 +1,7 @@
-"""INTEGRATION.md's reference-side ctypes module, executed as written (only
-the library path is pointed at this build): the P1 tetrahedral Laplace and
-P2 triangular mass matrices it returns equal the oracle's."""
+"""The reference-side ctypes module of the integration guide (the first
+Python block of the guide at the repository root), executed as written
+(only the library path is pointed at this build): the P1 tetrahedral
+Laplace and P2 triangular mass matrices it returns equal the oracle's."""
 import os
 import re
 
