@@ -523,6 +523,20 @@ class DeviceBasis:
             # straight to the exact path until one matches a rows pattern
             self.rows_skip[name] = True
         plan = self._validated_plan(vkey, kappa) if (mfma or scalar) else None
+        if plan is not None and rows and plan.signature not in self.unfusable:
+            # the same input as a validated call (e.g. after the zeros
+            # changed once): its pattern gets a rows plan when it comes back
+            fp = self.rows_plans.get(plan.signature)
+            if fp is None:
+                plan.seen += 1
+                if plan.seen >= 2:
+                    fp = self.rows_plan_for(plan, name)
+            if fp is not None:
+                self.rows_last[name] = plan
+                self.rows_skip[name] = False
+                A = self.rows_fused_try(name, kappa)
+                if A is not None:
+                    return A
         if plan is not None:
             out = torch.empty(max(plan.nnz, 1), dtype=torch.float64,
                               device=self.p.device)[:plan.nnz]

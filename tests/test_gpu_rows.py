@@ -91,6 +91,11 @@ def test_rows_kernel_falls_back_when_the_zeros_change():
     A = laplace.assemble(b)
     assert d.last_verdict[0] == 0
     _same(A, R1)
+    # the new zero set is kept: the calls that follow on these coordinates
+    # get a rows plan of the new pattern and run fused again
+    for _ in range(2):
+        _same(laplace.assemble(b), R1)
+    assert d.last_verdict[0] == 1 and d.rows_last['laplace'].nnz == R1.nnz
 
 
 @pytest.mark.parametrize('kind,n', [('tri', 17), ('tet', 6)])
