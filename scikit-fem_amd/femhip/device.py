@@ -165,6 +165,7 @@ class DeviceBasis:
         self.rows_plans = {}     # plan serial -> RowsFusedPlan (P2 simplices)
         self.rows_last = {}      # form -> the last plan its rows path used
         self.rows_skip = {}      # form -> its last rows call failed the check
+        self.fused_skip = {}     # form -> its last fused P1 call failed the check
         self.mixed_plans = {}    # id(test device basis) -> (it, {serial: Plan})
 
     # -- coordinates ---------------------------------------------------------
@@ -509,11 +510,14 @@ class DeviceBasis:
         rows = (path != 'generic' and (kappa is None or kappa.dim() == 1)
                 and fused.rows_eligible(self.basis, name))
         vkey = (name, lam, mu, None if kappa is None else tuple(kappa.shape), self.p_version)
-        if fusable:
+        if fusable and not self.fused_skip.get(name):
             A = self.fused_try(name, kappa)
             if A is not None:
                 self.validated[vkey] = A.plan
                 return A
+            # the input's zeros differ from the pattern's: the next calls go
+            # to the exact path until one matches a pattern with a fused plan
+            self.fused_skip[name] = True
         if rows and not self.rows_skip.get(name):
             A = self.rows_fused_try(name, kappa)
             if A is not None:
@@ -556,6 +560,7 @@ class DeviceBasis:
                 try:
                     self.fused_plan(plan, name)
                     self.fused_last[name] = plan
+                    self.fused_skip[name] = False
                     self.validated[vkey] = plan
                 except (NotImplementedError, ValueError) as exc:
                     # a mesh the fused plan cannot encode (e.g. a CSR row
