@@ -491,7 +491,10 @@ class DeviceBasis:
         the first call, the exact path (local entries in the reference's
         arithmetic -> pattern matched by its exact-zero mask or built ->
         segmented sums).  A pattern gets its fused plan the first time the
-        form has none, else when it is matched a second time.
+        form has none, else when it is matched a second time; after a failed
+        check the form's calls take the exact path until one matches a
+        pattern that has a fused plan.  P2 simplices (laplace, mass,
+        kappa_laplace per element) do the same with the fused rows kernel.
 
         Vector Q2-hex elasticity (MFMA kernel) and the other scalar forms
         (exact path without the zero-mask pass) reuse the pattern validated
