@@ -502,6 +502,12 @@ class DeviceBasis:
         bit for bit on the device).
         """
         from . import fused
+        if self.E == 0:   # an empty element subset: the all-zero matrix
+            self.local_bilinear(name, kappa, lam, mu)   # form check only
+            dev = self.p.device
+            return DeviceCSR(torch.zeros(self.N + 1, dtype=torch.int32, device=dev),
+                             torch.zeros(0, dtype=torch.int32, device=dev),
+                             torch.zeros(0, dtype=torch.float64, device=dev), (self.N, self.N))
         fusable = (path != 'generic' and fused.eligible(self.basis, name)
                    and (kappa is None or kappa.dim() == 1))
         mfma = (path != 'generic' and name == 'linear_elasticity' and self.kind == 'hex'
@@ -635,6 +641,9 @@ class DeviceBasis:
         return plan.row_sums
 
     def assemble_linear(self, name='unit_load', kappa=None):
+        if self.E == 0:   # an empty element subset: the zero vector
+            self.local_linear(name, kappa)
+            return torch.zeros(self.N, dtype=torch.float64, device=self.p.device)
         contrib = self.local_linear(name, kappa)
         perm, seg = self.rows_plan()
         return self.segment_sum(contrib, perm, seg, self.N)

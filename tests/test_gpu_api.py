@@ -195,3 +195,20 @@ def test_default_mapping_argument_changes_nothing():
         A = laplace.assemble(femhip.CellBasis(m, getattr(femhip, el)()))
         B = laplace.assemble(femhip.CellBasis(m, getattr(femhip, el)(), mapping=m.mapping()))
         assert np.array_equal(A.indices, B.indices) and np.array_equal(A.data, B.data)
+
+
+@pytest.mark.parametrize('kind,el', [('tri', 'ElementTriP1'), ('tet', 'ElementTetP2')])
+def test_empty_element_subset(kind, el):
+    """No element selected: the all-zero matrix and load of the full DOF
+    count, as the reference's assembly over zero elements gives."""
+    m = _mesh(kind, 5, 0.2)
+    b = femhip.CellBasis(m, getattr(femhip, el)(), elements=np.zeros(m.nelements, bool))
+    assert b.nelems == 0
+    for _ in range(2):
+        A = laplace.assemble(b)
+        assert A.shape == (b.N, b.N) and A.nnz == 0
+    v = unit_load.assemble(b)
+    assert v.shape == (b.N,) and not v.any()
+    R = O.bilinear_assemble(O.form_laplace, O.CellBasis(_omesh(m), getattr(O, el)(),
+                                                        elements=b.tind))
+    assert R.nnz == 0 and R.shape == A.shape
