@@ -49,7 +49,7 @@ struct AdjArgs {
   int32_t mode;
 };
 
-constexpr int kAdjThreads = 256;
+constexpr int kAdjThreads = 512;
 
 // LDS: [dX accumulators (max_rows x D int64) | coordinates (max_nodes x D) |
 //  upstream diagonal (max_nodes) | upstream load (max_nodes) | reduction |
