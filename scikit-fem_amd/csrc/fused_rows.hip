@@ -62,7 +62,6 @@ struct RowsArgs {
   int32_t mode;
 };
 
-constexpr int kRowsThreads = 512;
 // per element: the metric |det| A^-1 A^-T (pairs (a, a), then (0,1), (0,2),
 // (1,2)) in [0, D(D+1)/2), the Laplace suspect scale 2^-44 D max|A^-1|^2
 // |det| in [6] and |det| in [7]
@@ -87,9 +86,10 @@ struct RowsLds {
   }
 };
 
-// without the form tables (rows_tab_doubles() more per launch)
+// without the form tables (rows_tab_doubles() more per launch); the
+// workgroup size is the plan's block (512 or 1024)
 size_t rows_lds_bytes(const femhip_fused_plan_info_t& I) {
-  return 8 * (size_t)I.max_slots + 8 * (size_t)I.max_rows + 8 * 2 * (kRowsThreads / 64);
+  return 8 * (size_t)I.max_slots + 8 * (size_t)I.max_rows + 8 * 2 * (size_t)(I.block / 64);
 }
 
 // doubles of the form tables: T (NB^2 D(D+1)/2 for Laplace, NB^2 for mass)
@@ -409,19 +409,26 @@ __global__ void __launch_bounds__(256) rows_metric_kernel(const double* __restri
   for (int k = 0; k < kMetric / 2; ++k) dst[k] = make_double2(o[2 * k], o[2 * k + 1]);
 }
 
-template <int D, int NB, int Q>
-int launch_rows(const RowsArgs& a, int form, size_t lds, unsigned grid, hipStream_t s) {
+template <int D, int NB, int Q, int NT>
+int launch_rows_nt(const RowsArgs& a, int form, size_t lds, unsigned grid, hipStream_t s) {
   if (form == FEMHIP_KAPPA_LAPLACE)
-    p2_fused_rows_kernel<D, NB, Q, FEMHIP_KAPPA_LAPLACE, kRowsThreads>
-        <<<grid, kRowsThreads, lds + 8 * rows_tab_doubles(D, NB, true), s>>>(a);
+    p2_fused_rows_kernel<D, NB, Q, FEMHIP_KAPPA_LAPLACE, NT>
+        <<<grid, NT, lds + 8 * rows_tab_doubles(D, NB, true), s>>>(a);
   else if (form == FEMHIP_LAPLACE)
-    p2_fused_rows_kernel<D, NB, Q, FEMHIP_LAPLACE, kRowsThreads>
-        <<<grid, kRowsThreads, lds + 8 * rows_tab_doubles(D, NB, true), s>>>(a);
+    p2_fused_rows_kernel<D, NB, Q, FEMHIP_LAPLACE, NT>
+        <<<grid, NT, lds + 8 * rows_tab_doubles(D, NB, true), s>>>(a);
   else
-    p2_fused_rows_kernel<D, NB, Q, FEMHIP_MASS, kRowsThreads>
-        <<<grid, kRowsThreads, lds + 8 * rows_tab_doubles(D, NB, false), s>>>(a);
+    p2_fused_rows_kernel<D, NB, Q, FEMHIP_MASS, NT>
+        <<<grid, NT, lds + 8 * rows_tab_doubles(D, NB, false), s>>>(a);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
+}
+
+template <int D, int NB, int Q>
+int launch_rows(const RowsArgs& a, int form, int block, size_t lds, unsigned grid,
+                hipStream_t s) {
+  return block == 1024 ? launch_rows_nt<D, NB, Q, 1024>(a, form, lds, grid, s)
+                       : launch_rows_nt<D, NB, Q, 512>(a, form, lds, grid, s);
 }
 
 }  // namespace
@@ -471,6 +478,8 @@ extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const 
                                         check->queue != nullptr && check->capacity > 0),
                  FEMHIP_EINVAL, "rows fused: the check needs a verdict, a zero mask and a queue");
   const femhip_fused_plan_info_t& I = plan->info;
+  FEMHIP_REQUIRE(I.block == 512 || I.block == 1024, FEMHIP_EINVAL,
+                 "rows fused: plan block must be 512 or 1024");
   FEMHIP_REQUIRE(I.idbits == 16 && I.offbits == 0 &&
                      I.rec_words == (sp->nbs * 16 + 31) / 32 + 1,
                  FEMHIP_EINVAL, "rows fused: not a rows plan of this element");
@@ -493,10 +502,10 @@ extern "C" int femhip_rows_fused_assemble(const femhip_fused_dev_t* plan, const 
       const unsigned grid =
           (unsigned)(mode == 0 || I.nparts < (int)kRepeatGrid ? I.nparts : kRepeatGrid);
       int rc = FEMHIP_EUNSUPPORTED;
-      if (tet && g->nq == 11) rc = launch_rows<3, 10, 11>(a, form, lds, grid, st);
-      else if (tet && g->nq == 4) rc = launch_rows<3, 10, 4>(a, form, lds, grid, st);
-      else if (tri && g->nq == 6) rc = launch_rows<2, 6, 6>(a, form, lds, grid, st);
-      else if (tri && g->nq == 3) rc = launch_rows<2, 6, 3>(a, form, lds, grid, st);
+      if (tet && g->nq == 11) rc = launch_rows<3, 10, 11>(a, form, I.block, lds, grid, st);
+      else if (tet && g->nq == 4) rc = launch_rows<3, 10, 4>(a, form, I.block, lds, grid, st);
+      else if (tri && g->nq == 6) rc = launch_rows<2, 6, 6>(a, form, I.block, lds, grid, st);
+      else if (tri && g->nq == 3) rc = launch_rows<2, 6, 3>(a, form, I.block, lds, grid, st);
       else set_error("rows fused: unsupported number of quadrature points");
       if (rc != FEMHIP_OK) return rc;
     }

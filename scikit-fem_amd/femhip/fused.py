@@ -208,10 +208,13 @@ class FusedPlan:
         return out
 
 
-# Rows plans (P2 simplices): 9,000 slots x 8 B of accumulators (72 KB of
-# LDS, two 512-thread workgroups per CU) measured faster than 16,000 (one
-# per CU, less halo) and 6,000 (three per CU, more halo) on 41^3 P2 tets
-ROWS_SLOT_CAP = int(os.environ.get('FEMHIP_ROWS_SLOT_CAP', 9000))
+# Rows plans (P2 simplices): 18,500 slots x 8 B of accumulators (148 KB of
+# LDS) in one 1024-thread workgroup per CU, measured on 41^3 P2 tets against
+# 16,000 / 12,000 / 9,000 / 6,000 slots with 1024 threads and 9,000 with 512
+ROWS_SLOT_CAP = int(os.environ.get('FEMHIP_ROWS_SLOT_CAP', 18500))
+ROWS_BLOCK = int(os.environ.get('FEMHIP_ROWS_BLOCK', 1024))
+ROWS_TABLE_DOUBLES = 700   # the form tables of P2 tetrahedra (Laplace): 10*10*6 + 10*10
+LDS_BYTES = 160 * 1024
 ROWS_NODE_CAP = int(os.environ.get('FEMHIP_ROWS_NODE_CAP', 16384))
 _ROWS_FORM = {'laplace': _lib.LAPLACE, 'mass': _lib.MASS, 'kappa_laplace': _lib.KAPPA_LAPLACE}
 ROW_RECORD_WORDS = 6   # (visit << 4) | DOF, then 2 x 10 seven-bit column offsets
@@ -249,11 +252,22 @@ class RowsFusedPlan:
         ix = np.ascontiguousarray(indices, dtype=np.int32)
         handle = ctypes.c_void_p()
         info = _lib.FusedInfo()
-        _lib.check(lib.femhip_rows_fused_plan_build(
-            ed.shape[0], pos.shape[0], ed.shape[1], N, ed.ctypes.data_as(ctypes.c_void_p),
-            pos.ctypes.data_as(ctypes.c_void_p), ip.ctypes.data_as(ctypes.c_void_p),
-            ix.ctypes.data_as(ctypes.c_void_p), int(slot_cap), int(node_cap), 512,
-            ctypes.byref(handle), ctypes.byref(info)))
+        # the slot cap bounds a partition's accumulators and row count
+        # together; the row tables and the form tables must fit too, so a
+        # plan that overflows the LDS is rebuilt with smaller partitions
+        for _ in range(8):
+            _lib.check(lib.femhip_rows_fused_plan_build(
+                ed.shape[0], pos.shape[0], ed.shape[1], N, ed.ctypes.data_as(ctypes.c_void_p),
+                pos.ctypes.data_as(ctypes.c_void_p), ip.ctypes.data_as(ctypes.c_void_p),
+                ix.ctypes.data_as(ctypes.c_void_p), int(slot_cap), int(node_cap), ROWS_BLOCK,
+                ctypes.byref(handle), ctypes.byref(info)))
+            if lib.femhip_rows_fused_lds_bytes(ctypes.byref(info)) + 8 * ROWS_TABLE_DOUBLES \
+                    <= LDS_BYTES:
+                break
+            lib.femhip_fused_plan_free(handle)
+            slot_cap = int(slot_cap * 0.85)
+        else:
+            raise ValueError("rows plan: no partition size fits the LDS")
         try:
             parts = np.empty(8 * info.nparts, np.int32)
             nodemap = np.empty(info.nlocal, np.int32)
