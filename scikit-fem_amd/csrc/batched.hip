@@ -16,7 +16,7 @@ namespace femhip {
 namespace {
 
 constexpr int kSlots = 64;   // slots per workgroup (one wave wide)
-constexpr int kGroups = 4;   // sample groups per workgroup
+constexpr int kGroups = 1;   // sample groups per workgroup (2, 4, 16: slower)
 constexpr int kTS = 16;      // samples per thread and pass
 
 __global__ void __launch_bounds__(kSlots * kGroups)
