@@ -13,7 +13,12 @@
 // |detDF| per point, from femhip_geometry) and the reference gradients.
 //
 // One workgroup (4 waves) per element at a time (grid-stride over
-// elements); each wave owns 6 of the 24 output tiles.  The six 27 x 27
+// elements, 16,384 workgroups by default); each wave owns 6 of the 24
+// output tiles.  A thread keeps the reference gradients of its table
+// entries in registers for the whole launch and loads the mapping of the
+// workgroup's next element while the current one is assembled; the kernel is
+// held to 128 registers, so that the four workgroups LDS allows per CU stay
+// resident.  The six 27 x 27
 // products share LDS with the operand tables they are computed from, and the
 // reference gradients are read through the cache, so a workgroup holds
 // 38.5 KB and four fit on a CU: while some assemble their local matrices,
@@ -33,7 +38,7 @@ constexpr int kThreads = 256;
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
 hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restrict__ W,
                             const double* __restrict__ dphi, int64_t E, double lam,
                             double mu, double* __restrict__ out) {
@@ -58,31 +63,68 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
   if (tid < kTail) s_buf[2 * kTab + tid] = 0.0;
 
 
+  // The table build gives thread t the (function a, point q) pairs
+  // i = t + 256 it, a = i / 28, q = i % 28, for every element: their
+  // reference gradients stay in registers for the whole launch.
+  constexpr int kPairs = kNB * kQP;
+  constexpr int kPairIt = (kPairs + kThreads - 1) / kThreads;
+  double dref[kPairIt][3];
+#pragma unroll
+  for (int it = 0; it < kPairIt; ++it) {
+    const int i = tid + it * kThreads;
+    const int a = i / kQP, q = i % kQP;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      dref[it][d] = (i < kPairs && q < kQ) ? dphi[(a * 3 + d) * kQ + q] : 0.0;
+  }
+  // the mapping of the workgroup's next element (invDF, then w_q |det J|),
+  // loaded while the current one is assembled and written
+  constexpr int kMapIt = (kQ * 10 + kThreads - 1) / kThreads;
+  double mnext[kMapIt];
+  auto load_map = [&](int64_t e) {
+#pragma unroll
+    for (int it = 0; it < kMapIt; ++it) {
+      const int i = tid + it * kThreads;
+      mnext[it] = 0.0;
+      if (i < kQ * 10 && e < E) {
+        const int q = i / 10, c = i % 10;
+        double v = G[(int64_t)c * EQ + e * kQ + q];
+        if (c == 9) v *= W[q];
+        mnext[it] = v;
+      }
+    }
+  };
+  load_map(blockIdx.x);
+
   for (int64_t e = blockIdx.x; e < E; e += gridDim.x) {
-    __syncthreads();
-    for (int i = tid; i < kQ * 10; i += kThreads) {
-      const int q = i / 10, c = i % 10;
-      double v = G[(int64_t)c * EQ + e * kQ + q];
-      if (c == 9) v *= W[q];
-      s_map[i] = v;
+    __syncthreads();   // the previous element's products have been written out
+#pragma unroll
+    for (int it = 0; it < kMapIt; ++it) {
+      const int i = tid + it * kThreads;
+      if (i < kQ * 10) s_map[i] = mnext[it];
     }
     __syncthreads();
+    load_map(e + gridDim.x);
     // global gradients g^a_k(q) = sum_i invDF[i][k] dphi[a][i][q], and the
     // zeros of the padded point q = 27
-    for (int i = tid; i < kTab; i += kThreads) {
-      const int k = i / (kNB * kQP);
-      const int a = (i / kQP) % kNB;
-      const int q = i % kQP;
-      double g = 0.0, gw = 0.0;
-      if (q < kQ) {
-        const double* m = s_map + q * 10;
-        g = m[0 * 3 + k] * dphi[(a * 3 + 0) * kQ + q] +
-            m[1 * 3 + k] * dphi[(a * 3 + 1) * kQ + q] +
-            m[2 * 3 + k] * dphi[(a * 3 + 2) * kQ + q];
-        gw = g * m[9];
+#pragma unroll
+    for (int it = 0; it < kPairIt; ++it) {
+      const int i = tid + it * kThreads;
+      if (i < kPairs) {
+        const int a = i / kQP, q = i % kQP;
+        const double* m = s_map + (q < kQ ? q : 0) * 10;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double g = 0.0, gw = 0.0;
+          if (q < kQ) {
+            g = m[0 * 3 + k] * dref[it][0] + m[1 * 3 + k] * dref[it][1] +
+                m[2 * 3 + k] * dref[it][2];
+            gw = g * m[9];
+          }
+          s_ga[k][a][q] = g;
+          s_gb[k][q][a] = gw;
+        }
       }
-      s_ga[k][a][q] = g;
-      s_gb[k][q][a] = gw;
     }
     __syncthreads();
     // 6 products x 4 tiles = 24 tiles, 6 per wave, kept in registers until
@@ -327,7 +369,7 @@ extern "C" int femhip_hex2_elasticity_mfma(const femhip_geom_t* g, const femhip_
                  FEMHIP_EUNSUPPORTED,
                  "hex2 mfma: ElementVector(ElementHex2) with 27 quadrature points only");
   if (g->nelems == 0) return FEMHIP_OK;
-  const int64_t grid = groups > 0 ? groups : (g->nelems < 4096 ? g->nelems : 4096);
+  const int64_t grid = groups > 0 ? groups : (g->nelems < 16384 ? g->nelems : 16384);
   hex2_elasticity_mfma_kernel<<<(unsigned)grid, kThreads, 0,
                                 reinterpret_cast<hipStream_t>(s)>>>(
       G, g->W, sp->dphi, g->nelems, lam, mu, out);
