@@ -347,7 +347,7 @@ typedef struct femhip_fused_state {
   uint64_t maxk, maxv;   /* (epoch << 11) | exponent field of the pass's largest
                             |contribution| / load entry (atomicMax)           */
   int32_t flags;         /* FEMHIP_FUSED_BAD | FEMHIP_FUSED_OVERFLOW           */
-  int32_t reserved2;
+  int32_t checked;       /* workgroups of the recheck that have finished      */
   int64_t nsusp;         /* suspects queued by the last first pass            */
   int64_t mismatch, zeros; /* recheck counters                                */
   uint32_t reserved3[20];

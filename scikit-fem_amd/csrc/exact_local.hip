@@ -272,8 +272,8 @@ constexpr int kCheckBlocks = 128;
 // einsum-ordered gradients, (kappa grad u) . grad v summed from 0 or
 // u * v, times dx, NumPy's pairwise sum over the points) compared with 0.0;
 // counts the zero entries and those whose zero-ness differs from the plan's
-// mask bit of contribution (j * NB + i) * E + e
-// (fused_recheck_verdict_kernel then forms the verdict).
+// mask bit of contribution (j * NB + i) * E + e; the last workgroup to
+// finish forms the verdict.
 template <int D, int Q, int FK, int NB = D + 1>
 __global__ void __launch_bounds__(kBlock) fused_recheck_kernel(RecheckArgs a) {
 #pragma clang fp contract(off)
@@ -331,15 +331,19 @@ __global__ void __launch_bounds__(kBlock) fused_recheck_kernel(RecheckArgs a) {
   }
   if (mism) atomicAdd(reinterpret_cast<unsigned long long*>(&st->mismatch), mism);
   if (zeros) atomicAdd(reinterpret_cast<unsigned long long*>(&st->zeros), zeros);
-}
-
-// after the recheck: the verdict, and the queue counters and flags cleared
-// for the next pass
-__global__ void fused_recheck_verdict_kernel(femhip_fused_state_t* st, int64_t nzeros,
-                                             int32_t* verdict) {
-  const int64_t M = st->mismatch, Z = st->zeros;
-  const int32_t flags = st->flags;
-  verdict[0] = (flags == 0 && M == 0 && Z == nzeros) ? 1 : 0;
+  // Every workgroup's thread 0 counts itself in st->checked after its adds
+  // (release); the one that completes the count (acquire: it then sees
+  // every workgroup's adds) forms the verdict and clears the counters and
+  // flags for the next pass.  The count returns to zero with them, so no
+  // fill is needed between calls.
+  const int32_t before = __hip_atomic_fetch_add(&st->checked, 1, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+  if (before + 1 != (int32_t)gridDim.x) return;
+  const int64_t M = __hip_atomic_load(&st->mismatch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int64_t Z = __hip_atomic_load(&st->zeros, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int32_t flags = __hip_atomic_load(&st->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int32_t* verdict = a.verdict;
+  verdict[0] = (flags == 0 && M == 0 && Z == a.nzeros) ? 1 : 0;
   verdict[1] = (int32_t)(M < INT32_MAX ? M : INT32_MAX);
   verdict[2] = (int32_t)(Z < INT32_MAX ? Z : INT32_MAX);
   verdict[3] = flags;
@@ -347,6 +351,7 @@ __global__ void fused_recheck_verdict_kernel(femhip_fused_state_t* st, int64_t n
   st->flags = 0;
   st->mismatch = 0;
   st->zeros = 0;
+  st->checked = 0;
 }
 
 // the mass form has no zero entries on valid elements: the verdict only
@@ -369,8 +374,6 @@ int recheck_form(const RecheckArgs& a, int form, hipStream_t s) {
     fused_recheck_kernel<D, Q, FEMHIP_LAPLACE><<<kCheckBlocks, kBlock, 0, s>>>(a);
   else
     fused_recheck_kernel<D, Q, FEMHIP_KAPPA_LAPLACE><<<kCheckBlocks, kBlock, 0, s>>>(a);
-  FEMHIP_CHECK_LAUNCH();
-  fused_recheck_verdict_kernel<<<1, 1, 0, s>>>(a.st, a.nzeros, a.verdict);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }
@@ -410,7 +413,7 @@ int fused_check_run(const femhip_fused_check_t* c, femhip_fused_state_t* st, int
 }
 
 // P2 simplices (rows plans): the suspects of the last first pass rechecked
-// against the plan's zero mask, then the verdict
+// against the plan's zero mask, and the verdict
 template <int D, int NB, int Q>
 int rows_recheck_form(const RecheckArgs& a, int form, hipStream_t s) {
   if (form == FEMHIP_LAPLACE)
@@ -419,8 +422,6 @@ int rows_recheck_form(const RecheckArgs& a, int form, hipStream_t s) {
     fused_recheck_kernel<D, Q, FEMHIP_KAPPA_LAPLACE, NB><<<kCheckBlocks, kBlock, 0, s>>>(a);
   else
     fused_recheck_kernel<D, Q, FEMHIP_MASS, NB><<<kCheckBlocks, kBlock, 0, s>>>(a);
-  FEMHIP_CHECK_LAUNCH();
-  fused_recheck_verdict_kernel<<<1, 1, 0, s>>>(a.st, a.nzeros, a.verdict);
   FEMHIP_CHECK_LAUNCH();
   return FEMHIP_OK;
 }

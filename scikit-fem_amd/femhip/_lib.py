@@ -72,7 +72,7 @@ class FusedState(ctypes.Structure):
                 ('sv_first', _i32), ('reserved0', _i32), ('epoch', ctypes.c_uint64),
                 ('reserved1', ctypes.c_uint32 * 24),
                 ('maxk', ctypes.c_uint64), ('maxv', ctypes.c_uint64),
-                ('flags', _i32), ('reserved2', _i32),
+                ('flags', _i32), ('checked', _i32),
                 ('nsusp', _i64), ('mismatch', _i64), ('zeros', _i64),
                 ('reserved3', ctypes.c_uint32 * 20)]
 
