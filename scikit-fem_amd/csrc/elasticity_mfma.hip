@@ -272,12 +272,43 @@ row_sum_pos_kernel(const int32_t* __restrict__ perm, const int32_t* __restrict__
     }
 }
 
+// U local rows of one global row added into the row's LDS slots: their
+// loads issued together, then the adds.  LDS adds without a return value:
+// no wait between the local rows; one wave's adds reach a slot in
+// instruction order (u), and no two lanes of one instruction share a slot,
+// so every slot is summed in the same order on every call.
+template <int U>
+__device__ __forceinline__ void add_local_rows(const double* __restrict__ loc,
+                                               const int16_t* __restrict__ pos,
+                                               const int32_t* __restrict__ lrs, int nb,
+                                               int lane, double* a) {
+  const int j0 = min(lane, nb - 1), j1 = min(lane + 64, nb - 1);
+  double v0[U], v1[U];
+  int p0[U], p1[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t base = (int64_t)lrs[u] * nb;
+    v0[u] = loc[base + j0];
+    p0[u] = pos[base + j0];
+    v1[u] = loc[base + j1];
+    p1[u] = pos[base + j1];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (lane < nb && p0[u] >= 0)
+      __hip_atomic_fetch_add(a + p0[u], v0[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (lane + 64 < nb && p1[u] >= 0)
+      __hip_atomic_fetch_add(a + p1[u], v1[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
 // One wave per global row r: the row's slots accumulate in LDS while the
 // wave walks the local rows of r (rowe[rowe_ptr[r] .. rowe_ptr[r+1]), in
 // increasing element order), each read as nb contiguous values with their
-// offsets; the entries of one local row go to distinct slots, so the lanes
-// add without atomics, and every slot is summed in the same order on every
-// call.  The row is then written once, coalesced.
+// offsets; the row is then written once, coalesced.  The row index is
+// uniform in the wave, so its bounds come from scalar loads and the local
+// rows go eight at a time through a body unrolled for exactly their count
+// (a row of the vector Q2 pattern has 1, 2, 4 or 8).
 constexpr int kRowWaves = 4;
 constexpr int kMaxRowLen = 384;
 __global__ void __launch_bounds__(64 * kRowWaves)
@@ -286,39 +317,24 @@ row_sum_em_kernel(const double* __restrict__ loc, const int16_t* __restrict__ po
                   const int32_t* __restrict__ indptr, int64_t nrows, int nb,
                   double* __restrict__ out) {
   __shared__ double acc[kRowWaves][kMaxRowLen];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t r = (int64_t)blockIdx.x * kRowWaves + w;
   if (r >= nrows) return;   // a whole wave: no barrier below
   double* a = acc[w];
   const int32_t k0 = indptr[r], len = indptr[r + 1] - k0;
   for (int q = lane; q < len; q += 64) a[q] = 0.0;
   const int32_t t0 = rowe_ptr[r], nt = rowe_ptr[r + 1] - t0;
-  // kLr local rows at a time, their loads issued together (an index past
-  // the row's last local row reloads that one; its values are not added)
-  constexpr int kLr = 8;
-  for (int c = 0; c < nt; c += kLr) {
-    double v0[kLr], v1[kLr];
-    int p0[kLr], p1[kLr];
-#pragma unroll
-    for (int u = 0; u < kLr; ++u) {
-      const int64_t base = (int64_t)rowe[t0 + min(c + u, nt - 1)] * nb;
-      const int j1 = min(lane + 64, nb - 1);
-      v0[u] = loc[base + min(lane, nb - 1)];
-      p0[u] = pos[base + min(lane, nb - 1)];
-      v1[u] = loc[base + j1];
-      p1[u] = pos[base + j1];
-    }
-    // LDS adds without a return value: no wait between the local rows; one
-    // wave's adds reach a slot in instruction order (u), and no two lanes of
-    // one instruction share a slot, so every slot is summed in the same
-    // order on every call
-#pragma unroll
-    for (int u = 0; u < kLr; ++u) {
-      if (c + u >= nt) break;
-      if (lane < nb && p0[u] >= 0)
-        __hip_atomic_fetch_add(a + p0[u], v0[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (lane + 64 < nb && p1[u] >= 0)
-        __hip_atomic_fetch_add(a + p1[u], v1[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  for (int c = 0; c < nt; c += 8) {
+    const int32_t* lrs = rowe + t0 + c;
+    switch (min(nt - c, 8)) {
+      case 1: add_local_rows<1>(loc, pos, lrs, nb, lane, a); break;
+      case 2: add_local_rows<2>(loc, pos, lrs, nb, lane, a); break;
+      case 3: add_local_rows<3>(loc, pos, lrs, nb, lane, a); break;
+      case 4: add_local_rows<4>(loc, pos, lrs, nb, lane, a); break;
+      case 5: add_local_rows<5>(loc, pos, lrs, nb, lane, a); break;
+      case 6: add_local_rows<6>(loc, pos, lrs, nb, lane, a); break;
+      case 7: add_local_rows<7>(loc, pos, lrs, nb, lane, a); break;
+      default: add_local_rows<8>(loc, pos, lrs, nb, lane, a); break;
     }
   }
   for (int q = lane; q < len; q += 64) out[k0 + q] = a[q];
