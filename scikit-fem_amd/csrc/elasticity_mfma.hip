@@ -13,7 +13,7 @@
 // |detDF| per point, from femhip_geometry) and the reference gradients.
 //
 // One workgroup (4 waves) per element at a time (grid-stride over
-// elements, 16,384 workgroups by default); each wave owns 6 of the 24
+// elements, 32,768 workgroups by default); each wave owns 6 of the 24
 // output tiles.  A thread keeps the reference gradients of its table
 // entries in registers for the whole launch and loads the mapping of the
 // workgroup's next element while the current one is assembled; the kernel is
@@ -385,7 +385,7 @@ extern "C" int femhip_hex2_elasticity_mfma(const femhip_geom_t* g, const femhip_
                  FEMHIP_EUNSUPPORTED,
                  "hex2 mfma: ElementVector(ElementHex2) with 27 quadrature points only");
   if (g->nelems == 0) return FEMHIP_OK;
-  const int64_t grid = groups > 0 ? groups : (g->nelems < 16384 ? g->nelems : 16384);
+  const int64_t grid = groups > 0 ? groups : (g->nelems < 32768 ? g->nelems : 32768);
   hex2_elasticity_mfma_kernel<<<(unsigned)grid, kThreads, 0,
                                 reinterpret_cast<hipStream_t>(s)>>>(
       G, g->W, sp->dphi, g->nelems, lam, mu, out);
