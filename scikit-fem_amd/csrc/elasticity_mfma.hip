@@ -21,7 +21,7 @@
 // resident.  The six 27 x 27
 // products share LDS with the operand tables they are computed from, and the
 // reference gradients are read through the cache, so a workgroup holds
-// 38.5 KB and four fit on a CU: while some assemble their local matrices,
+// 39.2 KB and four fit on a CU: while some assemble their local matrices,
 // others run their products.  The local matrix is written
 // element-major and row-major, out[e][i*81 + j] (i test, j trial),
 // contiguous per element.
@@ -48,19 +48,25 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
   // the buffer and its columns 27..31 of B the next row's first entries (or
   // the zeroed tail), which reach only the discarded rows / columns of the
   // product; the padded point q = 27 is an explicit zero in both tables.
-  constexpr int kTab = 3 * kNB * kQP;
+  // The rows of G_k^T are kQA = 29 doubles apart: with an odd stride the
+  // 16 rows that one group of 16 lanes fetches for an MFMA operand fall in
+  // distinct LDS banks (with 28 they met four to a bank).
+  constexpr int kQA = kQP + 1;
+  constexpr int kTabA = 3 * kNB * kQA;
+  constexpr int kTabB = 3 * kNB * kQP;
   constexpr int kTail = 8;
-  constexpr int kBuf = 2 * kTab + kTail;
+  constexpr int kBuf = kTabA + kTabB + kTail;
   static_assert(6 * kNB * kNB <= kBuf, "products fit the table memory");
+  static_assert((2 * kNB + 31) * kQA + kQP <= kBuf, "padded A rows stay in the buffer");
   __shared__ double s_buf[kBuf];
-  double(*s_ga)[kNB][kQP] = reinterpret_cast<double(*)[kNB][kQP]>(s_buf);  // G_k^T
+  double(*s_ga)[kNB][kQA] = reinterpret_cast<double(*)[kNB][kQA]>(s_buf);  // G_k^T
   double(*s_gb)[kQP][kNB] =
-      reinterpret_cast<double(*)[kQP][kNB]>(s_buf + kTab);  // diag(omega) G_l
+      reinterpret_cast<double(*)[kQP][kNB]>(s_buf + kTabA);  // diag(omega) G_l
   double(*s_S)[kNB][kNB] = reinterpret_cast<double(*)[kNB][kNB]>(s_buf);  // S_kl, k <= l
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int64_t EQ = E * kQ;
-  if (tid < kTail) s_buf[2 * kTab + tid] = 0.0;
+  if (tid < kTail) s_buf[kTabA + kTabB + tid] = 0.0;
 
 
   // The table build gives thread t the (function a, point q) pairs
@@ -140,8 +146,8 @@ hex2_elasticity_mfma_kernel(const double* __restrict__ G, const double* __restri
       acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int q0 = 0; q0 < kQP; q0 += 4) {
-        const double a = s_buf[(k * kNB + ti * 16 + r16) * kQP + q0 + k4];           // A[i][kk]
-        const double b = s_buf[kTab + (l * kQP + q0 + k4) * kNB + tj * 16 + r16];   // B[kk][j]
+        const double a = s_buf[(k * kNB + ti * 16 + r16) * kQA + q0 + k4];            // A[i][kk]
+        const double b = s_buf[kTabA + (l * kQP + q0 + k4) * kNB + tj * 16 + r16];   // B[kk][j]
         acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
       }
     }
